@@ -1,0 +1,423 @@
+"""Test helpers: problem construction from reference-style inputs and the
+ctypes binding of the CPU oracle (oracle/_build/libqs_oracle.so).
+
+Only tests (and bench.py's cpu_baseline leg, smoke()) use this module.
+
+The model builders restate, for test purposes only:
+  helix Hamiltonian    src/hamiltonian.f90:83-163 (helix_coordinate, soc_matrix_elements)
+  custom template      src/hamiltonian.f90:741-755
+  lead couplings       src/input_output.f90:310-346 (setup_leads, default atoms)
+  Hubbard U            src/input_output.f90:432-462 (setup_hubbard)
+  energy grid          src/quichotte.f90:81-93
+  beta, n_volt_steps   src/input_output.f90:123-126
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(REPO, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "libqs_oracle.so")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+KB = 8.6173303e-5  # common/constants.f90:19
+
+
+# --------------------------------------------------------------------------
+# input.dat
+# --------------------------------------------------------------------------
+def parse_input(path):
+    """key = value lines, '#' comments; Fortran reals (1.d-6) accepted."""
+    d = {}
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line or "=" not in line:
+                continue
+            k, v = line.split("=", 1)
+            d[k.strip().lower()] = v.strip()
+    return d
+
+
+def freal(s):
+    return float(s.lower().replace("d", "e"))
+
+
+def flogical(s):
+    s = s.strip().lower().strip(".")
+    return s in ("t", "true", "1", "y", "yes")
+
+
+def helix_hamiltonian(N_ions, N_turns, hel_radius, hel_length, hand, E_CC, t_hop, lamb):
+    nat = N_ions * N_turns
+    norb = 2 * nat
+    H = np.zeros((norb, norb), complex)
+
+    def coord(i):  # 1-based site
+        return np.array([
+            hel_radius * np.cos(hand * (i - 1) * 2 * np.pi / N_ions),
+            hel_radius * np.sin(hand * (i - 1) * 2 * np.pi / N_ions),
+            hel_length * (i - 1) / (N_turns * (N_ions - 1)),
+        ])
+
+    for i in range(1, nat + 1):
+        ii = 2 * i
+        H[ii - 2, ii - 2] = E_CC
+        H[ii - 1, ii - 1] = E_CC
+        if i <= nat - 1:
+            jj = 2 * (i + 1)
+            H[ii - 2, jj - 2] = t_hop
+            H[ii - 1, jj - 1] = t_hop
+        if i <= nat - 2:
+            jj = 2 * (i + 2)
+            r, r1, r2 = coord(i), coord(i + 1), coord(i + 2)
+            d1 = (r - r1) / np.linalg.norm(r - r1)
+            d2 = (r - r2) / np.linalg.norm(r - r2)
+            v = np.array([d1[1] * d2[2] - d1[2] * d2[1],
+                          d1[2] * d2[0] - d1[0] * d2[2],
+                          d1[0] * d2[1] - d2[0] * d1[1]])
+            H[ii - 2, jj - 2] = 1j * lamb * v[2]
+            H[ii - 2, jj - 1] = 1j * lamb * (v[0] - 1j * v[1])
+            H[ii - 1, jj - 2] = 1j * lamb * (v[0] + 1j * v[1])
+            H[ii - 1, jj - 1] = 1j * lamb * (-v[2])
+    for i in range(norb):
+        for j in range(i, norb):
+            H[j, i] = np.conj(H[i, j])
+    return H
+
+
+def custom_hamiltonian(N_sites, E_CC, t_hop):
+    norb = 2 * N_sites
+    H = np.zeros((norb, norb), complex)
+    for i in range(1, N_sites + 1):
+        ii = 2 * i
+        H[ii - 2, ii - 2] = E_CC
+        H[ii - 1, ii - 1] = E_CC
+        if i < N_sites:
+            H[ii - 2, ii] = t_hop
+            H[ii - 1, ii + 1] = t_hop
+    for i in range(norb):
+        for j in range(i, norb):
+            H[j, i] = np.conj(H[i, j])
+    return H
+
+
+def leads(norb, GammaL_up, GammaL_down, GammaR, left_sites=(1,), right_sites=None):
+    nsites = norb // 2
+    if right_sites is None:
+        right_sites = (nsites,)
+    GL = np.zeros((norb, norb), complex)
+    GR = np.zeros((norb, norb), complex)
+    for s in left_sites:
+        GL[2 * s - 2, 2 * s - 2] = GammaL_up
+        GL[2 * s - 1, 2 * s - 1] = GammaL_down
+    for s in right_sites:
+        GR[2 * s - 2, 2 * s - 2] = GammaR
+        GR[2 * s - 1, 2 * s - 1] = GammaR
+    return GL, GR
+
+
+class Problem:
+    """Everything the SCF path reads, built like the reference's setup."""
+
+    def __init__(self, H, GL, GR, U_site, order, T, mu, dw, up, delta, voltages,
+                 alpha_mix=0.5, nhist=16, maxscf=300, tolf=1e-6, tolf_order1=1e-8,
+                 single_history=True, continuation=True, n_omega=None, w_init=None):
+        self.H = np.asfortranarray(H, complex)
+        self.GL = np.asfortranarray(GL, complex)
+        self.GR = np.asfortranarray(GR, complex)
+        self.U_site = np.asarray(U_site, float)
+        self.norb = self.H.shape[0]
+        self.order = int(order)
+        self.T = T
+        self.beta = 1.0 / (KB * T)
+        self.mu = mu
+        self.delta = delta
+        if n_omega is None:
+            ev = np.linalg.eigvalsh(self.H)
+            w_init = ev[0] - dw
+            w_fin = ev[-1] + up
+            n_omega = int((w_fin - w_init) / delta)
+        self.w_init = w_init
+        self.n_omega = int(n_omega)
+        self.omega = w_init + delta * np.arange(1, self.n_omega + 1)
+        self.voltages = list(voltages)
+        self.alpha_mix = alpha_mix
+        self.nhist = nhist
+        self.maxscf = maxscf
+        self.tolf = tolf
+        self.tolf_order1 = tolf_order1
+        self.single_history = single_history
+        self.continuation = continuation
+
+    @staticmethod
+    def from_case(case_dir):
+        fn = os.path.join(case_dir, "input.dat")
+        if not os.path.exists(fn):
+            fn = os.path.join(case_dir, "inputSOC.dat")
+        d = parse_input(fn)
+        model = d.get("hamiltonian", "helix")
+        E_CC, t_hop = freal(d["e_cc"]), freal(d["t_hop"])
+        if model == "helix":
+            H = helix_hamiltonian(int(d["n_ions"]), int(d["n_turns"]), freal(d["hel_radius"]),
+                                  freal(d["hel_length"]), freal(d["hand"]), E_CC, t_hop, freal(d["lamb"]))
+        elif model == "custom":
+            H = custom_hamiltonian(int(d["n_sites"]), E_CC, t_hop)
+        else:
+            raise ValueError("model not covered by the test helper: " + model)
+        norb = H.shape[0]
+        GL, GR = leads(norb, freal(d["gammal_up"]), freal(d["gammal_down"]),
+                       freal(d.get("gammar", d.get("gammar_both"))))
+        U_int = freal(d.get("u_int", d.get("hubbard")))
+        U = np.zeros(norb)
+        if "hubbard_file" in d and d["hubbard_file"]:
+            vals = [freal(l.split()[0]) for l in open(os.path.join(case_dir, d["hubbard_file"])) if l.strip()]
+            U[0::2] = vals[: norb // 2]
+        else:
+            U[0::2] = U_int
+        V, Vf, delv = freal(d["v"]), freal(d["vf"]), freal(d["delv"])
+        nsteps = int(round((Vf - V) / delv))
+        volts = [V + k * delv for k in range(nsteps + 1)]
+        return Problem(H, GL, GR, U, int(d["order"]), freal(d["t"]), freal(d["mu"]), freal(d["dw"]),
+                       freal(d["up"]), freal(d["delta"]), volts,
+                       alpha_mix=freal(d.get("alpha_mix", d.get("pulay", "0.5"))),
+                       nhist=int(d.get("nhist", 16)), maxscf=int(d.get("maxscf", 300)),
+                       tolf=freal(d.get("tolf", "1e-6")), tolf_order1=freal(d.get("tolf_order1", "1e-8")),
+                       single_history=flogical(d.get("broyden_single_precision", "T")),
+                       continuation=flogical(d.get("continuation", "T")))
+
+
+def helix_problem(N_ions, N_turns, U, order=2, n_omega=256, wlo=-4.0, whi=4.0, hel_radius=1.4,
+                  hel_length=3.4, hand=1.0, E_CC=0.0, t_hop=-1.0, lamb=0.01, gamma=0.1, T=300.0, mu=0.0,
+                  voltages=(0.0,), **kw):
+    """Helix on an explicit grid of n_omega points on (wlo, whi]."""
+    H = helix_hamiltonian(N_ions, N_turns, hel_radius, hel_length, hand, E_CC, t_hop, lamb)
+    norb = H.shape[0]
+    GL, GR = leads(norb, gamma, gamma, gamma)
+    Us = np.zeros(norb)
+    Us[0::2] = U
+    delta = (whi - wlo) / n_omega
+    return Problem(H, GL, GR, Us, order, T, mu, 0, 0, delta, voltages, n_omega=n_omega, w_init=wlo, **kw)
+
+
+# --------------------------------------------------------------------------
+# oracle binding
+# --------------------------------------------------------------------------
+class _OrcProblem(ctypes.Structure):
+    _fields_ = [
+        ("norb", ctypes.c_int32), ("n_omega", ctypes.c_int32),
+        ("w_init", ctypes.c_double), ("delta", ctypes.c_double),
+        ("H", ctypes.c_void_p), ("gammaL", ctypes.c_void_p), ("gammaR", ctypes.c_void_p),
+        ("U_site", ctypes.c_void_p), ("order", ctypes.c_int32),
+        ("beta", ctypes.c_double), ("mu", ctypes.c_double),
+        ("n_atoms", ctypes.c_int32), ("atom_norb", ctypes.c_void_p),
+        ("alpha_mix", ctypes.c_double), ("nhist", ctypes.c_int32), ("maxscf", ctypes.c_int32),
+        ("tolf", ctypes.c_double), ("tolf_order1", ctypes.c_double),
+        ("single_history", ctypes.c_int32), ("nthreads", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def oracle_lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = ctypes.CDLL(ORACLE_SO)
+        P, D, I32, I64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [P]
+        L.orc_destroy.argtypes = [P]
+        L.orc_sizes.argtypes = [P] + [P] * 6
+        L.orc_maps.argtypes = [P, P, P, P]
+        L.orc_solve_scf.argtypes = [P, D, I32, P, P, P, P, P]
+        L.orc_current.argtypes = [P, P, P]
+        L.orc_g_less_obs.argtypes = [P, P]
+        L.orc_dosd.argtypes = [P, P]
+        L.orc_trUD.argtypes = [P, P, P]
+        L.orc_final_flags.argtypes = [P, P, P]
+        L.orc_init2.argtypes = [P, D, P]
+        L.orc_residual2.argtypes = [P, D, I64, P, P, P]
+        L.orc_final_sweep.argtypes = [P, D]
+        L.orc_get_gin.argtypes = [P, P, P]
+        L.orc_get_sigma2.argtypes = [P, P, P]
+        L.orc_get_sig1.argtypes = [P, P]
+        L.orc_get_diag.argtypes = [P, P, P]
+        L.orc_good_fft_size.argtypes = [ctypes.c_int]
+        L.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, P]
+        L.orc_invert.argtypes = [ctypes.c_int, P]
+        L.orc_sigma2.argtypes = [ctypes.c_int, ctypes.c_int, D, P, P, P, P, P, ctypes.c_int]
+        L.orc_pack_gf.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int]
+        L.orc_unpack_gf.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int]
+        L.orc_check_diag_sanity.argtypes = [P, P, ctypes.c_int, ctypes.c_int, D, P, P, P, P, P, P]
+        L.orc_fermi.restype = D
+        L.orc_fermi.argtypes = [D, D, D, D]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    def __init__(self, prob, nthreads=0):
+        self.prob = prob
+        L = oracle_lib()
+        self._keep = [prob.H, prob.GL, prob.GR, prob.U_site]
+        p = _OrcProblem()
+        p.norb = prob.norb
+        p.n_omega = prob.n_omega
+        p.w_init = prob.w_init
+        p.delta = prob.delta
+        p.H = ptr(prob.H)
+        p.gammaL = ptr(prob.GL)
+        p.gammaR = ptr(prob.GR)
+        p.U_site = ptr(prob.U_site)
+        p.order = prob.order
+        p.beta = prob.beta
+        p.mu = prob.mu
+        p.n_atoms = 0
+        p.atom_norb = None
+        p.alpha_mix = prob.alpha_mix
+        p.nhist = prob.nhist
+        p.maxscf = prob.maxscf
+        p.tolf = prob.tolf
+        p.tolf_order1 = prob.tolf_order1
+        p.single_history = int(bool(prob.single_history))
+        p.nthreads = nthreads
+        self.h = L.orc_create(ctypes.byref(p))
+        a = [ctypes.c_int32() for _ in range(4)]
+        nv = ctypes.c_int64()
+        nf = ctypes.c_int32()
+        L.orc_sizes(self.h, *[ctypes.byref(x) for x in a], ctypes.byref(nv), ctypes.byref(nf))
+        self.nA, self.nS, self.nCL, self.nAs = [x.value for x in a]
+        self.n_vec = nv.value
+        self.Nf = nf.value
+        self.L = L
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def solve(self, V, fresh):
+        conv, nit, fmax = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        lf = np.zeros(max(self.prob.maxscf, 1))
+        lr = np.zeros(max(self.prob.maxscf, 1))
+        rc = self.L.orc_solve_scf(self.h, V, int(fresh), ctypes.byref(conv), ctypes.byref(nit),
+                                  ctypes.byref(fmax), ptr(lf), ptr(lr))
+        if rc != 0:
+            raise RuntimeError("oracle solve_scf failed: %d" % rc)
+        return dict(converged=bool(conv.value), n_iter=nit.value, fmax=fmax.value,
+                    log_fmax=lf[: nit.value].copy(), log_rms=lr[: nit.value].copy())
+
+    def current(self):
+        u, d = ctypes.c_double(), ctypes.c_double()
+        self.L.orc_current(self.h, ctypes.byref(u), ctypes.byref(d))
+        return u.value, d.value
+
+    def g_less_obs(self):
+        G = np.zeros((self.prob.norb, self.prob.norb), complex, order="F")
+        self.L.orc_g_less_obs(self.h, ptr(G))
+        return G
+
+    def dosd(self):
+        d = np.zeros((self.prob.norb, self.prob.n_omega), order="F")
+        self.L.orc_dosd(self.h, ptr(d))
+        return d
+
+    def trUD(self):
+        u = np.zeros(self.prob.n_omega)
+        d = np.zeros(self.prob.n_omega)
+        self.L.orc_trUD(self.h, ptr(u), ptr(d))
+        return u, d
+
+    def init2(self, V):
+        x = np.zeros(self.n_vec)
+        rc = self.L.orc_init2(self.h, V, ptr(x))
+        assert rc == 0
+        return x
+
+    def residual2(self, V, x):
+        F = np.zeros(self.n_vec)
+        ab = ctypes.c_int32()
+        rc = self.L.orc_residual2(self.h, V, self.n_vec, ptr(np.ascontiguousarray(x)), ptr(F), ctypes.byref(ab))
+        assert rc == 0
+        return F, bool(ab.value)
+
+    def final_sweep(self, V):
+        assert self.L.orc_final_sweep(self.h, V) == 0
+
+    def sigma2(self):
+        n = self.nA * self.nA * self.prob.n_omega
+        r = np.zeros(n, complex)
+        l = np.zeros(n, complex)
+        self.L.orc_get_sigma2(self.h, ptr(r), ptr(l))
+        return r, l
+
+    def sig1(self):
+        S = np.zeros((self.prob.norb, self.prob.norb), complex, order="F")
+        self.L.orc_get_sig1(self.h, ptr(S))
+        return S
+
+    def diag(self):
+        r = np.zeros(self.nA * self.prob.n_omega, complex)
+        l = np.zeros(self.nA * self.prob.n_omega, complex)
+        self.L.orc_get_diag(self.h, ptr(r), ptr(l))
+        return r, l
+
+
+def spin_texture_rows(V, G):
+    """Rows of populations.dat (src/observables.f90:41-86): V, site, Sx, Sy, Sz, n_dn, n_up, n_tot."""
+    norb = G.shape[0]
+    px = np.array([[0, 1], [1, 0]], complex)
+    py = np.array([[0, -1j], [1j, 0]], complex)
+    pz = np.array([[1, 0], [0, -1]], complex)
+    pref = -0.5j
+    rows = []
+    for i in range(0, norb, 2):
+        tr = [0j, 0j, 0j]
+        for s1 in range(2):
+            for s2 in range(2):
+                g = G[i + s2, i + s1]
+                tr[0] += px[s1, s2] * g
+                tr[1] += py[s1, s2] * g
+                tr[2] += pz[s1, s2] * g
+        up = -1j * G[i, i]
+        dn = -1j * G[i + 1, i + 1]
+        rows.append([V, i // 2 + 1, (tr[0] * pref).real, (tr[1] * pref).real, (tr[2] * pref).real,
+                     dn.real, up.real, (up + dn).real])
+    return rows
+
+
+def read_table(path):
+    rows = []
+    for line in open(path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        rows.append([float(v.replace("D", "E").replace("d", "e")) for v in t])
+    return rows
+
+
+def assert_close_table(new, ref, tol_abs=1e-6, tol_rel=1e-5):
+    """Field-by-field comparison as tests/run_tests.f90:178-233 does."""
+    assert len(new) == len(ref), (len(new), len(ref))
+    worst = 0.0
+    for a, b in zip(new, ref):
+        assert len(a) == len(b)
+        for x, y in zip(a, b):
+            err = abs(x - y)
+            lim = tol_abs + tol_rel * abs(y)
+            worst = max(worst, err / lim)
+    assert worst <= 1.0, "worst |new-ref|/tol = %g" % worst
+    return worst
