@@ -1,0 +1,115 @@
+/*
+ * qs_hip.h -- C ABI of the MI355X (gfx950) implementation of QUICHOTTE's
+ * self-consistent NEGF path (orders 0, 1 and 2; order 2 is the hot path).
+ *
+ * Plain pointers and sizes only.  Host arrays use the reference's conventions:
+ * complex numbers are interleaved (re, im) doubles, matrices are column-major
+ * (Fortran order), orbital indices are 0-based here where the reference's are
+ * 1-based, packed vectors use pack_gf's layout (common/linear_algebra.f90:142-191).
+ *
+ * Errors: every entry point returns QS_OK (0) or a nonzero status;
+ * qs_last_error() gives the message.  QS_ERR_SINGULAR is the reference's
+ * `stop 1` on a failed factorisation (common/linear_algebra.f90:73-76,84-87)
+ * and is fatal for the run.  A runaway SCF is not an error: it is reported in
+ * the result (src/scf.f90:180-186).
+ *
+ * Threading: one handle per request.  Handles share no mutable state, so
+ * different handles may be used from different host threads.  A handle itself
+ * must not be used by two threads at once.
+ */
+#ifndef QS_HIP_H
+#define QS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QS_OK 0
+#define QS_ERR_SINGULAR 1
+#define QS_ERR_ARG 2
+#define QS_ERR_HIP 3
+#define QS_ERR_NOMEM 4
+#define QS_ERR_UNSUPPORTED 5
+#define QS_ERR_STATE 6
+
+/* Everything solve_scf reads from module state (src/green_functions.f90:38-67,
+ * src/parameters.f90:7-90, src/scf.f90:38-43).  Copied at qs_create. */
+typedef struct qs_problem {
+    int32_t norb;             /* spin-orbitals = 2 N_sites; basis 2i = (site i, up), 2i+1 = (site i, down) */
+    int32_t n_omega;          /* energy points: omega_i = w_init + i*delta, i = 1..n_omega (src/quichotte.f90:85-93) */
+    double w_init, delta;
+    const double *H;          /* (norb,norb) complex, column-major */
+    const double *gammaL;     /* (norb,norb) complex: left lead coupling (src/input_output.f90:310-346) */
+    const double *gammaR;     /* (norb,norb) complex */
+    const double *U_site;     /* (norb): Hubbard U on the spin-up index of each site (src/input_output.f90:432-462) */
+    int32_t order;            /* 0, 1 or 2 */
+    double beta, mu;          /* 1/(kB T), chemical potential */
+    int32_t n_atoms;          /* atoms; 0 => one orbital per atom */
+    const int32_t *atom_norb; /* (n_atoms) spatial orbitals of each atom, or NULL */
+    double alpha_mix;         /* Broyden settings (src/scf.f90:38-43) */
+    int32_t nhist, maxscf;
+    double tolf, tolf_order1;
+    int32_t single_history;   /* broyden_single_precision */
+    int32_t device;           /* HIP device ordinal */
+} qs_problem;
+
+typedef struct qs_scf_result {
+    int32_t converged;        /* Broyden converged and the final sweep is physical (src/scf.f90:142-150) */
+    int32_t n_iter;           /* residual evaluations */
+    double fmax;              /* max|F| of the last residual */
+    int32_t runaway;          /* the SCF was aborted (src/scf.f90:180-186) */
+    int32_t unphysical;       /* final_sanity (src/scf.f90:260-275) */
+    int32_t i_worst;          /* worst state (1-based), its occupation and spectral weight */
+    double n_worst, w_worst;
+} qs_scf_result;
+
+typedef struct qs_handle qs_handle;
+
+/* setup_storage + scf_setup_sizes + sigma2_init (src/green_functions.f90:90-190,
+ * src/scf.f90:61-68, src/self_energies.f90:91-106): active subspace, device
+ * storage and FFT plan.  *out is NULL on failure. */
+int qs_create(const qs_problem *p, qs_handle **out);
+/* sigma2_finalize (src/self_energies.f90:108-113) and every device allocation */
+void qs_destroy(qs_handle *h);
+const char *qs_last_error(const qs_handle *h);
+
+/* sizes of the active subspace and of the packed vector (n_vec, src/scf.f90:61-68);
+ * nf is the padded FFT length used on the device (a power of two >= 2 n_omega - 1) */
+int qs_sizes(const qs_handle *h, int32_t *nA, int32_t *nS, int32_t *nCL, int32_t *nAs, int64_t *n_vec,
+             int32_t *nf);
+/* bytes of device memory held by the handle */
+int64_t qs_device_bytes(const qs_handle *h);
+
+/* solve_scf (src/scf.f90:78-151): converge at voltage V (fresh start from the
+ * non-interacting solution, or continue from the last input), then the final
+ * sweep.  log_fmax / log_rms (length >= maxscf, may be NULL) receive max|F|
+ * and rms|F| per step, the lines of scf_V_<V>.dat (common/broyden_mixing.f90:92-95). */
+int qs_solve_scf(qs_handle *h, double V, int32_t fresh, qs_scf_result *res, double *log_fmax, double *log_rms);
+
+/* observables of the last final sweep */
+int qs_current(const qs_handle *h, double *J_up, double *J_down);    /* Current, src/observables.f90:22-39 */
+int qs_g_less_obs(const qs_handle *h, double *G);                    /* G_less_obs (norb,norb) complex */
+int qs_dosd(const qs_handle *h, double *dosd);                       /* dosd (norb, n_omega) real */
+int qs_trUD(const qs_handle *h, double *trU, double *trD);           /* current integrand (n_omega) each */
+
+/* --- seams inside the SCF step (order 2) ------------------------------------ */
+/* sweep(M_INIT2) + pack_gf (src/scf.f90:116-122): the non-interacting active
+ * G^r, G^< packed into the handle's x; copied to x_out (n_vec) if not NULL */
+int qs_init2(qs_handle *h, double V, double *x_out);
+/* scf_residual2 (src/scf.f90:158-188), the residual_function contract of
+ * common/broyden_mixing.f90:39-47: F = pack(G_out[unpack(x)]) - x, or F = 0 and
+ * *abort = 1 on a runaway.  x and F are host arrays of length n (= n_vec). */
+int qs_residual2(qs_handle *h, double V, int64_t n, const double *x, double *F, int32_t *abort_flag);
+/* sigma2_compute (src/self_energies.f90:135-219) of the input of the last
+ * residual: SigR2, SigL2 as (nA,nA,n_omega) complex column-major, and Sigma1 as
+ * (norb,norb) complex (src/self_energies.f90:45-72); any pointer may be NULL */
+int qs_get_sigma(const qs_handle *h, double *SigR2, double *SigL2, double *Sig1);
+/* sweep(M_FINAL) + gather_onsite + gather_atoms + final_sanity with the
+ * self-energies of the last residual (src/scf.f90:134-140) */
+int qs_final_sweep(qs_handle *h, double V, qs_scf_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

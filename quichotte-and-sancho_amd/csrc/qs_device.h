@@ -1,0 +1,84 @@
+// qs_device.h -- device helpers shared by the gfx950 kernels of the SCF path.
+//
+// Complex numbers are double2 (x = real, y = imaginary) in FP64.  Wavefronts
+// are 64 lanes wide; block reductions assume blockDim.x is a multiple of 64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define QS_WAVE 64
+
+struct cplx { double x, y; };
+
+__device__ __forceinline__ double2 cmk(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// acc += a * b
+__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+// acc -= a * b
+__device__ __forceinline__ void cfms(double2 &acc, double2 a, double2 b) {
+    acc.x = fma(-a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(-a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+// acc += a * conj(b)
+__device__ __forceinline__ void cfmac(double2 &acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(-a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ double2 cinv(double2 a) {
+    double d = a.x * a.x + a.y * a.y;
+    return make_double2(a.x / d, -a.y / d);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = QS_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, QS_WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = QS_WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, QS_WAVE));
+    return v;
+}
+
+// Block-wide sum; every thread gets the result.  scratch: >= 16 doubles of LDS.
+__device__ __forceinline__ double block_sum(double v, double *scratch) {
+    v = wave_sum(v);
+    int w = threadIdx.x / QS_WAVE, nw = blockDim.x / QS_WAVE;
+    __syncthreads();
+    if ((threadIdx.x & (QS_WAVE - 1)) == 0) scratch[w] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < nw; i++) s += scratch[i];
+    return s;
+}
+__device__ __forceinline__ double block_max(double v, double *scratch) {
+    v = wave_max(v);
+    int w = threadIdx.x / QS_WAVE, nw = blockDim.x / QS_WAVE;
+    __syncthreads();
+    if ((threadIdx.x & (QS_WAVE - 1)) == 0) scratch[w] = v;
+    __syncthreads();
+    double s = scratch[0];
+    for (int i = 1; i < nw; i++) s = fmax(s, scratch[i]);
+    return s;
+}
+
+// Fermi-Dirac as the reference evaluates it (src/green_functions.f90:74-84):
+// f = 1/(exp(min(hbar*(w - mu + V/hbar)*beta, 700)) + 1), hbar = 1.
+__device__ __host__ __forceinline__ double qs_fermi(double w, double V, double mu, double beta) {
+    double e = (w - mu + V) * beta;
+    if (e > 700.0) e = 700.0;
+    return 1.0 / (exp(e) + 1.0);
+}

@@ -1,0 +1,444 @@
+// qs_dyson.hip -- per-energy Dyson and Keldysh solve on gfx950 (FP64).
+//
+// Replaces G_at_energy + invert_complex (src/scf.f90:305-415,
+// common/linear_algebra.f90:58-88) for every energy of the grid at once:
+//
+//   G^r = [ w + Hemb - Sigma1 - Sigma2^r(w) ]^-1                  (Dyson)
+//   G^< = G^r(:,sup) [ i(f_L GamL + f_R GamR) + Sigma2^<(w) ] G^r(:,sup)^dagger   (Keldysh)
+//
+// One workgroup of 256 threads (4 wavefronts) per energy.  The threads form a
+// 16 x 16 grid; thread (ti, tj) keeps the R x R elements (ti + 16a, tj + 16b) of
+// the matrix in registers ("cyclic" distribution, n <= 16 R).
+//
+// Inverse: in-place Gauss-Jordan with partial row pivoting on |Re| + |Im|, the
+// pivot rule of LAPACK's zgetf2.  Rows are never moved: step k picks the unused
+// physical row p_k with the largest candidate, so the registers end with M where
+//   G^r[q_r][p_c] = M[r][c],  q = p^-1.
+// The candidates at step k equal LU's column k, so the pivot sequence and the
+// stability are those of zgetrf.
+//
+// Keldysh in the pivoted basis: with S''[c][c'] = SL[p_c][p_c'] (rows/columns
+// outside sup are zero), P = M S'' M^dagger gives G^<[q_r][q_r'] = P[r][r'].
+//   phase 1: T = M S''   M staged in LDS, S'' streamed in 16-row chunks,
+//   phase 2: P = T M^dag T staged in LDS, M^dag chunks written from registers.
+// Both products are register-tiled complex FMA loops (FP64 vector FMA and FP64
+// MFMA have the same peak on MI355X, 78.6 TFLOP/s).
+//
+// The epilogue writes, by mode:
+//   INIT2  packed non-interacting G^r, G^< of the active block into x
+//   SCF2   F = pack(G^r, G^<) - x, Im diagonals, and partial max|F|, |F|^2, |F-Fold|^2
+//   SCF1   Im diagonals of the active states, on-site 2x2 blocks of G^<
+//   FINAL  Im G^r_ii, on-site and intra-atom blocks of G^<, current integrand
+// Packed layout per energy (common/linear_algebra.f90:142-191):
+//   [Re G^r row-major | Im G^r row-major | Re G^<(i<j) row-wise | Im G^<(i<=j) row-wise]
+#include "qs_device.h"
+#include "qs_internal.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+
+__device__ __forceinline__ int sig_block_index(int a, int b, int nAs) {
+    return ((((a >> 1) * nAs + (b >> 1)) * 2) << 2) + ((a & 1) << 1) + (b & 1);
+}
+
+struct LdsLayout {
+    double2 *big;   // n x (n+1)
+    double2 *chunk; // 16 x 16R
+    int *piv, *qinv, *spc;
+    double *redv;
+    int *redi;
+    double *scratch;
+    double2 *grcl;
+};
+
+__device__ __forceinline__ LdsLayout lds_layout(char *base, int n, int R) {
+    LdsLayout L;
+    L.big = (double2 *)base;
+    L.chunk = L.big + (size_t)n * (n + 1);
+    L.piv = (int *)(L.chunk + 16 * 16 * R);
+    L.qinv = L.piv + 16 * R;
+    L.spc = L.qinv + 16 * R;
+    L.redv = (double *)(L.spc + 16 * R + ((3 * 16 * R) & 1));  // keep 8-byte alignment
+    L.redi = (int *)(L.redv + 16);
+    L.scratch = (double *)(L.redi + 16);
+    L.grcl = (double2 *)(L.scratch + 16);
+    return L;
+}
+
+// SL(b1, b2) on the support: i (f_L GamL + f_R GamR) + Sigma2^<  (src/scf.f90:333-336)
+__device__ __forceinline__ double2 sl_elem(const DysonArgs &A, int b1, int b2, double fL, double fR,
+                                           const double2 *sigw) {
+    double2 gl = A.gamL_s[b1 * A.nS + b2], gr = A.gamR_s[b1 * A.nS + b2];
+    double re = fL * gl.x + fR * gr.x, im = fL * gl.y + fR * gr.y;
+    double2 v = make_double2(-im, re);
+    if (sigw) v = cadd(v, sigw[sig_block_index(b1, b2, A.nAs) + 4]);
+    return v;
+}
+
+template <int R>
+__global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
+    extern __shared__ __align__(16) char lds_raw[];
+    const int n = A.norb, pitch = n + 1;
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int iw = A.iw0 + blockIdx.x;
+    LdsLayout L = lds_layout(lds_raw, n, R);
+    const double w = A.w_init + (double)(iw + 1) * A.delta;
+    const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
+    const int nA = A.nA, nA2 = nA * nA;
+    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
+
+    // ---- 1. A = w + Hemb - Sigma1 - Sigma2^r, in registers -------------------------
+    double2 M[R][R];
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) {
+            const int i = ti + 16 * a, j = tj + 16 * b;
+            double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
+            if (i < n && j < n) {
+                v = A.Hemb[i * n + j];
+                if (i == j) v.x += w;
+                if (A.use_sigma) {
+                    if ((i >> 1) == (j >> 1) && A.sig1) v = csub(v, A.sig1[(i >> 1) * 4 + (i & 1) * 2 + (j & 1)]);
+                    if (sigw) {
+                        const int pa = A.pos[i], pb = A.pos[j];
+                        if (pa >= 0 && pb >= 0) v = csub(v, sigw[sig_block_index(pa, pb, A.nAs)]);
+                    }
+                }
+            }
+            M[a][b] = v;
+        }
+    unsigned used = 0;  // bit a: my row ti + 16a already pivoted
+#pragma unroll
+    for (int a = 0; a < R; a++)
+        if (ti + 16 * a >= n) used |= 1u << a;
+
+    double2 *prow = L.chunk, *pcol = L.chunk + 16 * R;
+
+    // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
+    for (int k = 0; k < n; k++) {
+        const int bk = k >> 4, tk = k & 15;
+        if (tj == tk) {
+            double best = -1.0;
+            int bi = -1;
+#pragma unroll
+            for (int a = 0; a < R; a++)
+#pragma unroll
+                for (int b = 0; b < R; b++)
+                    if (b == bk && !(used & (1u << a))) {
+                        double v = fabs(M[a][b].x) + fabs(M[a][b].y);
+                        if (v != v) v = -0.5;  // NaN never wins over a number
+                        if (bi < 0 || v > best) { best = v; bi = ti + 16 * a; }
+                    }
+            L.redv[ti] = best;
+            L.redi[ti] = bi;
+        }
+        __syncthreads();
+        double best = -2.0;
+        int p = -1;
+        for (int t = 0; t < 16; t++) {
+            const int ii = L.redi[t];
+            const double v = L.redv[t];
+            if (ii >= 0 && (p < 0 || v > best || (v == best && ii < p))) { best = v; p = ii; }
+        }
+        const int ap = p >> 4, tp = p & 15;
+        if (ti == tp) {
+#pragma unroll
+            for (int a = 0; a < R; a++)
+                if (a == ap)
+#pragma unroll
+                    for (int b = 0; b < R; b++) prow[tj + 16 * b] = M[a][b];
+        }
+        if (tj == tk) {
+#pragma unroll
+            for (int b = 0; b < R; b++)
+                if (b == bk)
+#pragma unroll
+                    for (int a = 0; a < R; a++) pcol[ti + 16 * a] = M[a][b];
+        }
+        if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
+        __syncthreads();
+        const double2 pv = pcol[p];
+        if (tid == 0 && pv.x == 0.0 && pv.y == 0.0) atomicOr(A.status, 1);
+        const double2 rinv = cinv(pv);
+        double2 sp[R], fr[R];
+#pragma unroll
+        for (int b = 0; b < R; b++) sp[b] = cmul(prow[tj + 16 * b], rinv);
+#pragma unroll
+        for (int a = 0; a < R; a++) fr[a] = pcol[ti + 16 * a];
+#pragma unroll
+        for (int a = 0; a < R; a++)
+#pragma unroll
+            for (int b = 0; b < R; b++) cfms(M[a][b], fr[a], sp[b]);
+        if (tj == tk) {
+#pragma unroll
+            for (int a = 0; a < R; a++)
+#pragma unroll
+                for (int b = 0; b < R; b++)
+                    if (b == bk) {
+                        double2 t = cmul(fr[a], rinv);
+                        M[a][b] = make_double2(-t.x, -t.y);
+                    }
+        }
+        if (ti == tp) {
+#pragma unroll
+            for (int a = 0; a < R; a++)
+                if (a == ap) {
+                    used |= 1u << a;
+#pragma unroll
+                    for (int b = 0; b < R; b++) M[a][b] = (b == bk && tj == tk) ? rinv : sp[b];
+                }
+        }
+    }
+    __syncthreads();
+
+    // ---- 3. G^r = permuted M staged in LDS; mode extraction -----------------------
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) {
+            const int r = ti + 16 * a, c = tj + 16 * b;
+            if (r < n && c < n) L.big[L.qinv[r] * pitch + L.piv[c]] = M[a][b];
+        }
+    for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
+    __syncthreads();
+
+    const size_t nblk = 3 * (size_t)nA2, off = (size_t)iw * nblk;
+    double st_max = 0.0, st_s2 = 0.0, st_d2 = 0.0;
+    if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
+        for (int e = tid; e < nA2; e += NT) {
+            const int a = e / nA, b = e - a * nA;
+            const double2 g = L.big[A.act[a] * pitch + A.act[b]];
+            if (A.mode == QS_M_INIT2) {
+                A.xout[off + e] = g.x;
+                A.xout[off + nA2 + e] = g.y;
+            } else {
+                const size_t d0 = off + e, d1 = off + nA2 + e;
+                const double f0 = g.x - A.x[d0], f1 = g.y - A.x[d1];
+                A.F[d0] = f0;
+                A.F[d1] = f1;
+                st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
+                st_s2 += f0 * f0 + f1 * f1;
+                const double e0 = f0 - A.Fold[d0], e1 = f1 - A.Fold[d1];
+                st_d2 += e0 * e0 + e1 * e1;
+                if (a == b) A.diagR[(size_t)iw * nA + a] = g.y;
+            }
+        }
+    } else if (A.mode == QS_M_SCF1) {
+        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = L.big[A.act[k2] * pitch + A.act[k2]].y;
+    } else {
+        for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = L.big[i * pitch + i].y;
+        for (int e = tid; e < A.nCL * A.nCL; e += NT) {
+            const int k2 = e / A.nCL, a = e - k2 * A.nCL;
+            L.grcl[e] = L.big[A.cl[k2] * pitch + A.cl[a]];
+        }
+    }
+    __syncthreads();
+
+    // ---- 4. phase 1: T = M S'' --------------------------------------------------
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) {
+            const int r = ti + 16 * a, c = tj + 16 * b;
+            if (r < n && c < n) L.big[r * pitch + c] = M[a][b];
+        }
+    double2 acc[R][R];
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) acc[a][b] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        __syncthreads();
+        for (int e = tid; e < 16 * n; e += NT) {
+            const int kk = e / n, c = e - kk * n, k = k0 + kk;
+            double2 v = make_double2(0.0, 0.0);
+            if (k < n) {
+                const int b1 = L.spc[k], b2 = L.spc[c];
+                if (b1 >= 0 && b2 >= 0) v = sl_elem(A, b1, b2, fL, fR, sigw);
+            }
+            L.chunk[kk * n + c] = v;
+        }
+        __syncthreads();
+        const int kmax = min(16, n - k0);
+        for (int kk = 0; kk < kmax; kk++) {
+            double2 av[R], bv[R];
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                const int r = ti + 16 * a;
+                av[a] = r < n ? L.big[r * pitch + k0 + kk] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int b = 0; b < R; b++) {
+                const int c = tj + 16 * b;
+                bv[b] = c < n ? L.chunk[kk * n + c] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int a = 0; a < R; a++)
+#pragma unroll
+                for (int b = 0; b < R; b++) cfma(acc[a][b], av[a], bv[b]);
+        }
+    }
+    __syncthreads();
+    // ---- 5. phase 2: P = T M^dagger ---------------------------------------------
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) {
+            const int r = ti + 16 * a, c = tj + 16 * b;
+            if (r < n && c < n) L.big[r * pitch + c] = acc[a][b];
+            acc[a][b] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+    for (int bb = 0; bb < R; bb++) {
+        const int k0 = 16 * bb;
+        if (k0 >= n) break;
+        __syncthreads();
+        // chunk[kk][j] = conj(M[j][k0 + kk]); thread (ti, tj) owns column k0 + tj
+#pragma unroll
+        for (int a = 0; a < R; a++) {
+            const int j = ti + 16 * a;
+            if (j < n) L.chunk[tj * n + j] = cconj(M[a][bb]);
+        }
+        __syncthreads();
+        const int kmax = min(16, n - k0);
+        for (int kk = 0; kk < kmax; kk++) {
+            double2 av[R], bv[R];
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                const int r = ti + 16 * a;
+                av[a] = r < n ? L.big[r * pitch + k0 + kk] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int b = 0; b < R; b++) {
+                const int c = tj + 16 * b;
+                bv[b] = c < n ? L.chunk[kk * n + c] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int a = 0; a < R; a++)
+#pragma unroll
+                for (int b = 0; b < R; b++) cfma(acc[a][b], av[a], bv[b]);
+        }
+    }
+    __syncthreads();
+    // ---- 6. G^< = permuted P staged in LDS; mode extraction ------------------------
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) {
+            const int r = ti + 16 * a, c = tj + 16 * b;
+            if (r < n && c < n) L.big[L.qinv[r] * pitch + L.qinv[c]] = acc[a][b];
+        }
+    __syncthreads();
+
+    if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
+        const int nsu = (nA2 - nA) / 2;
+        for (int e = tid; e < nA2; e += NT) {
+            const int code = A.tri_map[e], i = code >> 16, j = code & 0xffff;
+            const double2 g = L.big[A.act[i] * pitch + A.act[j]];
+            const double val = e < nsu ? g.x : g.y;
+            const size_t d = off + 2 * (size_t)nA2 + e;
+            if (A.mode == QS_M_INIT2) {
+                A.xout[d] = val;
+            } else {
+                const double f = val - A.x[d];
+                A.F[d] = f;
+                st_max = fmax(st_max, fabs(f));
+                st_s2 += f * f;
+                const double e0 = f - A.Fold[d];
+                st_d2 += e0 * e0;
+                if (e >= nsu && i == j) A.diagL[(size_t)iw * nA + i] = g.y;
+            }
+        }
+        if (A.mode == QS_M_SCF2) {
+            const double m = block_max(st_max, L.scratch);
+            const double s2 = block_sum(st_s2, L.scratch);
+            const double d2 = block_sum(st_d2, L.scratch);
+            if (tid == 0) {
+                double *pp = A.partials + (size_t)blockIdx.x * 4;
+                pp[0] = m; pp[1] = s2; pp[2] = d2; pp[3] = 0.0;
+            }
+        }
+    } else {
+        if (A.mode == QS_M_SCF1)
+            for (int k2 = tid; k2 < nA; k2 += NT) A.diagL[(size_t)iw * nA + k2] = L.big[A.act[k2] * pitch + A.act[k2]].y;
+        for (int e = tid; e < 4 * A.nsites; e += NT) {
+            const int si = e >> 2, a = e & 1, b = (e >> 1) & 1;
+            A.onsL[(size_t)iw * 4 * A.nsites + e] = L.big[(2 * si + a) * pitch + 2 * si + b];
+        }
+        if (A.mode == QS_M_FINAL) {
+            if (A.atomL) {
+                for (int e = tid; e < A.atom_tot; e += NT) {
+                    int at = 0;
+                    while (at + 1 < A.n_atoms && A.atom_off[at + 1] <= e) at++;
+                    const int m = 2 * A.atom_norb[at], loc = e - A.atom_off[at];
+                    const int b = loc / m, a = loc - b * m, lo = 2 * A.atom_first[at];
+                    A.atomL[(size_t)iw * A.atom_tot + e] = L.big[(lo + a) * pitch + lo + b];
+                }
+            }
+            if (tid == 0) {
+                // Tr_s{ i GamL [ (f_L - 1) G^< - f_L G^> ] } / 2pi on the left contact  (src/scf.f90:390-409)
+                const int nc = A.nCL;
+                double tu = 0.0, td = 0.0;
+                for (int a = 0; a < nc; a++) {
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int l = 0; l < nc; l++) {
+                        const double2 glc = L.big[A.cl[l] * pitch + A.cl[a]];
+                        const double2 gr_la = L.grcl[l * nc + a], gr_al = L.grcl[a * nc + l];
+                        const double2 gg = cadd(glc, csub(gr_la, cconj(gr_al)));
+                        const double2 X = make_double2((fL - 1.0) * glc.x - fL * gg.x, (fL - 1.0) * glc.y - fL * gg.y);
+                        cfma(s, A.gamL_cl[a * nc + l], X);
+                    }
+                    // Mc(a,a) = i s
+                    if ((A.cl[a] & 1) == 0) tu += -s.y; else td += -s.y;
+                }
+                A.trUD[(size_t)iw * 2 + 0] = tu / TWO_PI;
+                A.trUD[(size_t)iw * 2 + 1] = td / TWO_PI;
+            }
+        }
+    }
+}
+
+template <int R>
+hipError_t launch_r(const DysonArgs &a, int count, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_dyson<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    const size_t lds = qs_dyson_lds_bytes(a.norb, a.nCL);
+    hipLaunchKernelGGL(k_dyson<R>, dim3(count), dim3(NT), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t qs_dyson_lds_bytes(int n, int nCL) {
+    const int R = (n + 15) / 16;
+    size_t b = (size_t)n * (n + 1) * 16 + (size_t)16 * 16 * R * 16;
+    b += (size_t)(3 * 16 * R + 1) * 4 + 16 * 8 + 16 * 4 + 16 * 8;
+    b = (b + 15) & ~(size_t)15;
+    b += (size_t)nCL * nCL * 16;
+    return b;
+}
+
+int qs_dyson_max_norb() { return 88; }
+
+hipError_t qs_launch_dyson(const DysonArgs &a, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int R = (a.norb + 15) / 16;
+    if (qs_dyson_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
+    switch (R) {
+        case 1: return launch_r<1>(a, count, s);
+        case 2: return launch_r<2>(a, count, s);
+        case 3: return launch_r<3>(a, count, s);
+        case 4: return launch_r<4>(a, count, s);
+        case 5: return launch_r<5>(a, count, s);
+        case 6: return launch_r<6>(a, count, s);
+        default: return hipErrorInvalidValue;
+    }
+}

@@ -1,0 +1,1102 @@
+// qs_engine.cpp -- the SCF engine behind include/qs_hip.h.
+//
+// Host side of one bias-point solve.  State that the reference keeps in module
+// variables (src/green_functions.f90:38-67, src/self_energies.f90:23-26,
+// src/scf.f90:46-54) lives in a handle; every array that grows with the energy
+// grid lives in HBM.  Per SCF step only scalars cross PCIe: max|F|, |F|^2,
+// |F - Fold|^2, the 2 nA sanity sums and 2 nh Broyden dot products.
+//
+//   solve_scf          src/scf.f90:78-151
+//   scf_residual2      src/scf.f90:158-188   (device: Sigma1, Sigma2, Dyson/Keldysh)
+//   scf_residual1      src/scf.f90:194-224   (device sweep, host vector of 8/site)
+//   broyden_solve      common/broyden_mixing.f90:51-184 (host scalars, device vectors)
+//   final_sanity       src/scf.f90:260-275
+//   check_diag_sanity  common/scf_checks.f90:59-91
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qs_hip.h"
+#include "qs_internal.h"
+
+namespace {
+
+using zd = std::complex<double>;
+constexpr double PI = 3.14159265358979323846;
+constexpr int MAXH_DEV = 32;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct qs_handle {
+    qs_problem p{};
+    int norb = 0, nw = 0, nsites = 0, order = 0;
+    int nA = 0, nS = 0, nCL = 0, nAs = 0, n_int = 0;
+    int Nf = 0, N1 = 0, N2 = 0, log2N1 = 0, log2N2 = 0;
+    int64_t n_vec = 0;
+    int n_atoms = 0, atom_tot = 0;
+    bool has_atomL = false;
+    std::vector<int> act, pos, sup, suppos, cl, asite, atom_norb, atom_first, atom_off;
+    std::vector<char> site_int;
+    std::vector<double> U;
+    std::vector<zd> H, GamL, GamR, Hemb, GamL_s, GamR_s, GamL_cl;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t dev_bytes = 0;
+    std::vector<void *> allocs;
+
+    // device constants
+    double2 *d_Hemb = nullptr, *d_gamL_s = nullptr, *d_gamR_s = nullptr, *d_gamL_cl = nullptr;
+    int *d_act = nullptr, *d_cl = nullptr, *d_pos = nullptr, *d_suppos = nullptr, *d_tri = nullptr;
+    int *d_blk_src = nullptr, *d_et_src = nullptr, *d_act_site = nullptr;
+    int *d_atom_first = nullptr, *d_atom_norb = nullptr, *d_atom_off = nullptr;
+    double *d_U_act = nullptr, *d_Upair = nullptr;
+    double2 *d_tw = nullptr;
+    // device state
+    double *d_x = nullptr, *d_F = nullptr, *d_Fold = nullptr, *d_xold = nullptr;
+    void *d_hist_dF = nullptr, *d_hist_U = nullptr;
+    double2 *d_Gt = nullptr, *d_Z = nullptr;  // Gt doubles as the Sigma2 output
+    double2 *d_sig1 = nullptr, *d_gt0 = nullptr;
+    double *d_partials = nullptr, *d_red = nullptr, *d_scratch = nullptr;
+    double *d_diagR = nullptr, *d_diagL = nullptr, *d_dosd = nullptr, *d_trUD = nullptr;
+    double2 *d_onsL = nullptr, *d_atomL = nullptr;
+    int *d_status = nullptr;
+    double *d_sums = nullptr;  // column sums
+    size_t scratch_doubles = 0;
+    double *h_pin = nullptr;   // pinned scalars
+
+    // host state
+    bool x_valid = false;      // d_x holds the input of the last residual / a fresh start
+    bool sigma_valid = false;
+    double V_scf = 0.0;
+    std::vector<zd> G_less_t0, Sig1, G_less_obs;
+    std::vector<double> trU_sum;  // [2]
+    double J_up = 0, J_down = 0;
+    int runaway_final = 0, unphys_final = 0, iworst_final = 1;
+    double nworst_final = 0.5, wworst_final = 1.0;
+    bool final_valid = false;
+};
+
+namespace {
+
+int fail(qs_handle *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) return fail((h), QS_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T>
+int dalloc(qs_handle *h, T **ptr, size_t count) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(h, QS_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    h->allocs.push_back(p);
+    h->dev_bytes += (int64_t)bytes;
+    *ptr = (T *)p;
+    return QS_OK;
+}
+
+template <class T>
+int upload(qs_handle *h, T **ptr, const std::vector<T> &v) {
+    int rc = dalloc(h, ptr, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(h, hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return QS_OK;
+}
+
+int upload_z(qs_handle *h, double2 **ptr, const std::vector<zd> &v) {
+    int rc = dalloc(h, ptr, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(h, hipMemcpy(*ptr, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return QS_OK;
+}
+
+inline int su_idx(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
+inline int u_idx(int i, int j, int n) { return i * n - i * (i - 1) / 2 + (j - i); }
+
+// active subspace exactly as setup_storage (src/green_functions.f90:90-190)
+void setup_storage(qs_handle *h) {
+    const int n = h->norb, ns = h->nsites;
+    h->site_int.assign(ns, 0);
+    std::vector<char> site_act(ns, 0), conL(n, 0), conR(n, 0);
+    h->n_int = 0;
+    for (int s = 0; s < ns; s++) {
+        h->site_int[s] = (h->order >= 1 && h->U[2 * s] != 0.0);
+        h->n_int += h->site_int[s];
+        site_act[s] = h->site_int[s];
+    }
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < n; k++) {
+            if (h->GamL[(size_t)k * n + i] != 0.0 || h->GamL[(size_t)i * n + k] != 0.0) conL[i] = 1;
+            if (h->GamR[(size_t)k * n + i] != 0.0 || h->GamR[(size_t)i * n + k] != 0.0) conR[i] = 1;
+        }
+        if (conL[i] || conR[i]) site_act[i / 2] = 1;
+    }
+    h->act.clear();
+    h->pos.assign(n, -1);
+    for (int i = 0; i < n; i++)
+        if (site_act[i / 2]) { h->pos[i] = (int)h->act.size(); h->act.push_back(i); }
+    h->nA = (int)h->act.size();
+    h->asite.clear();
+    for (int s = 0; s < ns; s++)
+        if (site_act[s]) h->asite.push_back(s);
+    h->nAs = (int)h->asite.size();
+    h->cl.clear();
+    for (int i = 0; i < n; i++)
+        if (conL[i]) h->cl.push_back(i);
+    h->nCL = (int)h->cl.size();
+    h->sup.clear();
+    if (h->order == 2) h->sup = h->act;
+    else
+        for (int i = 0; i < n; i++)
+            if (conL[i] || conR[i]) h->sup.push_back(i);
+    h->nS = (int)h->sup.size();
+    h->suppos.assign(n, -1);
+    for (int b = 0; b < h->nS; b++) h->suppos[h->sup[b]] = b;
+    const int nS = h->nS, nCL = h->nCL;
+    h->GamL_s.assign((size_t)nS * nS, 0.0);
+    h->GamR_s.assign((size_t)nS * nS, 0.0);
+    h->GamL_cl.assign((size_t)nCL * nCL, 0.0);
+    // row-major on the device
+    for (int a = 0; a < nS; a++)
+        for (int b = 0; b < nS; b++) {
+            h->GamL_s[(size_t)a * nS + b] = h->GamL[(size_t)h->sup[b] * n + h->sup[a]];
+            h->GamR_s[(size_t)a * nS + b] = h->GamR[(size_t)h->sup[b] * n + h->sup[a]];
+        }
+    for (int a = 0; a < nCL; a++)
+        for (int b = 0; b < nCL; b++) h->GamL_cl[(size_t)a * nCL + b] = h->GamL[(size_t)h->cl[b] * n + h->cl[a]];
+    // Hemb = -H + (i/2hbar)(GammaL + GammaR), row-major (src/scf.f90:92)
+    h->Hemb.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            const size_t cm = (size_t)j * n + i;
+            h->Hemb[(size_t)i * n + j] = -h->H[cm] + zd(0.0, 0.5) * (h->GamL[cm] + h->GamR[cm]);
+        }
+    if (h->order == 1) h->n_vec = 8 * (int64_t)h->n_int;
+    else if (h->order == 2) h->n_vec = 3 * (int64_t)h->nA * h->nA * h->nw;
+    else h->n_vec = 0;
+}
+
+int setup_device(qs_handle *h) {
+    int rc;
+    const int n = h->norb, nA = h->nA, nA2 = nA * nA, nw = h->nw;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamCreate(&h->stream));
+    HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 4096 * sizeof(double), hipHostMallocDefault));
+    if ((rc = upload_z(h, &h->d_Hemb, h->Hemb))) return rc;
+    if ((rc = upload_z(h, &h->d_gamL_s, h->GamL_s))) return rc;
+    if ((rc = upload_z(h, &h->d_gamR_s, h->GamR_s))) return rc;
+    if ((rc = upload_z(h, &h->d_gamL_cl, h->GamL_cl))) return rc;
+    if ((rc = upload(h, &h->d_act, h->act))) return rc;
+    if ((rc = upload(h, &h->d_cl, h->cl))) return rc;
+    if ((rc = upload(h, &h->d_pos, h->pos))) return rc;
+    if ((rc = upload(h, &h->d_suppos, h->suppos))) return rc;
+    if ((rc = upload(h, &h->d_atom_first, h->atom_first))) return rc;
+    if ((rc = upload(h, &h->d_atom_norb, h->atom_norb))) return rc;
+    if ((rc = upload(h, &h->d_atom_off, h->atom_off))) return rc;
+    if ((rc = dalloc(h, &h->d_status, 1))) return rc;
+    HIPCHK(h, hipMemset(h->d_status, 0, sizeof(int)));
+    if ((rc = dalloc(h, &h->d_sig1, (size_t)4 * h->nsites))) return rc;
+    HIPCHK(h, hipMemset(h->d_sig1, 0, sizeof(double2) * 4 * h->nsites));
+    if ((rc = dalloc(h, &h->d_onsL, (size_t)nw * 4 * h->nsites))) return rc;
+    if ((rc = dalloc(h, &h->d_dosd, (size_t)nw * n))) return rc;
+    if ((rc = dalloc(h, &h->d_trUD, (size_t)nw * 2))) return rc;
+    if (h->has_atomL && (rc = dalloc(h, &h->d_atomL, (size_t)nw * h->atom_tot))) return rc;
+    if ((rc = dalloc(h, &h->d_diagR, (size_t)nw * std::max(nA, 1)))) return rc;
+    if ((rc = dalloc(h, &h->d_diagL, (size_t)nw * std::max(nA, 1)))) return rc;
+    if ((rc = dalloc(h, &h->d_partials, (size_t)nw * 4))) return rc;
+    if ((rc = dalloc(h, &h->d_red, 64))) return rc;
+    // column-sum scratch: 64 row-chunks x the widest column set
+    int widest = std::max({8 * h->nsites, n, 2 * std::max(nA, 1), 2 * std::max(h->atom_tot, 1), 2});
+    size_t need = (size_t)64 * widest;
+    size_t hist_part = (size_t)qs_vec_blocks(std::max<int64_t>(h->n_vec, 1)) * 2 * MAXH_DEV;
+    size_t et_part = (size_t)256 * 4 * std::max(h->nAs, 1);
+    h->scratch_doubles = std::max({need, hist_part + 64 * 2 * MAXH_DEV, et_part, (size_t)1024});
+    if ((rc = dalloc(h, &h->d_scratch, h->scratch_doubles))) return rc;
+    if ((rc = dalloc(h, &h->d_sums, (size_t)std::max(widest, 64)))) return rc;
+
+    if (h->order == 2) {
+        // packed-vector index tables
+        const int nsu = (nA2 - nA) / 2;
+        std::vector<int> tri(nA2);
+        int e = 0;
+        for (int i = 0; i < nA; i++)
+            for (int j = i + 1; j < nA; j++) tri[e++] = (i << 16) | j;
+        for (int i = 0; i < nA; i++)
+            for (int j = i; j < nA; j++) tri[e++] = (i << 16) | j;
+        if ((rc = upload(h, &h->d_tri, tri))) return rc;
+        std::vector<int> blk(2 * (size_t)2 * nA2);
+        for (int ia = 0; ia < h->nAs; ia++)
+            for (int ja = 0; ja < h->nAs; ja++)
+                for (int kind = 0; kind < 2; kind++)
+                    for (int sa = 0; sa < 2; sa++)
+                        for (int sb = 0; sb < 2; sb++) {
+                            const int c = ((ia * h->nAs + ja) * 2 + kind) * 4 + sa * 2 + sb;
+                            const int i = 2 * ia + sa, j = 2 * ja + sb;
+                            int re, im;
+                            if (kind == 0) {
+                                re = i * nA + j + 1;
+                                im = nA2 + i * nA + j;
+                            } else if (i < j) {
+                                re = 2 * nA2 + su_idx(i, j, nA) + 1;
+                                im = 2 * nA2 + nsu + u_idx(i, j, nA);
+                            } else if (i == j) {
+                                re = 0;
+                                im = 2 * nA2 + nsu + u_idx(i, i, nA);
+                            } else {
+                                re = -(2 * nA2 + su_idx(j, i, nA) + 1);
+                                im = 2 * nA2 + nsu + u_idx(j, i, nA);
+                            }
+                            blk[2 * (size_t)c] = re;
+                            blk[2 * (size_t)c + 1] = im;
+                        }
+        if ((rc = upload(h, &h->d_blk_src, blk))) return rc;
+        std::vector<int> et(4 * (size_t)h->nAs), asite_v(h->nAs);
+        std::vector<double> uact(h->nAs);
+        for (int ia = 0; ia < h->nAs; ia++) {
+            const int i = 2 * ia;
+            et[4 * ia + 0] = 2 * nA2 + nsu + u_idx(i, i, nA);
+            et[4 * ia + 1] = 2 * nA2 + nsu + u_idx(i + 1, i + 1, nA);
+            et[4 * ia + 2] = 2 * nA2 + su_idx(i, i + 1, nA);
+            et[4 * ia + 3] = 2 * nA2 + nsu + u_idx(i, i + 1, nA);
+            asite_v[ia] = h->asite[ia];
+            uact[ia] = h->U[2 * h->asite[ia]];
+        }
+        if ((rc = upload(h, &h->d_et_src, et))) return rc;
+        if ((rc = upload(h, &h->d_act_site, asite_v))) return rc;
+        if ((rc = upload(h, &h->d_U_act, uact))) return rc;
+        const double pp = h->p.delta / (2.0 * PI);
+        std::vector<double> upair((size_t)h->nAs * h->nAs);
+        for (int ia = 0; ia < h->nAs; ia++)
+            for (int ja = 0; ja < h->nAs; ja++) upair[(size_t)ia * h->nAs + ja] = uact[ia] * uact[ja] * pp * pp / h->Nf;
+        if ((rc = upload(h, &h->d_Upair, upair))) return rc;
+        std::vector<zd> tw(h->Nf);
+        for (int k = 0; k < h->Nf; k++) {
+            long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)h->Nf;
+            tw[k] = zd((double)cosl(a), (double)-sinl(a));
+        }
+        if ((rc = upload_z(h, &h->d_tw, tw))) return rc;
+        const size_t nv = (size_t)h->n_vec;
+        if ((rc = dalloc(h, &h->d_x, nv))) return rc;
+        if ((rc = dalloc(h, &h->d_F, nv))) return rc;
+        if ((rc = dalloc(h, &h->d_Fold, nv))) return rc;
+        if ((rc = dalloc(h, &h->d_xold, nv))) return rc;
+        HIPCHK(h, hipMemset(h->d_Fold, 0, nv * sizeof(double)));
+        HIPCHK(h, hipMemset(h->d_xold, 0, nv * sizeof(double)));
+        const int nhist = h->p.nhist;
+        if (nhist > 0) {
+            if (h->p.single_history) {
+                float *a, *b;
+                if ((rc = dalloc(h, &a, nv * nhist))) return rc;
+                if ((rc = dalloc(h, &b, nv * nhist))) return rc;
+                h->d_hist_dF = a;
+                h->d_hist_U = b;
+            } else {
+                double *a, *b;
+                if ((rc = dalloc(h, &a, nv * nhist))) return rc;
+                if ((rc = dalloc(h, &b, nv * nhist))) return rc;
+                h->d_hist_dF = a;
+                h->d_hist_U = b;
+            }
+        }
+        if ((rc = dalloc(h, &h->d_Gt, (size_t)nw * 2 * nA2))) return rc;
+        if ((rc = dalloc(h, &h->d_Z, (size_t)h->Nf * 2 * nA2))) return rc;
+        if ((rc = dalloc(h, &h->d_gt0, (size_t)4 * h->nAs))) return rc;
+    }
+    HIPCHK(h, hipDeviceSynchronize());
+    return QS_OK;
+}
+
+DysonArgs dyson_args(qs_handle *h, int mode, bool use_sigma, double V) {
+    DysonArgs a{};
+    a.norb = h->norb; a.nA = h->nA; a.nS = h->nS; a.nCL = h->nCL; a.nsites = h->nsites; a.nAs = h->nAs;
+    a.n_atoms = h->n_atoms; a.atom_tot = h->atom_tot;
+    a.nw = h->nw; a.iw0 = 0;
+    a.w_init = h->p.w_init; a.delta = h->p.delta;
+    a.mu = h->p.mu; a.beta = h->p.beta; a.V = V;
+    a.mode = mode; a.use_sigma = use_sigma ? 1 : 0; a.order = h->order;
+    a.Hemb = h->d_Hemb;
+    a.sig1 = h->d_sig1;
+    a.sig = (h->order == 2 && use_sigma) ? h->d_Gt : nullptr;
+    a.gamL_s = h->d_gamL_s; a.gamR_s = h->d_gamR_s; a.gamL_cl = h->d_gamL_cl;
+    a.act = h->d_act; a.cl = h->d_cl; a.pos = h->d_pos; a.suppos = h->d_suppos;
+    a.tri_map = h->d_tri;
+    a.atom_first = h->d_atom_first; a.atom_norb = h->d_atom_norb; a.atom_off = h->d_atom_off;
+    a.x = h->d_x; a.xout = h->d_x; a.F = h->d_F; a.Fold = h->d_Fold;
+    a.partials = h->d_partials;
+    a.diagR = h->d_diagR; a.diagL = h->d_diagL;
+    a.onsL = h->d_onsL; a.dosd = h->d_dosd; a.atomL = h->d_atomL; a.trUD = h->d_trUD;
+    a.status = h->d_status;
+    return a;
+}
+
+int run_dyson(qs_handle *h, int mode, bool use_sigma, double V) {
+    DysonArgs a = dyson_args(h, mode, use_sigma, V);
+    HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(int), h->stream));
+    HIPCHK(h, qs_launch_dyson(a, h->nw, h->stream));
+    return QS_OK;
+}
+
+int check_status(qs_handle *h) {
+    int st = 0;
+    HIPCHK(h, hipMemcpyAsync(&st, h->d_status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (st & 1) return fail(h, QS_ERR_SINGULAR, "singular matrix in the Dyson inverse (zgetrf INFO != 0)");
+    return QS_OK;
+}
+
+// column sums over the energy grid of a [nw][ncols] array -> host vector
+int colsum(qs_handle *h, const double *in, int ncols, std::vector<double> &out) {
+    HIPCHK(h, qs_launch_reduce_cols(in, ncols, h->nw, h->d_sums, h->d_scratch, h->stream));
+    out.resize(ncols);
+    HIPCHK(h, hipMemcpyAsync(out.data(), h->d_sums, ncols * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QS_OK;
+}
+
+struct Sanity {
+    int runaway = 0, unphysical = 0, i_worst = 1;
+    double n_worst = 0.5, w_worst = 1.0;
+};
+
+// check_diag_sanity (common/scf_checks.f90:59-91) from the sums over omega
+Sanity diag_sanity(const double *sumImGL, const double *sumImGR, int nb, double delta, const int *label) {
+    Sanity s;
+    bool ra = false, un = false;
+    for (int i = 0; i < nb; i++) {
+        const int lab = label ? label[i] + 1 : i + 1;
+        const double nocc = (delta / (2.0 * PI)) * sumImGL[i];
+        const double wsum = -(delta / PI) * sumImGR[i];
+        if (!std::isfinite(nocc) || !std::isfinite(wsum)) {
+            s.runaway = 1; s.unphysical = un ? 1 : 0; s.i_worst = lab; s.n_worst = nocc; s.w_worst = wsum;
+            return s;
+        }
+        if (nocc < -1.0 || nocc > 2.0) ra = true;
+        if (nocc < -0.05 || nocc > 1.05 || wsum < 0.5) un = true;
+        if (std::fabs(nocc - 0.5) > std::fabs(s.n_worst - 0.5) || (wsum < s.w_worst && std::fabs(s.n_worst - 0.5) <= 0.55)) {
+            s.i_worst = lab; s.n_worst = nocc; s.w_worst = wsum;
+        }
+    }
+    s.runaway = ra;
+    s.unphysical = (ra || un);
+    return s;
+}
+
+// scf_residual2 on the device: d_x -> d_F; stats[3] = max|F|, |F|^2, |F-Fold|^2
+int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, Sanity *san) {
+    const int nA = h->nA;
+    h->V_scf = V;
+    const double pp = h->p.delta / (2.0 * PI);
+    HIPCHK(h, qs_launch_equal_time_sigma1(h->d_x, h->d_et_src, h->d_act_site, h->d_U_act, h->d_gt0, h->d_sig1, nA,
+                                          h->nAs, h->nw, pp, h->d_scratch, h->stream));
+    HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, nA, h->nw, h->stream));
+    Sigma2Plan sp{};
+    sp.nA = nA; sp.nAs = h->nAs; sp.nw = h->nw; sp.Nf = h->Nf; sp.N1 = h->N1; sp.N2 = h->N2;
+    sp.log2N1 = h->log2N1; sp.log2N2 = h->log2N2; sp.delta = h->p.delta; sp.tw = h->d_tw; sp.Upair = h->d_Upair;
+    HIPCHK(h, qs_launch_sigma2(sp, h->d_Gt, h->d_Z, h->d_Gt, h->stream));
+    h->sigma_valid = true;
+    int rc = run_dyson(h, QS_M_SCF2, true, V);
+    if (rc) return rc;
+    HIPCHK(h, qs_launch_reduce_partials(h->d_partials, h->nw, h->d_red, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_pin, h->d_red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if ((rc = check_status(h))) return rc;
+    stats[0] = h->h_pin[0]; stats[1] = h->h_pin[1]; stats[2] = h->h_pin[2];
+    std::vector<double> sR, sL;
+    if ((rc = colsum(h, h->d_diagR, nA, sR))) return rc;
+    if ((rc = colsum(h, h->d_diagL, nA, sL))) return rc;
+    *san = diag_sanity(sL.data(), sR.data(), nA, h->p.delta, h->act.data());
+    *abort_flag = san->runaway;
+    return QS_OK;
+}
+
+// host Broyden on small vectors (order 1), common/broyden_mixing.f90:51-184
+template <class Fn>
+void broyden_host(Fn residual, std::vector<double> &x, double alpha, int nhist, int maxscf, double tolf, bool sp,
+                  int *converged, int *n_iter, double *fmax_out, double *lf, double *lr) {
+    const int64_t n = (int64_t)x.size();
+    const double w0 = 0.01;
+    const int lda = std::max(nhist, 1);
+    std::vector<double> F(n), Fold(n, 0.0), xold(n, 0.0), a((size_t)lda * lda), cvec(lda), gram((size_t)lda * lda, 0.0);
+    std::vector<float> dFs, Us;
+    std::vector<double> dF, U;
+    if (sp) { dFs.assign((size_t)n * lda, 0.f); Us.assign((size_t)n * lda, 0.f); }
+    else { dF.assign((size_t)n * lda, 0.0); U.assign((size_t)n * lda, 0.0); }
+    *converged = 0;
+    int nstored = 0, nh = 0;
+    double fmax = 1.79769313486231570e308;
+    for (int its = 1; its <= maxscf; its++) {
+        *n_iter = its;
+        int ab = 0;
+        residual(x, F, &ab);
+        if (ab) break;
+        fmax = 0.0;
+        double ss = 0.0;
+        for (int64_t i = 0; i < n; i++) { fmax = std::max(fmax, std::fabs(F[i])); ss += F[i] * F[i]; }
+        if (lf) lf[its - 1] = fmax;
+        if (lr) lr[its - 1] = std::sqrt(ss) / std::sqrt((double)n);
+        if (fmax < tolf) { *converged = 1; break; }
+        if (its > 1 && nhist > 0) {
+            const int slot = nstored % nhist;
+            if (sp) {
+                for (int64_t i = 0; i < n; i++) Fold[i] = F[i] - Fold[i];
+                double fn = 0.0;
+                for (int64_t i = 0; i < n; i++) fn += Fold[i] * Fold[i];
+                fn = std::sqrt(fn);
+                if (fn > 0.0) {
+                    const double rf = 1.0 / fn;
+                    for (int64_t i = 0; i < n; i++) {
+                        dFs[(size_t)slot * n + i] = (float)(Fold[i] * rf);
+                        Us[(size_t)slot * n + i] = (float)(alpha * Fold[i] * rf + (x[i] - xold[i]) * rf);
+                    }
+                    nstored++;
+                    nh = std::min(nstored, nhist);
+                    for (int k = 0; k < nh; k++) {
+                        double s = 0.0;
+                        for (int64_t i = 0; i < n; i++) s += (double)dFs[(size_t)k * n + i] * (double)dFs[(size_t)slot * n + i];
+                        gram[(size_t)slot * lda + k] = s;
+                        gram[(size_t)k * lda + slot] = s;
+                    }
+                }
+            } else {
+                double fn = 0.0;
+                for (int64_t i = 0; i < n; i++) {
+                    dF[(size_t)slot * n + i] = F[i] - Fold[i];
+                    fn += dF[(size_t)slot * n + i] * dF[(size_t)slot * n + i];
+                }
+                fn = std::sqrt(fn);
+                if (fn > 0.0) {
+                    for (int64_t i = 0; i < n; i++) {
+                        dF[(size_t)slot * n + i] /= fn;
+                        U[(size_t)slot * n + i] = alpha * dF[(size_t)slot * n + i] + (x[i] - xold[i]) / fn;
+                    }
+                    nstored++;
+                    nh = std::min(nstored, nhist);
+                    for (int k = 0; k < nh; k++) {
+                        double s = 0.0;
+                        for (int64_t i = 0; i < n; i++) s += dF[(size_t)k * n + i] * dF[(size_t)slot * n + i];
+                        gram[(size_t)slot * lda + k] = s;
+                        gram[(size_t)k * lda + slot] = s;
+                    }
+                }
+            }
+        }
+        nh = std::min(nstored, nhist);
+        xold = x;
+        Fold = F;
+        if (nh > 0) {
+            for (int j = 0; j < nh; j++)
+                for (int i = 0; i < nh; i++) a[(size_t)j * lda + i] = gram[(size_t)j * lda + i];
+            for (int i = 0; i < nh; i++) a[(size_t)i * lda + i] += w0 * w0;
+            for (int k = 0; k < nh; k++) {
+                double s = 0.0;
+                for (int64_t i = 0; i < n; i++) s += (sp ? (double)dFs[(size_t)k * n + i] : dF[(size_t)k * n + i]) * F[i];
+                cvec[k] = s;
+            }
+            // dposv('U')
+            bool ok = true;
+            for (int j = 0; j < nh && ok; j++) {
+                double ajj = a[(size_t)j * lda + j];
+                for (int k = 0; k < j; k++) ajj -= a[(size_t)j * lda + k] * a[(size_t)j * lda + k];
+                if (!(ajj > 0.0)) { ok = false; break; }
+                ajj = std::sqrt(ajj);
+                a[(size_t)j * lda + j] = ajj;
+                for (int i = j + 1; i < nh; i++) {
+                    double t = a[(size_t)i * lda + j];
+                    for (int k = 0; k < j; k++) t -= a[(size_t)j * lda + k] * a[(size_t)i * lda + k];
+                    a[(size_t)i * lda + j] = t / ajj;
+                }
+            }
+            if (ok) {
+                for (int i = 0; i < nh; i++) {
+                    double t = cvec[i];
+                    for (int k = 0; k < i; k++) t -= a[(size_t)i * lda + k] * cvec[k];
+                    cvec[i] = t / a[(size_t)i * lda + i];
+                }
+                for (int i = nh - 1; i >= 0; i--) {
+                    double t = cvec[i];
+                    for (int k = i + 1; k < nh; k++) t -= a[(size_t)k * lda + i] * cvec[k];
+                    cvec[i] = t / a[(size_t)i * lda + i];
+                }
+            } else {
+                nstored = 0;
+                nh = 0;
+            }
+        }
+        for (int64_t i = 0; i < n; i++) {
+            double xi = x[i] + alpha * F[i];
+            for (int k = 0; k < nh; k++) xi -= (sp ? (double)Us[(size_t)k * n + i] : U[(size_t)k * n + i]) * cvec[k];
+            x[i] = xi;
+        }
+    }
+    *fmax_out = fmax;
+}
+
+// dposv('U') on the Gram system; returns false when not positive definite
+bool solve_spd(std::vector<double> &a, int nh, int lda, std::vector<double> &b) {
+    for (int j = 0; j < nh; j++) {
+        double ajj = a[(size_t)j * lda + j];
+        for (int k = 0; k < j; k++) ajj -= a[(size_t)j * lda + k] * a[(size_t)j * lda + k];
+        if (!(ajj > 0.0)) return false;
+        ajj = std::sqrt(ajj);
+        a[(size_t)j * lda + j] = ajj;
+        for (int i = j + 1; i < nh; i++) {
+            double t = a[(size_t)i * lda + j];
+            for (int k = 0; k < j; k++) t -= a[(size_t)j * lda + k] * a[(size_t)i * lda + k];
+            a[(size_t)i * lda + j] = t / ajj;
+        }
+    }
+    for (int i = 0; i < nh; i++) {
+        double t = b[i];
+        for (int k = 0; k < i; k++) t -= a[(size_t)i * lda + k] * b[k];
+        b[i] = t / a[(size_t)i * lda + i];
+    }
+    for (int i = nh - 1; i >= 0; i--) {
+        double t = b[i];
+        for (int k = i + 1; k < nh; k++) t -= a[(size_t)k * lda + i] * b[k];
+        b[i] = t / a[(size_t)i * lda + i];
+    }
+    return true;
+}
+
+// broyden_solve with x, F and the history in HBM (order 2)
+int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *fmax_out, Sanity *last, double *lf,
+                   double *lr) {
+    const int64_t n = h->n_vec;
+    const int nhist = h->p.nhist, lda = std::max(nhist, 1);
+    const double alpha = h->p.alpha_mix, w0 = 0.01;
+    const bool sp = h->p.single_history != 0;
+    std::vector<double> gram((size_t)lda * lda, 0.0), a((size_t)lda * lda), cvec(lda), gam(lda);
+    const int nblocks = qs_vec_blocks(n);
+    int nstored = 0, nh = 0, rc;
+    double fmax = 1.79769313486231570e308;
+    *converged = 0;
+    for (int its = 1; its <= h->p.maxscf; its++) {
+        *n_iter = its;
+        double st[3];
+        int ab = 0;
+        if ((rc = residual2_device(h, V, st, &ab, last))) return rc;
+        if (ab) break;
+        fmax = st[0];
+        const double rms = std::sqrt(st[1]) / std::sqrt((double)n);
+        if (lf) lf[its - 1] = fmax;
+        if (lr) lr[its - 1] = rms;
+        if (fmax < h->p.tolf) { *converged = 1; break; }
+        bool need_proj = false;
+        int slot = -1;
+        double fnorm = 0.0;
+        if (its > 1 && nhist > 0) {
+            fnorm = std::sqrt(st[2]);
+            if (fnorm > 0.0) {
+                slot = nstored % nhist;
+                nstored++;
+            }
+        }
+        nh = std::min(nstored, nhist);
+        if (nh > 0) need_proj = true;
+        if (need_proj) {
+            if (sp)
+                HIPCHK(h, qs_launch_broyden_hist_f32(n, h->d_F, h->d_Fold, h->d_x, h->d_xold, slot >= 0 ? 1.0 / fnorm : 0.0,
+                                                     alpha, (float *)h->d_hist_dF, (float *)h->d_hist_U, slot, nh,
+                                                     h->d_scratch, nblocks, h->stream));
+            else
+                HIPCHK(h, qs_launch_broyden_hist_f64(n, h->d_F, h->d_Fold, h->d_x, h->d_xold, fnorm, alpha,
+                                                     (double *)h->d_hist_dF, (double *)h->d_hist_U, slot, nh,
+                                                     h->d_scratch, nblocks, h->stream));
+            // sum the per-block partials [nblocks][2 MAXH]
+            HIPCHK(h, qs_launch_reduce_cols(h->d_scratch, 2 * MAXH_DEV, nblocks, h->d_sums,
+                                            h->d_scratch + (size_t)nblocks * 2 * MAXH_DEV, h->stream));
+            HIPCHK(h, hipMemcpyAsync(h->h_pin, h->d_sums, 2 * MAXH_DEV * sizeof(double), hipMemcpyDeviceToHost,
+                                     h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            if (slot >= 0)
+                for (int k = 0; k < nh; k++) {
+                    gram[(size_t)slot * lda + k] = h->h_pin[k];
+                    gram[(size_t)k * lda + slot] = h->h_pin[k];
+                }
+            for (int k = 0; k < nh; k++) cvec[k] = h->h_pin[MAXH_DEV + k];
+        }
+        // xold = x, Fold = F happen as buffer swaps after the update
+        int nh_use = nh;
+        if (nh > 0) {
+            for (int j = 0; j < nh; j++)
+                for (int i = 0; i < nh; i++) a[(size_t)j * lda + i] = gram[(size_t)j * lda + i];
+            for (int i = 0; i < nh; i++) a[(size_t)i * lda + i] += w0 * w0;
+            for (int k = 0; k < nh; k++) gam[k] = cvec[k];
+            if (!solve_spd(a, nh, lda, gam)) {
+                nstored = 0;
+                nh_use = 0;
+            }
+        }
+        if (sp)
+            HIPCHK(h, qs_launch_broyden_update_f32(n, h->d_x, h->d_F, (const float *)h->d_hist_U, gam.data(), nh_use,
+                                                   alpha, h->d_xold, h->stream));
+        else
+            HIPCHK(h, qs_launch_broyden_update_f64(n, h->d_x, h->d_F, (const double *)h->d_hist_U, gam.data(), nh_use,
+                                                   alpha, h->d_xold, h->stream));
+        std::swap(h->d_x, h->d_xold);  // x <- x_new, xold <- x
+        std::swap(h->d_F, h->d_Fold);  // Fold <- F
+    }
+    *fmax_out = fmax;
+    return QS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// order-1 vector: Re, Im of the 2x2 block of G^<(t=0) of each interacting site (src/scf.f90:226-255)
+void pack_nil(const qs_handle *h, const std::vector<zd> &G, std::vector<double> &x) {
+    const int n = h->norb;
+    x.assign(8 * (size_t)h->n_int, 0.0);
+    size_t k = 0;
+    for (int s = 0; s < h->nsites; s++) {
+        if (!h->site_int[s]) continue;
+        for (int b = 2 * s; b <= 2 * s + 1; b++)
+            for (int a = 2 * s; a <= 2 * s + 1; a++) {
+                x[k] = G[(size_t)b * n + a].real();
+                x[k + 1] = G[(size_t)b * n + a].imag();
+                k += 2;
+            }
+    }
+}
+void unpack_nil(const qs_handle *h, const std::vector<double> &x, std::vector<zd> &G) {
+    const int n = h->norb;
+    G.assign((size_t)n * n, 0.0);
+    size_t k = 0;
+    for (int s = 0; s < h->nsites; s++) {
+        if (!h->site_int[s]) continue;
+        for (int b = 2 * s; b <= 2 * s + 1; b++)
+            for (int a = 2 * s; a <= 2 * s + 1; a++) {
+                G[(size_t)b * n + a] = zd(x[k], x[k + 1]);
+                k += 2;
+            }
+    }
+}
+
+// sigma_first_order (src/self_energies.f90:45-72) on the host (order 1), uploaded as (nsites, 2, 2) row-major
+int upload_sigma1(qs_handle *h, const std::vector<zd> &Gt0) {
+    const int n = h->norb;
+    std::vector<zd> s1((size_t)4 * h->nsites, 0.0);
+    h->Sig1.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i += 2) {
+        const zd hart = Gt0[(size_t)i * n + i] + Gt0[(size_t)(i + 1) * n + i + 1];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                const zd g = Gt0[(size_t)(i + b) * n + i + a];
+                const zd v = (a == b) ? zd(0, 1) * h->U[i] * (g - hart) : zd(0, 1) * h->U[i] * g;
+                s1[(size_t)(i / 2) * 4 + a * 2 + b] = v;
+                h->Sig1[(size_t)(i + b) * n + i + a] = v;
+            }
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_sig1, s1.data(), s1.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    return QS_OK;
+}
+
+// gather_onsite (src/green_functions.f90:215-238) from the summed onsL
+void gather_onsite(const qs_handle *h, const std::vector<double> &sum_ons, const char *mask, std::vector<zd> &G) {
+    const int n = h->norb;
+    const double pp = h->p.delta / (2.0 * PI);
+    for (int s = 0; s < h->nsites; s++) {
+        if (mask && !mask[s]) continue;
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) {
+                const int k = 4 * s + 2 * b + a;
+                G[(size_t)(2 * s + b) * n + 2 * s + a] = zd(sum_ons[2 * k], sum_ons[2 * k + 1]) * pp;
+            }
+    }
+}
+
+// M_SCF1 sweep + sums: gRd/gLd sanity and the on-site blocks
+int sweep_scf1(qs_handle *h, bool use_sigma, std::vector<double> &sum_ons, Sanity *san) {
+    int rc = run_dyson(h, QS_M_SCF1, use_sigma, h->V_scf);
+    if (rc) return rc;
+    if ((rc = check_status(h))) return rc;
+    if ((rc = colsum(h, (const double *)h->d_onsL, 8 * h->nsites, sum_ons))) return rc;
+    if (san) {
+        std::vector<double> sR, sL;
+        if ((rc = colsum(h, h->d_diagR, h->nA, sR))) return rc;
+        if ((rc = colsum(h, h->d_diagL, h->nA, sL))) return rc;
+        *san = diag_sanity(sL.data(), sR.data(), h->nA, h->p.delta, h->act.data());
+    }
+    return QS_OK;
+}
+
+int final_sweep(qs_handle *h, double V) {
+    h->V_scf = V;
+    int rc = run_dyson(h, QS_M_FINAL, h->order >= 1, V);
+    if (rc) return rc;
+    if ((rc = check_status(h))) return rc;
+    const int n = h->norb;
+    std::vector<double> sum_ons, sum_dos, sum_tr, sum_atom;
+    if ((rc = colsum(h, (const double *)h->d_onsL, 8 * h->nsites, sum_ons))) return rc;
+    if ((rc = colsum(h, h->d_dosd, n, sum_dos))) return rc;
+    if ((rc = colsum(h, h->d_trUD, 2, sum_tr))) return rc;
+    h->G_less_obs.assign((size_t)n * n, 0.0);
+    gather_onsite(h, sum_ons, nullptr, h->G_less_obs);
+    if (h->has_atomL) {
+        if ((rc = colsum(h, (const double *)h->d_atomL, 2 * h->atom_tot, sum_atom))) return rc;
+        const double pp = h->p.delta / (2.0 * PI);
+        for (int at = 0; at < h->n_atoms; at++) {
+            const int m = 2 * h->atom_norb[at], lo = 2 * h->atom_first[at];
+            for (int b = 0; b < m; b++)
+                for (int a = 0; a < m; a++) {
+                    const int e = h->atom_off[at] + b * m + a;
+                    h->G_less_obs[(size_t)(lo + b) * n + lo + a] = zd(sum_atom[2 * e], sum_atom[2 * e + 1]) * pp;
+                }
+        }
+    }
+    h->J_up = sum_tr[0] * h->p.delta;
+    h->J_down = sum_tr[1] * h->p.delta;
+    // final_sanity: G^r_ii = i dosd, G^<_ii from the on-site blocks
+    std::vector<double> sImGL(n), sImGR(n);
+    for (int i = 0; i < n; i++) {
+        const int s = i / 2, a = i % 2, k = 4 * s + 3 * a;
+        sImGL[i] = sum_ons[2 * k + 1];
+        sImGR[i] = sum_dos[i];
+    }
+    Sanity fs = diag_sanity(sImGL.data(), sImGR.data(), n, h->p.delta, nullptr);
+    h->runaway_final = fs.runaway;
+    h->unphys_final = fs.unphysical;
+    h->iworst_final = fs.i_worst;
+    h->nworst_final = fs.n_worst;
+    h->wworst_final = fs.w_worst;
+    h->final_valid = true;
+    return QS_OK;
+}
+
+int check_problem(const qs_problem *p, std::string &msg) {
+    if (!p) { msg = "null problem"; return QS_ERR_ARG; }
+    if (p->norb < 2 || (p->norb & 1)) { msg = "norb must be even and >= 2"; return QS_ERR_ARG; }
+    if (p->n_omega < 1) { msg = "n_omega must be >= 1"; return QS_ERR_ARG; }
+    if (!(p->delta > 0)) { msg = "delta must be > 0"; return QS_ERR_ARG; }
+    if (!p->H || !p->gammaL || !p->gammaR || !p->U_site) { msg = "H, gammaL, gammaR and U_site are required"; return QS_ERR_ARG; }
+    if (p->order < 0 || p->order > 2) { msg = "order must be 0, 1 or 2"; return QS_ERR_ARG; }
+    if (p->nhist < 0 || p->nhist > MAXH_DEV) { msg = "nhist must be in [0, 32]"; return QS_ERR_ARG; }
+    if (p->maxscf < 1) { msg = "maxscf must be >= 1"; return QS_ERR_ARG; }
+    if (p->norb > qs_dyson_max_norb()) {
+        char b[128];
+        snprintf(b, sizeof b, "norb = %d exceeds the LDS-resident Dyson kernel limit of %d", p->norb, qs_dyson_max_norb());
+        msg = b;
+        return QS_ERR_UNSUPPORTED;
+    }
+    if (p->n_omega > (1 << 17)) { msg = "n_omega must be <= 131072"; return QS_ERR_UNSUPPORTED; }
+    return QS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qs_create(const qs_problem *p, qs_handle **out) {
+    if (!out) return QS_ERR_ARG;
+    *out = nullptr;
+    std::string msg;
+    int rc = check_problem(p, msg);
+    if (rc) {
+        fprintf(stderr, "qs_create: %s\n", msg.c_str());
+        return rc;
+    }
+    qs_handle *h = new qs_handle();
+    h->p = *p;
+    h->norb = p->norb; h->nw = p->n_omega; h->nsites = p->norb / 2; h->order = p->order;
+    const size_t nn = (size_t)h->norb * h->norb;
+    h->H.assign((const zd *)p->H, (const zd *)p->H + nn);
+    h->GamL.assign((const zd *)p->gammaL, (const zd *)p->gammaL + nn);
+    h->GamR.assign((const zd *)p->gammaR, (const zd *)p->gammaR + nn);
+    h->U.assign(p->U_site, p->U_site + h->norb);
+    if (p->n_atoms > 0 && p->atom_norb) {
+        h->n_atoms = p->n_atoms;
+        h->atom_norb.assign(p->atom_norb, p->atom_norb + p->n_atoms);
+    } else {
+        h->n_atoms = h->nsites;
+        h->atom_norb.assign(h->nsites, 1);
+    }
+    int acc = 0, maxn = 0;
+    h->atom_first.resize(h->n_atoms);
+    for (int a = 0; a < h->n_atoms; a++) { h->atom_first[a] = acc; acc += h->atom_norb[a]; maxn = std::max(maxn, h->atom_norb[a]); }
+    if (acc != h->nsites) { fprintf(stderr, "qs_create: atom_norb does not sum to norb/2\n"); delete h; return QS_ERR_ARG; }
+    h->atom_off.assign(h->n_atoms + 1, 0);
+    for (int a = 0; a < h->n_atoms; a++) h->atom_off[a + 1] = h->atom_off[a] + 4 * h->atom_norb[a] * h->atom_norb[a];
+    h->has_atomL = maxn > 1;
+    h->atom_tot = h->has_atomL ? h->atom_off[h->n_atoms] : 0;
+    int m = 1;
+    while ((1 << m) < 2 * h->nw - 1) m++;
+    h->Nf = 1 << m;
+    h->log2N2 = std::min(m / 2, 8);
+    h->log2N1 = m - h->log2N2;
+    h->N1 = 1 << h->log2N1;
+    h->N2 = 1 << h->log2N2;
+    setup_storage(h);
+    if (h->nCL > 64) { fprintf(stderr, "qs_create: more than 64 left-contact states\n"); delete h; return QS_ERR_UNSUPPORTED; }
+    if (h->order == 2 && (h->log2N1 > 10 || h->log2N2 < 1)) { delete h; return QS_ERR_UNSUPPORTED; }
+    rc = setup_device(h);
+    if (rc) {
+        fprintf(stderr, "qs_create: %s\n", h->err.c_str());
+        qs_destroy(h);
+        return rc;
+    }
+    h->G_less_t0.assign(nn, 0.0);
+    h->Sig1.assign(nn, 0.0);
+    h->G_less_obs.assign(nn, 0.0);
+    *out = h;
+    return QS_OK;
+}
+
+void qs_destroy(qs_handle *h) {
+    if (!h) return;
+    if (h->p.device >= 0) (void)hipSetDevice(h->p.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->h_pin) (void)hipHostFree(h->h_pin);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *qs_last_error(const qs_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int qs_sizes(const qs_handle *h, int32_t *nA, int32_t *nS, int32_t *nCL, int32_t *nAs, int64_t *n_vec, int32_t *nf) {
+    if (!h) return QS_ERR_ARG;
+    if (nA) *nA = h->nA;
+    if (nS) *nS = h->nS;
+    if (nCL) *nCL = h->nCL;
+    if (nAs) *nAs = h->nAs;
+    if (n_vec) *n_vec = h->n_vec;
+    if (nf) *nf = h->Nf;
+    return QS_OK;
+}
+
+int64_t qs_device_bytes(const qs_handle *h) { return h ? h->dev_bytes : 0; }
+
+int qs_init2(qs_handle *h, double V, double *x_out) {
+    if (!h) return QS_ERR_ARG;
+    if (h->order != 2) return fail(h, QS_ERR_STATE, "qs_init2 needs order 2");
+    HIPCHK(h, hipSetDevice(h->p.device));
+    h->V_scf = V;
+    int rc = run_dyson(h, QS_M_INIT2, false, V);
+    if (rc) return rc;
+    if ((rc = check_status(h))) return rc;
+    h->x_valid = true;
+    if (x_out) {
+        HIPCHK(h, hipMemcpyAsync(x_out, h->d_x, h->n_vec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    return QS_OK;
+}
+
+int qs_residual2(qs_handle *h, double V, int64_t n, const double *x, double *F, int32_t *abort_flag) {
+    if (!h || !x || !F || !abort_flag) return QS_ERR_ARG;
+    if (h->order != 2) return fail(h, QS_ERR_STATE, "qs_residual2 needs order 2");
+    if (n != h->n_vec) return fail(h, QS_ERR_ARG, "n = %lld but n_vec = %lld", (long long)n, (long long)h->n_vec);
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipMemcpyAsync(h->d_x, x, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    double st[3];
+    int ab = 0;
+    Sanity san;
+    int rc = residual2_device(h, V, st, &ab, &san);
+    if (rc) return rc;
+    h->x_valid = true;
+    *abort_flag = ab;
+    if (ab) {
+        std::memset(F, 0, n * sizeof(double));
+        return QS_OK;
+    }
+    HIPCHK(h, hipMemcpyAsync(F, h->d_F, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QS_OK;
+}
+
+int qs_get_sigma(const qs_handle *hc, double *SigR2, double *SigL2, double *Sig1) {
+    qs_handle *h = const_cast<qs_handle *>(hc);
+    if (!h) return QS_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    const int n = h->norb, nA = h->nA, nA2 = nA * nA;
+    if (Sig1) {
+        if (h->order == 2) {
+            std::vector<zd> s1((size_t)4 * h->nsites);
+            HIPCHK(h, hipMemcpyAsync(s1.data(), h->d_sig1, s1.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            zd *o = (zd *)Sig1;
+            std::fill(o, o + (size_t)n * n, zd(0.0));
+            for (int s = 0; s < h->nsites; s++)
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++) o[(size_t)(2 * s + b) * n + 2 * s + a] = s1[(size_t)s * 4 + a * 2 + b];
+        } else {
+            std::memcpy(Sig1, h->Sig1.data(), sizeof(zd) * n * n);
+        }
+    }
+    if ((SigR2 || SigL2) && h->order == 2) {
+        if (!h->sigma_valid) return fail(h, QS_ERR_STATE, "no second-order self-energy computed yet");
+        std::vector<zd> blk((size_t)h->nw * 2 * nA2);
+        HIPCHK(h, hipMemcpyAsync(blk.data(), h->d_Gt, blk.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        zd *r = (zd *)SigR2, *l = (zd *)SigL2;
+        for (int iw = 0; iw < h->nw; iw++)
+            for (int j = 0; j < nA; j++)
+                for (int i = 0; i < nA; i++) {
+                    const int c = (((i >> 1) * h->nAs + (j >> 1)) * 2) * 4 + (i & 1) * 2 + (j & 1);
+                    const size_t d = (size_t)iw * nA2 + (size_t)j * nA + i;
+                    if (r) r[d] = blk[(size_t)iw * 2 * nA2 + c];
+                    if (l) l[d] = blk[(size_t)iw * 2 * nA2 + c + 4];
+                }
+    }
+    return QS_OK;
+}
+
+int qs_final_sweep(qs_handle *h, double V, qs_scf_result *res) {
+    if (!h) return QS_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    int rc = final_sweep(h, V);
+    if (rc) return rc;
+    if (res) {
+        res->runaway = h->runaway_final;
+        res->unphysical = h->unphys_final;
+        res->i_worst = h->iworst_final;
+        res->n_worst = h->nworst_final;
+        res->w_worst = h->wworst_final;
+    }
+    return QS_OK;
+}
+
+int qs_solve_scf(qs_handle *h, double V, int32_t fresh, qs_scf_result *res, double *log_fmax, double *log_rms) {
+    if (!h) return QS_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    qs_scf_result r{};
+    r.converged = 1;
+    h->V_scf = V;
+    h->final_valid = false;
+    int rc;
+    if (h->order == 1) {
+        const int n = h->norb;
+        if (fresh) {
+            // G^<(t=0) of the non-interacting system (src/scf.f90:103-108)
+            h->G_less_t0.assign((size_t)n * n, 0.0);
+            std::vector<double> sum_ons;
+            if ((rc = sweep_scf1(h, false, sum_ons, nullptr))) return rc;
+            gather_onsite(h, sum_ons, h->site_int.data(), h->G_less_t0);
+        }
+        std::vector<double> x;
+        pack_nil(h, h->G_less_t0, x);
+        int err = QS_OK;
+        auto resid = [&](const std::vector<double> &xin, std::vector<double> &F, int *ab) {
+            *ab = 0;
+            if (err) { *ab = 1; return; }
+            unpack_nil(h, xin, h->G_less_t0);
+            if ((err = upload_sigma1(h, h->G_less_t0))) { *ab = 1; return; }
+            std::vector<double> sum_ons;
+            Sanity san;
+            if ((err = sweep_scf1(h, true, sum_ons, &san))) { *ab = 1; return; }
+            if (san.runaway) {
+                *ab = 1;
+                r.runaway = 1;
+                std::fill(F.begin(), F.end(), 0.0);
+                return;
+            }
+            std::vector<zd> Gout((size_t)n * n, 0.0);
+            gather_onsite(h, sum_ons, h->site_int.data(), Gout);
+            std::vector<double> xo;
+            pack_nil(h, Gout, xo);
+            for (size_t i = 0; i < F.size(); i++) F[i] = xo[i] - xin[i];
+        };
+        int conv = 0, nit = 0;
+        double fmax = 0.0;
+        broyden_host(resid, x, h->p.alpha_mix, h->p.nhist, h->p.maxscf, h->p.tolf_order1, h->p.single_history != 0,
+                     &conv, &nit, &fmax, log_fmax, log_rms);
+        if (err) return err;
+        r.converged = conv;
+        r.n_iter = nit;
+        r.fmax = fmax;
+    } else if (h->order == 2) {
+        if (fresh || !h->x_valid) {
+            if ((rc = run_dyson(h, QS_M_INIT2, false, V))) return rc;
+            if ((rc = check_status(h))) return rc;
+            h->x_valid = true;
+        }
+        int conv = 0, nit = 0;
+        double fmax = 0.0;
+        Sanity last;
+        if ((rc = broyden_device(h, V, &conv, &nit, &fmax, &last, log_fmax, log_rms))) return rc;
+        r.converged = conv;
+        r.n_iter = nit;
+        r.fmax = fmax;
+        r.runaway = last.runaway;
+        // the last residual's input is x (converged) or the last x before abort/maxscf
+    } else {
+        r.n_iter = 1;
+        if (log_fmax) log_fmax[0] = 0.0;
+        if (log_rms) log_rms[0] = 0.0;
+    }
+    if ((rc = final_sweep(h, V))) return rc;
+    if (r.converged && h->unphys_final) r.converged = 0;
+    r.unphysical = h->unphys_final;
+    r.i_worst = h->iworst_final;
+    r.n_worst = h->nworst_final;
+    r.w_worst = h->wworst_final;
+    if (res) *res = r;
+    return QS_OK;
+}
+
+int qs_current(const qs_handle *h, double *J_up, double *J_down) {
+    if (!h || !h->final_valid) return QS_ERR_STATE;
+    if (J_up) *J_up = h->J_up;
+    if (J_down) *J_down = h->J_down;
+    return QS_OK;
+}
+
+int qs_g_less_obs(const qs_handle *h, double *G) {
+    if (!h || !h->final_valid || !G) return QS_ERR_STATE;
+    std::memcpy(G, h->G_less_obs.data(), sizeof(zd) * h->G_less_obs.size());
+    return QS_OK;
+}
+
+int qs_dosd(const qs_handle *hc, double *dosd) {
+    qs_handle *h = const_cast<qs_handle *>(hc);
+    if (!h || !h->final_valid || !dosd) return QS_ERR_STATE;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    std::vector<double> tmp((size_t)h->nw * h->norb);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), h->d_dosd, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int iw = 0; iw < h->nw; iw++)
+        for (int i = 0; i < h->norb; i++) dosd[(size_t)iw * h->norb + i] = tmp[(size_t)iw * h->norb + i];
+    return QS_OK;
+}
+
+int qs_trUD(const qs_handle *hc, double *trU, double *trD) {
+    qs_handle *h = const_cast<qs_handle *>(hc);
+    if (!h || !h->final_valid) return QS_ERR_STATE;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    std::vector<double> tmp((size_t)h->nw * 2);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), h->d_trUD, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int iw = 0; iw < h->nw; iw++) {
+        if (trU) trU[iw] = tmp[2 * (size_t)iw];
+        if (trD) trD[iw] = tmp[2 * (size_t)iw + 1];
+    }
+    return QS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// qs_internal.h -- launch interfaces between the engine (qs_engine.cpp) and the
+// gfx950 kernels.  Not part of the public C ABI (include/qs_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// modes of the per-energy solve, as G_at_energy's (src/scf.f90:49)
+enum { QS_M_INIT2 = 1, QS_M_SCF2 = 2, QS_M_SCF1 = 3, QS_M_FINAL = 4 };
+
+// Arguments of the fused per-energy Dyson + Keldysh kernel.  Matrices are
+// row-major complex (double2).  The second-order self-energy uses the
+// site-pair blocked layout described in qs_sigma2.hip.
+struct DysonArgs {
+    int norb, nA, nS, nCL, nsites, nAs, n_atoms, atom_tot;
+    int nw, iw0;             // energies handled: iw0 .. iw0 + gridDim.x - 1
+    double w_init, delta;    // omega(iw) = w_init + (iw + 1) * delta
+    double mu, beta, V;
+    int mode, use_sigma, order;
+    const double2 *Hemb;     // (norb, norb): -H + (i/2)(GammaL + GammaR)
+    const double2 *sig1;     // (nsites, 2, 2) on-site first-order self-energy
+    const double2 *sig;      // [nw][2 nA^2] second-order self-energy, blocked layout
+    const double2 *gamL_s, *gamR_s;  // (nS, nS)
+    const double2 *gamL_cl;  // (nCL, nCL)
+    const int *act, *cl, *pos, *suppos;  // suppos: orbital -> index in sup or -1
+    const int *tri_map;      // (n_su + n_u): (i << 16) | j of the packed G^< entries
+    const int *atom_first, *atom_norb, *atom_off;
+    const double *x;         // packed input vector (SCF2)
+    double *xout;            // INIT2: packed non-interacting G written here
+    double *F;               // SCF2: pack(G_out) - x
+    const double *Fold;      // SCF2: previous residual (for |F - Fold|^2)
+    double *partials;        // SCF2: [block][4] max|F|, sum F^2, sum (F-Fold)^2
+    double *diagR, *diagL;   // SCF1/SCF2: [iw][nA] Im G^r_kk, Im G^<_kk
+    double2 *onsL;           // SCF1/FINAL: [iw][4 nsites]
+    double *dosd;            // FINAL: [iw][norb] Im G^r_ii
+    double2 *atomL;          // FINAL: [iw][atom_tot]
+    double *trUD;            // FINAL: [iw][2] current integrand up / down
+    int *status;             // bit 0: singular pivot met
+};
+
+// Dyson + Keldysh at every energy of [iw0, iw0 + count).  Returns the hip error.
+hipError_t qs_launch_dyson(const DysonArgs &a, int count, hipStream_t s);
+size_t qs_dyson_lds_bytes(int norb, int nCL);
+int qs_dyson_max_norb();
+
+// Second-order self-energy stage (qs_sigma2.hip)
+struct Sigma2Plan {
+    int nA, nAs, nw, Nf, N1, N2, log2N1, log2N2;
+    double delta;
+    const double2 *tw;       // (Nf) e^{-2 pi i k / Nf}
+    const double *Upair;     // (nAs, nAs) U_i * U_j * hbar^2 (delta/2pi)^2 / Nf
+};
+// x (packed, reference layout) -> Gt [nw][2 nA^2] blocked layout
+hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int nw, hipStream_t s);
+hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, hipStream_t s);
+
+// small vector kernels (qs_vector.hip)
+// part: >= 256 * 4 nAs doubles of scratch
+hipError_t qs_launch_equal_time_sigma1(const double *x, const int *srcidx, const int *act_site,
+                                       const double *U_site_act, double2 *gt0, double2 *sig1, int nA, int nAs, int nw,
+                                       double pp, double *part, hipStream_t s);
+// out[c] = sum_r in[r][c]; scratch >= 64 * ncols doubles
+hipError_t qs_launch_reduce_cols(const double *in, int ncols, int nrows, double *out, double *scratch,
+                                 hipStream_t s);
+hipError_t qs_launch_reduce_partials(const double *partials, int nblocks, double *out, hipStream_t s);
+hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *Fold, const double *x,
+                                      const double *xold, double rf, double alpha, float *dFs, float *Us,
+                                      int slot, int nh, double *partials, int nblocks, hipStream_t s);
+hipError_t qs_launch_broyden_hist_f64(int64_t n, const double *F, const double *Fold, const double *x,
+                                      const double *xold, double fnorm, double alpha, double *dF, double *U,
+                                      int slot, int nh, double *partials, int nblocks, hipStream_t s);
+hipError_t qs_launch_broyden_update_f32(int64_t n, const double *x, const double *F, const float *Us,
+                                        const double *gamma, int nh, double alpha, double *xnew, hipStream_t s);
+hipError_t qs_launch_broyden_update_f64(int64_t n, const double *x, const double *F, const double *U,
+                                        const double *gamma, int nh, double alpha, double *xnew, hipStream_t s);
+hipError_t qs_launch_vec_stats(int64_t n, const double *F, const double *Fold, double *partials, int nblocks,
+                               hipStream_t s);
+hipError_t qs_launch_fill(double *p, int64_t n, double v, hipStream_t s);
+int qs_vec_blocks(int64_t n);

@@ -1,0 +1,285 @@
+// qs_sigma2.hip -- second-order self-energy on the whole energy grid (FP64).
+//
+// Replaces sigma2_compute / load_fft / adj2 (src/self_energies.f90:116-228):
+// per pair of active sites (i, j) and frequency l of the zero-padded DFT,
+//   a_ij(l) = r_ji(-l)^dag,  G_ij = L_ij + r_ij - a_ij  (greater)
+//   chi1 = L_ji(-l) G_ij(l),  chi2 = L_ji(-l) r_ij(l) + a_ji(-l) L_ij(l)
+//   S^r_ij = c [ r_ij adj(chi1) + L_ij adj(chi2) ],  S^<_ij = c L_ij adj(G_ji(-l) L_ij),
+//   adj(M) = Tr(M) 1 - M,  c = U_i U_j hbar^2 (delta/2pi)^2 / Nf,
+// then an inverse DFT keeps the first n_omega points.  Zero padding to
+// Nf >= 2 n_omega - 1 makes each term an exact linear convolution, so the result
+// does not depend on Nf; this file uses the power of two Nf = 2^m (the reference
+// uses the smallest 7-smooth length, good_fft_size).
+//
+// Data layout ("blocked"): per energy, 2 nA^2 complex values ordered as
+//   c = ((ia * nAs + ja) * 2 + kind) * 4 + sa * 2 + sb
+// (active sites ia, ja; kind 0 = G^r, 1 = G^<; spins sa, sb), so the 8 values a
+// site pair needs are one 128-byte segment.  Gt and Sig use [omega][c]; the
+// frequency-domain buffer Z uses [k1][n2][c].
+//
+// The length-Nf DFT is split as Nf = N1 * N2 (n = n2 + N2 n1, l = k1 + N1 k2):
+//   F1: DFT over n1 (length N1) for every (c, n2), twiddle e^{-2 pi i k1 n2/Nf}
+//   M : per site pair and row pair (k1, -k1): DFT over n2 (length N2), the 2x2
+//       spin algebra at l and -l, inverse DFT over k2, twiddle e^{+2 pi i k1 n2/Nf}
+//   I2: inverse DFT over k1 (length N1) for every (c, n2), keep n < n_omega.
+// Each pass reads and writes HBM once in 128-byte segments; the DFTs run in LDS
+// (radix-2, decimation in time, bit-reversed load).
+#include "qs_device.h"
+#include "qs_internal.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int CB = 8;  // columns per site-pair block
+
+__device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned)v) >> (32 - bits); }
+
+// In-place radix-2 DIT on `nseq` sequences of length 2^lg, stored with stride `ld`
+// (complex), input already in bit-reversed order.  sign = -1 forward, +1 inverse.
+// twstep = Nf / 2^lg; tw[k] = e^{-2 pi i k / Nf}.
+__device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const double2 *tw, int twstep) {
+    const int L = 1 << lg, half_total = nseq * (L >> 1);
+    for (int s = 1; s <= lg; s++) {
+        const int half = 1 << (s - 1);
+        const int tws = twstep * (L >> s);  // e^{-2 pi i j / 2^s} = tw[j * Nf / 2^s]
+        for (int t = threadIdx.x; t < half_total; t += blockDim.x) {
+            const int seq = t >> (lg - 1);
+            const int bf = t & ((L >> 1) - 1);
+            const int j = bf & (half - 1), g = bf >> (s - 1);
+            const int i1 = g * (half << 1) + j, i2 = i1 + half;
+            double2 w = tw[j * tws];
+            if (sign > 0) w.y = -w.y;
+            double2 *b = buf + (size_t)seq * ld;
+            const double2 u = b[i1], v = cmul(w, b[i2]);
+            b[i1] = cadd(u, v);
+            b[i2] = csub(u, v);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- x (reference packed layout) -> Gt[omega][c] (blocked) ----------------------
+__global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x, double2 *__restrict__ Gt,
+                                                   const int2 *__restrict__ src, int nA, int nw) {
+    const int ncol = 2 * nA * nA;
+    const size_t nblk = 3 * (size_t)nA * nA;
+    const size_t total = (size_t)nw * ncol;
+    for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < total; e += (size_t)gridDim.x * NT) {
+        const size_t iw = e / ncol;
+        const int c = (int)(e - iw * ncol);
+        const int2 s = src[c];
+        const double *xb = x + iw * nblk;
+        double re = 0.0;
+        if (s.x > 0) re = xb[s.x - 1];
+        else if (s.x < 0) re = -xb[-s.x - 1];
+        Gt[e] = make_double2(re, xb[s.y]);
+    }
+}
+
+// ---- F1 -----------------------------------------------------------------------
+// grid: (ncol / CB, N2).  LDS: CB * N1 complex.
+__global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
+                                               Sigma2Plan p) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double2 *buf = (double2 *)lds_raw;
+    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
+    const int N1 = p.N1;
+    for (int t = threadIdx.x; t < CB * N1; t += NT) {
+        const int col = t & (CB - 1), n1 = t >> 3;
+        const int n = n2 + p.N2 * n1;
+        double2 v = make_double2(0.0, 0.0);
+        if (n < p.nw) v = Gt[(size_t)n * ncol + cb * CB + col];
+        buf[col * N1 + bitrev(n1, p.log2N1)] = v;
+    }
+    __syncthreads();
+    lds_fft(buf, CB, N1, p.log2N1, -1, p.tw, p.Nf / N1);
+    for (int t = threadIdx.x; t < CB * N1; t += NT) {
+        const int col = t & (CB - 1), k1 = t >> 3;
+        const double2 v = cmul(buf[col * N1 + k1], p.tw[k1 * n2]);
+        Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = v;
+    }
+}
+
+// ---- M ------------------------------------------------------------------------
+__device__ __forceinline__ void mm2(const double2 *A, const double2 *B, double2 *C) {
+    // 2x2 row-major: [0]=11 [1]=12 [2]=21 [3]=22
+    C[0] = cadd(cmul(A[0], B[0]), cmul(A[1], B[2]));
+    C[1] = cadd(cmul(A[0], B[1]), cmul(A[1], B[3]));
+    C[2] = cadd(cmul(A[2], B[0]), cmul(A[3], B[2]));
+    C[3] = cadd(cmul(A[2], B[1]), cmul(A[3], B[3]));
+}
+__device__ __forceinline__ void adj2(const double2 *M, double2 *A) {
+    A[0] = M[3];
+    A[1] = make_double2(-M[1].x, -M[1].y);
+    A[2] = make_double2(-M[2].x, -M[2].y);
+    A[3] = M[0];
+}
+__device__ __forceinline__ void dag2(const double2 *M, double2 *A) {
+    A[0] = cconj(M[0]);
+    A[1] = cconj(M[2]);
+    A[2] = cconj(M[1]);
+    A[3] = cconj(M[3]);
+}
+
+// Sigma^r, Sigma^< of the ordered pair (i, j) at +l from blocks at +l (r_ij, L_ij)
+// and at -l (r_ji, L_ji); src/self_energies.f90:175-198.
+__device__ __forceinline__ void pair_sigma(const double2 *r_ij, const double2 *L_ij, const double2 *r_ji,
+                                           const double2 *L_ji, double c, double2 *sR, double2 *sL) {
+    double2 a_ij[4], a_ji[4], G_ij[4], G_ji[4], chi1[4], chi2[4], t1[4], t2[4], ad1[4], ad2[4];
+    dag2(r_ji, a_ij);
+    dag2(r_ij, a_ji);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        G_ij[q] = csub(cadd(L_ij[q], r_ij[q]), a_ij[q]);
+        G_ji[q] = csub(cadd(L_ji[q], r_ji[q]), a_ji[q]);
+    }
+    mm2(L_ji, G_ij, chi1);
+    mm2(L_ji, r_ij, t1);
+    mm2(a_ji, L_ij, t2);
+#pragma unroll
+    for (int q = 0; q < 4; q++) chi2[q] = cadd(t1[q], t2[q]);
+    adj2(chi1, ad1);
+    adj2(chi2, ad2);
+    mm2(r_ij, ad1, t1);
+    mm2(L_ij, ad2, t2);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sR[q] = cscale(cadd(t1[q], t2[q]), c);
+    mm2(G_ji, L_ij, chi1);
+    adj2(chi1, ad1);
+    mm2(L_ij, ad1, t1);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sL[q] = cscale(t1[q], c);
+}
+
+// grid: (nAs (nAs+1)/2 unordered pairs, N1/2 + 1 row pairs).  LDS: 16 * 2 * N2 complex.
+// Thread work per position: the 16 outputs stay in registers across the barrier
+// that protects the in-place overwrite.
+constexpr int MAXPOS = 2;  // positions per thread: 2 * N2 / NT <= 2  (N2 <= 256)
+
+__global__ __launch_bounds__(NT) void k_fft_mid(double2 *__restrict__ Z, Sigma2Plan p) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double2 *buf = (double2 *)lds_raw;  // [s (16)][row (2)][N2]
+    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA;
+    // unordered pair index -> (ia <= ja)
+    int pid = blockIdx.x, ia = 0;
+    while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
+    const int ja = ia + pid;
+    const int k1 = blockIdx.y, k1p = (N1 - k1) & (N1 - 1);
+    const int nrow = (k1 == k1p) ? 1 : 2;
+    const int cb_ij = (ia * nAs + ja) * CB, cb_ji = (ja * nAs + ia) * CB;
+    const double c = p.Upair[ia * nAs + ja];
+    const int LD = 2 * N2;  // stride between the 16 sequences (rows packed)
+    if (c == 0.0) {
+        for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
+            const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+            const int row = rr ? k1p : k1;
+            const int cb = s < 8 ? cb_ij : cb_ji;
+            Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)] = make_double2(0.0, 0.0);
+        }
+        return;
+    }
+    for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
+        const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+        const int row = rr ? k1p : k1;
+        const int cb = s < 8 ? cb_ij : cb_ji;
+        buf[s * LD + rr * N2 + bitrev(n2, lg2)] = Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)];
+    }
+    __syncthreads();
+    // forward DFT over n2: 16 * nrow sequences of length N2 (rows packed with stride N2)
+    lds_fft(buf, 16 * 2, N2, lg2, -1, p.tw, p.Nf / N2);
+    // pointwise at l = (row, k2) and -l
+    double2 outv[MAXPOS][16];
+#pragma unroll
+    for (int q = 0; q < MAXPOS; q++) {
+        const int t = threadIdx.x + q * NT;
+        if (t < nrow * N2) {
+            const int rr = t / N2, k2 = t - rr * N2;
+            const int row = rr ? k1p : k1;
+            int rq, m2;
+            if (row == 0) { rq = 0; m2 = (N2 - k2) & (N2 - 1); }
+            else { rq = (N1 - row) & (N1 - 1); m2 = N2 - 1 - k2; }
+            const int rrq = (rq == k1) ? 0 : 1;
+            double2 P[16], Q[16];
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                P[s] = buf[s * LD + rr * N2 + k2];
+                Q[s] = buf[s * LD + rrq * N2 + m2];
+            }
+            // (ij): r_ij, L_ij at +l = P[0..3], P[4..7]; r_ji, L_ji at -l = Q[8..11], Q[12..15]
+            pair_sigma(P + 0, P + 4, Q + 8, Q + 12, c, outv[q] + 0, outv[q] + 4);
+            // (ji): r_ji, L_ji at +l = P[8..11], P[12..15]; r_ij, L_ij at -l = Q[0..3], Q[4..7]
+            pair_sigma(P + 8, P + 12, Q + 0, Q + 4, c, outv[q] + 8, outv[q] + 12);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXPOS; q++) {
+        const int t = threadIdx.x + q * NT;
+        if (t < nrow * N2) {
+            const int rr = t / N2, k2 = t - rr * N2;
+#pragma unroll
+            for (int s = 0; s < 16; s++) buf[s * LD + rr * N2 + bitrev(k2, lg2)] = outv[q][s];
+        }
+    }
+    __syncthreads();
+    lds_fft(buf, 16 * 2, N2, lg2, +1, p.tw, p.Nf / N2);
+    for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
+        const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+        const int row = rr ? k1p : k1;
+        const int cb = s < 8 ? cb_ij : cb_ji;
+        const double2 w = cconj(p.tw[row * n2]);
+        Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)] = cmul(buf[s * LD + rr * N2 + n2], w);
+    }
+}
+
+// ---- I2 -----------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
+                                               Sigma2Plan p) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double2 *buf = (double2 *)lds_raw;
+    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
+    const int N1 = p.N1;
+    for (int t = threadIdx.x; t < CB * N1; t += NT) {
+        const int col = t & (CB - 1), k1 = t >> 3;
+        buf[col * N1 + bitrev(k1, p.log2N1)] = Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col];
+    }
+    __syncthreads();
+    lds_fft(buf, CB, N1, p.log2N1, +1, p.tw, p.Nf / N1);
+    for (int t = threadIdx.x; t < CB * N1; t += NT) {
+        const int col = t & (CB - 1), n1 = t >> 3;
+        const int n = n2 + p.N2 * n1;
+        if (n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * N1 + n1];
+    }
+}
+
+}  // namespace
+
+hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int nw, hipStream_t s) {
+    const size_t total = (size_t)nw * 2 * nA * nA;
+    size_t blocks = (total + NT - 1) / NT;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_permute_in, dim3((unsigned)blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, nw);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, hipStream_t s) {
+    const int ncol = 2 * p.nA * p.nA, nblk = ncol / CB;
+    const size_t lds1 = (size_t)CB * p.N1 * sizeof(double2);
+    const size_t ldsm = (size_t)16 * 2 * p.N2 * sizeof(double2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_fft_f1, dim3(nblk, p.N2), dim3(NT), lds1, s, Gt, Z, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int npairs = p.nAs * (p.nAs + 1) / 2;
+    hipLaunchKernelGGL(k_fft_mid, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fft_i2, dim3(nblk, p.N2), dim3(NT), lds1, s, Z, Sig, p);
+    return hipGetLastError();
+}

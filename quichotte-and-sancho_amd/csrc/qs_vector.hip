@@ -1,0 +1,294 @@
+// qs_vector.hip -- HBM-streaming kernels of the SCF iteration (FP64, gfx950).
+//
+//   equal-time G^<(t=0) on the active sites + first-order Sigma
+//        src/green_functions.f90:192-213 (equal_time_lesser),
+//        src/self_energies.f90:45-72 (sigma_first_order)
+//   column sums over the energy grid (sanity sums, gathers, current)
+//        common/scf_checks.f90:59-91, src/green_functions.f90:215-257,
+//        src/observables.f90:22-39
+//   Broyden history + Gram/projection dots in one pass, and the update pass
+//        common/broyden_mixing.f90:102-178 (dF, u columns; dot_ss / dot_sd;
+//        x <- x + alpha F - sum_k gamma_k u_k)
+// Every reduction is two-stage with a fixed order, so results do not depend on
+// scheduling.
+#include "qs_device.h"
+#include "qs_internal.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int MAXH = 32;     // largest Broyden history handled on the device
+constexpr int RCHUNKS = 64;  // row chunks of the column sums
+
+// ---- equal-time lesser + Sigma1 ------------------------------------------------
+// srcidx[4 ia + q]: offsets in an energy block of Im G<(i,i), Im G<(i+1,i+1),
+// Re G<(i,i+1), Im G<(i,i+1) for the active site ia (i = 2 ia).
+__global__ __launch_bounds__(NT) void k_et_partial(const double *__restrict__ x, const int *__restrict__ srcidx,
+                                                   int nq, int nw, size_t nblk, double *__restrict__ part) {
+    const int q = blockIdx.x * NT + threadIdx.x;
+    const int chunk = blockIdx.y, nch = gridDim.y;
+    if (q >= nq) return;
+    const int off = srcidx[q];
+    const int per = (nw + nch - 1) / nch, w0 = chunk * per, w1 = min(nw, w0 + per);
+    double s = 0.0;
+    for (int iw = w0; iw < w1; iw++) s += x[(size_t)iw * nblk + off];
+    part[(size_t)chunk * nq + q] = s;
+}
+
+__global__ void k_et_final(const double *__restrict__ part, int nq, int nch, const int *__restrict__ act_site,
+                           const double *__restrict__ U_site_act, double pp, double2 *__restrict__ gt0,
+                           double2 *__restrict__ sig1) {
+    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ia >= nq / 4) return;
+    double S[4];
+    for (int q = 0; q < 4; q++) {
+        double s = 0.0;
+        for (int c = 0; c < nch; c++) s += part[(size_t)c * nq + 4 * ia + q];
+        S[q] = s;
+    }
+    // G<(t=0) block: g00 = i pp S0, g11 = i pp S1, g01 = pp (S2 + i S3), g10 = -conj(g01)
+    double2 g[4];
+    g[0] = make_double2(0.0, pp * S[0]);
+    g[3] = make_double2(0.0, pp * S[1]);
+    g[1] = make_double2(pp * S[2], pp * S[3]);
+    g[2] = make_double2(-g[1].x, g[1].y);
+    const int site = act_site[ia];
+    for (int q = 0; q < 4; q++) gt0[4 * ia + q] = g[q];
+    const double U = U_site_act[ia];
+    const double2 H = cadd(g[0], g[3]);
+    // Sigma1(a,a) = i U (g_aa - H);  Sigma1(a,b) = i U g_ab   (hbar = 1)
+    double2 s1[4];
+    s1[0] = make_double2(-U * (g[0].y - H.y), U * (g[0].x - H.x));
+    s1[3] = make_double2(-U * (g[3].y - H.y), U * (g[3].x - H.x));
+    s1[1] = make_double2(-U * g[1].y, U * g[1].x);
+    s1[2] = make_double2(-U * g[2].y, U * g[2].x);
+    for (int q = 0; q < 4; q++) sig1[4 * site + q] = s1[q];
+}
+
+// ---- column sums ---------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_colsum_partial(const double *__restrict__ in, int ncols, int nrows,
+                                                       double *__restrict__ part) {
+    const int c = blockIdx.x * NT + threadIdx.x, chunk = blockIdx.y;
+    if (c >= ncols) return;
+    const int per = (nrows + RCHUNKS - 1) / RCHUNKS, r0 = chunk * per, r1 = min(nrows, r0 + per);
+    double s = 0.0;
+    for (int r = r0; r < r1; r++) s += in[(size_t)r * ncols + c];
+    part[(size_t)chunk * ncols + c] = s;
+}
+__global__ __launch_bounds__(NT) void k_colsum_final(const double *__restrict__ part, int ncols,
+                                                     double *__restrict__ out) {
+    const int c = blockIdx.x * NT + threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0.0;
+    for (int k = 0; k < RCHUNKS; k++) s += part[(size_t)k * ncols + c];
+    out[c] = s;
+}
+
+// partials [nblocks][4] = (max, sum, sum, -) -> out[3]
+__global__ __launch_bounds__(1024) void k_reduce_partials(const double *__restrict__ p, int nb, double *out) {
+    __shared__ double scratch[16];
+    double m = 0.0, s = 0.0, d = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        m = fmax(m, p[(size_t)b * 4 + 0]);
+        s += p[(size_t)b * 4 + 1];
+        d += p[(size_t)b * 4 + 2];
+    }
+    m = block_max(m, scratch);
+    s = block_sum(s, scratch);
+    d = block_sum(d, scratch);
+    if (threadIdx.x == 0) { out[0] = m; out[1] = s; out[2] = d; }
+}
+
+// ---- Broyden -------------------------------------------------------------------
+struct Gam { double g[MAXH]; };
+
+// One pass: store the new (dF, u) pair (slot >= 0) and accumulate, for k < nh,
+//   gram_k = <dF_k, dF_slot>,  proj_k = <dF_k, F>   (double sums of FP32 history)
+// partials[block][2 MAXH].
+__global__ __launch_bounds__(NT) void k_hist_f32(int64_t n, const double *__restrict__ F,
+                                                 const double *__restrict__ Fold, const double *__restrict__ x,
+                                                 const double *__restrict__ xold, double rf, double alpha,
+                                                 float *__restrict__ dFs, float *__restrict__ Us, int slot, int nh,
+                                                 double *__restrict__ part) {
+    __shared__ double red[NT / QS_WAVE][2 * MAXH];
+    double g[MAXH], pr[MAXH];
+#pragma unroll
+    for (int k = 0; k < MAXH; k++) { g[k] = 0.0; pr[k] = 0.0; }
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        const double f = F[i];
+        float df = 0.0f;
+        if (slot >= 0) {
+            const double d = f - Fold[i];
+            df = (float)(d * rf);
+            const float u = (float)(alpha * d * rf + (x[i] - xold[i]) * rf);
+            dFs[(size_t)slot * n + i] = df;
+            Us[(size_t)slot * n + i] = u;
+        }
+#pragma unroll
+        for (int k = 0; k < MAXH; k++)
+            if (k < nh) {
+                const double hk = (k == slot) ? (double)df : (double)dFs[(size_t)k * n + i];
+                g[k] = fma(hk, (double)df, g[k]);
+                pr[k] = fma(hk, f, pr[k]);
+            }
+    }
+    const int w = threadIdx.x / QS_WAVE, lane = threadIdx.x & (QS_WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < MAXH; k++)
+        if (k < nh) {
+            const double a = wave_sum(g[k]), b = wave_sum(pr[k]);
+            if (lane == 0) { red[w][k] = a; red[w][MAXH + k] = b; }
+        }
+    __syncthreads();
+    if (threadIdx.x < 2 * MAXH) {
+        double s = 0.0;
+        for (int ww = 0; ww < NT / QS_WAVE; ww++) s += ((threadIdx.x & (MAXH - 1)) < nh) ? red[ww][threadIdx.x] : 0.0;
+        part[(size_t)blockIdx.x * 2 * MAXH + threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_hist_f64(int64_t n, const double *__restrict__ F,
+                                                 const double *__restrict__ Fold, const double *__restrict__ x,
+                                                 const double *__restrict__ xold, double fnorm, double alpha,
+                                                 double *__restrict__ dF, double *__restrict__ U, int slot, int nh,
+                                                 double *__restrict__ part) {
+    __shared__ double red[NT / QS_WAVE][2 * MAXH];
+    double g[MAXH], pr[MAXH];
+#pragma unroll
+    for (int k = 0; k < MAXH; k++) { g[k] = 0.0; pr[k] = 0.0; }
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        const double f = F[i];
+        double dfn = 0.0;
+        if (slot >= 0) {
+            dfn = (f - Fold[i]) / fnorm;
+            dF[(size_t)slot * n + i] = dfn;
+            U[(size_t)slot * n + i] = alpha * dfn + (x[i] - xold[i]) / fnorm;
+        }
+#pragma unroll
+        for (int k = 0; k < MAXH; k++)
+            if (k < nh) {
+                const double hk = (k == slot) ? dfn : dF[(size_t)k * n + i];
+                g[k] = fma(hk, dfn, g[k]);
+                pr[k] = fma(hk, f, pr[k]);
+            }
+    }
+    const int w = threadIdx.x / QS_WAVE, lane = threadIdx.x & (QS_WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < MAXH; k++)
+        if (k < nh) {
+            const double a = wave_sum(g[k]), b = wave_sum(pr[k]);
+            if (lane == 0) { red[w][k] = a; red[w][MAXH + k] = b; }
+        }
+    __syncthreads();
+    if (threadIdx.x < 2 * MAXH) {
+        double s = 0.0;
+        for (int ww = 0; ww < NT / QS_WAVE; ww++) s += ((threadIdx.x & (MAXH - 1)) < nh) ? red[ww][threadIdx.x] : 0.0;
+        part[(size_t)blockIdx.x * 2 * MAXH + threadIdx.x] = s;
+    }
+}
+
+// x_new = x + alpha F - sum_{k < nh} gamma_k u_k   (k in storage order, as the reference)
+__global__ __launch_bounds__(NT) void k_update_f32(int64_t n, const double *__restrict__ x,
+                                                   const double *__restrict__ F, const float *__restrict__ Us,
+                                                   Gam gm, int nh, double alpha, double *__restrict__ xnew) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        double v = x[i] + alpha * F[i];
+#pragma unroll
+        for (int k = 0; k < MAXH; k++)
+            if (k < nh) v = v - (double)Us[(size_t)k * n + i] * gm.g[k];
+        xnew[i] = v;
+    }
+}
+__global__ __launch_bounds__(NT) void k_update_f64(int64_t n, const double *__restrict__ x,
+                                                   const double *__restrict__ F, const double *__restrict__ U,
+                                                   Gam gm, int nh, double alpha, double *__restrict__ xnew) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        double v = x[i] + alpha * F[i];
+#pragma unroll
+        for (int k = 0; k < MAXH; k++)
+            if (k < nh) v = v - U[(size_t)k * n + i] * gm.g[k];
+        xnew[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_fill(double *p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) p[i] = v;
+}
+
+}  // namespace
+
+int qs_vec_blocks(int64_t n) {
+    int64_t b = (n + NT - 1) / NT;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+hipError_t qs_launch_equal_time_sigma1(const double *x, const int *srcidx, const int *act_site,
+                                       const double *U_site_act, double2 *gt0, double2 *sig1, int nA, int nAs, int nw,
+                                       double pp, double *part, hipStream_t s) {
+    const int nq = 4 * nAs, nch = 256;
+    const size_t nblk = 3 * (size_t)nA * nA;
+    hipLaunchKernelGGL(k_et_partial, dim3((nq + NT - 1) / NT, nch), dim3(NT), 0, s, x, srcidx, nq, nw, nblk, part);
+    hipLaunchKernelGGL(k_et_final, dim3((nAs + 63) / 64), dim3(64), 0, s, part, nq, nch, act_site, U_site_act, pp, gt0,
+                       sig1);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_reduce_cols(const double *in, int ncols, int nrows, double *out, double *scratch,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_colsum_partial, dim3((ncols + NT - 1) / NT, RCHUNKS), dim3(NT), 0, s, in, ncols, nrows,
+                       scratch);
+    hipLaunchKernelGGL(k_colsum_final, dim3((ncols + NT - 1) / NT), dim3(NT), 0, s, scratch, ncols, out);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_reduce_partials(const double *partials, int nblocks, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *Fold, const double *x,
+                                      const double *xold, double rf, double alpha, float *dFs, float *Us, int slot,
+                                      int nh, double *partials, int nblocks, hipStream_t s) {
+    if (nh > MAXH) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_hist_f32, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us, slot, nh,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_broyden_hist_f64(int64_t n, const double *F, const double *Fold, const double *x,
+                                      const double *xold, double fnorm, double alpha, double *dF, double *U, int slot,
+                                      int nh, double *partials, int nblocks, hipStream_t s) {
+    if (nh > MAXH) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_hist_f64, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, fnorm, alpha, dF, U, slot, nh,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_broyden_update_f32(int64_t n, const double *x, const double *F, const float *Us,
+                                        const double *gamma, int nh, double alpha, double *xnew, hipStream_t s) {
+    if (nh > MAXH) return hipErrorInvalidValue;
+    Gam gm;
+    for (int k = 0; k < MAXH; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
+    hipLaunchKernelGGL(k_update_f32, dim3(qs_vec_blocks(n)), dim3(NT), 0, s, n, x, F, Us, gm, nh, alpha, xnew);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_broyden_update_f64(int64_t n, const double *x, const double *F, const double *U,
+                                        const double *gamma, int nh, double alpha, double *xnew, hipStream_t s) {
+    if (nh > MAXH) return hipErrorInvalidValue;
+    Gam gm;
+    for (int k = 0; k < MAXH; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
+    hipLaunchKernelGGL(k_update_f64, dim3(qs_vec_blocks(n)), dim3(NT), 0, s, n, x, F, U, gm, nh, alpha, xnew);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_vec_stats(int64_t, const double *, const double *, double *, int, hipStream_t) {
+    return hipErrorNotSupported;
+}
+
+hipError_t qs_launch_fill(double *p, int64_t n, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3(qs_vec_blocks(n)), dim3(NT), 0, s, p, n, v);
+    return hipGetLastError();
+}

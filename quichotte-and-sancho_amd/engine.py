@@ -1,0 +1,234 @@
+"""ctypes binding of libqs_hip.so (include/qs_hip.h).
+
+ScfEngine mirrors the reference's SCF module interface for one bias point:
+  solve_scf(V, fresh)       src/scf.f90:78-151
+  current()                 src/observables.f90:22-39 (Current)
+  g_less_obs(), dosd()      the final-sweep observables (src/scf.f90:134-140)
+  init2(V), residual2(V,x)  the order-2 seams (src/scf.f90:116-122,158-188)
+  sigma()                   Sigma1 and Sigma2 of the last residual
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "lib", "libqs_hip.so")
+
+QS_ERRORS = {1: "singular matrix", 2: "bad argument", 3: "HIP error", 4: "out of device memory",
+             5: "unsupported size", 6: "bad state"}
+
+
+class QsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("qs error %d (%s): %s" % (code, QS_ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [
+        ("norb", ctypes.c_int32), ("n_omega", ctypes.c_int32),
+        ("w_init", ctypes.c_double), ("delta", ctypes.c_double),
+        ("H", ctypes.c_void_p), ("gammaL", ctypes.c_void_p), ("gammaR", ctypes.c_void_p),
+        ("U_site", ctypes.c_void_p), ("order", ctypes.c_int32),
+        ("beta", ctypes.c_double), ("mu", ctypes.c_double),
+        ("n_atoms", ctypes.c_int32), ("atom_norb", ctypes.c_void_p),
+        ("alpha_mix", ctypes.c_double), ("nhist", ctypes.c_int32), ("maxscf", ctypes.c_int32),
+        ("tolf", ctypes.c_double), ("tolf_order1", ctypes.c_double),
+        ("single_history", ctypes.c_int32), ("device", ctypes.c_int32),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("n_iter", ctypes.c_int32), ("fmax", ctypes.c_double),
+                ("runaway", ctypes.c_int32), ("unphysical", ctypes.c_int32), ("i_worst", ctypes.c_int32),
+                ("n_worst", ctypes.c_double), ("w_worst", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path=SO):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libqs_hip.so is not built (%s); run __graft_entry__.build()" % path)
+    L = ctypes.CDLL(path)
+    P, D, I32, I64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+    L.qs_create.argtypes = [P, ctypes.POINTER(P)]
+    L.qs_destroy.argtypes = [P]
+    L.qs_last_error.argtypes = [P]
+    L.qs_last_error.restype = ctypes.c_char_p
+    L.qs_sizes.argtypes = [P] + [P] * 6
+    L.qs_device_bytes.argtypes = [P]
+    L.qs_device_bytes.restype = I64
+    L.qs_solve_scf.argtypes = [P, D, I32, P, P, P]
+    L.qs_current.argtypes = [P, P, P]
+    L.qs_g_less_obs.argtypes = [P, P]
+    L.qs_dosd.argtypes = [P, P]
+    L.qs_trUD.argtypes = [P, P, P]
+    L.qs_init2.argtypes = [P, D, P]
+    L.qs_residual2.argtypes = [P, D, I64, P, P, P]
+    L.qs_get_sigma.argtypes = [P, P, P, P]
+    L.qs_final_sweep.argtypes = [P, D, P]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class ScfProblem:
+    """The inputs of solve_scf for one system (host arrays, reference conventions)."""
+    H: np.ndarray                 # (norb, norb) complex
+    gammaL: np.ndarray
+    gammaR: np.ndarray
+    U_site: np.ndarray            # (norb,), U on the spin-up index of each site
+    order: int
+    n_omega: int
+    w_init: float
+    delta: float
+    beta: float
+    mu: float
+    alpha_mix: float = 0.5
+    nhist: int = 16
+    maxscf: int = 300
+    tolf: float = 1e-6
+    tolf_order1: float = 1e-8
+    single_history: bool = True
+    atom_norb: np.ndarray = None
+    device: int = 0
+
+
+@dataclass
+class ScfResult:
+    converged: bool
+    n_iter: int
+    fmax: float
+    runaway: bool
+    unphysical: bool
+    i_worst: int
+    n_worst: float
+    w_worst: float
+    log_fmax: np.ndarray = field(default=None, repr=False)
+    log_rms: np.ndarray = field(default=None, repr=False)
+
+
+class ScfEngine:
+    def __init__(self, prob: ScfProblem):
+        L = load_library()
+        self.L = L
+        self.prob = prob
+        self._H = np.asfortranarray(prob.H, dtype=complex)
+        self._GL = np.asfortranarray(prob.gammaL, dtype=complex)
+        self._GR = np.asfortranarray(prob.gammaR, dtype=complex)
+        self._U = np.ascontiguousarray(prob.U_site, dtype=float)
+        self._an = None if prob.atom_norb is None else np.ascontiguousarray(prob.atom_norb, dtype=np.int32)
+        self.norb = self._H.shape[0]
+        p = _Problem()
+        p.norb = self.norb
+        p.n_omega = prob.n_omega
+        p.w_init = prob.w_init
+        p.delta = prob.delta
+        p.H = _ptr(self._H)
+        p.gammaL = _ptr(self._GL)
+        p.gammaR = _ptr(self._GR)
+        p.U_site = _ptr(self._U)
+        p.order = prob.order
+        p.beta = prob.beta
+        p.mu = prob.mu
+        p.n_atoms = 0 if self._an is None else len(self._an)
+        p.atom_norb = _ptr(self._an)
+        p.alpha_mix = prob.alpha_mix
+        p.nhist = prob.nhist
+        p.maxscf = prob.maxscf
+        p.tolf = prob.tolf
+        p.tolf_order1 = prob.tolf_order1
+        p.single_history = int(bool(prob.single_history))
+        p.device = prob.device
+        h = ctypes.c_void_p()
+        rc = L.qs_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise QsError(rc, "qs_create failed")
+        self.h = h
+        a = [ctypes.c_int32() for _ in range(4)]
+        nv, nf = ctypes.c_int64(), ctypes.c_int32()
+        L.qs_sizes(h, *[ctypes.byref(v) for v in a], ctypes.byref(nv), ctypes.byref(nf))
+        self.nA, self.nS, self.nCL, self.nAs = [v.value for v in a]
+        self.n_vec = nv.value
+        self.Nf = nf.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.qs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise QsError(rc, self.L.qs_last_error(self.h).decode())
+
+    def device_bytes(self):
+        return self.L.qs_device_bytes(self.h)
+
+    def solve_scf(self, V, fresh):
+        r = _Result()
+        m = max(self.prob.maxscf, 1)
+        lf, lr = np.zeros(m), np.zeros(m)
+        self._check(self.L.qs_solve_scf(self.h, float(V), int(bool(fresh)), ctypes.byref(r), _ptr(lf), _ptr(lr)))
+        return ScfResult(bool(r.converged), r.n_iter, r.fmax, bool(r.runaway), bool(r.unphysical), r.i_worst,
+                         r.n_worst, r.w_worst, lf[: r.n_iter].copy(), lr[: r.n_iter].copy())
+
+    def current(self):
+        u, d = ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.qs_current(self.h, ctypes.byref(u), ctypes.byref(d)))
+        return u.value, d.value
+
+    def g_less_obs(self):
+        G = np.zeros((self.norb, self.norb), complex, order="F")
+        self._check(self.L.qs_g_less_obs(self.h, _ptr(G)))
+        return G
+
+    def dosd(self):
+        d = np.zeros((self.norb, self.prob.n_omega), order="F")
+        self._check(self.L.qs_dosd(self.h, _ptr(d)))
+        return d
+
+    def trUD(self):
+        u, d = np.zeros(self.prob.n_omega), np.zeros(self.prob.n_omega)
+        self._check(self.L.qs_trUD(self.h, _ptr(u), _ptr(d)))
+        return u, d
+
+    def init2(self, V):
+        x = np.zeros(self.n_vec)
+        self._check(self.L.qs_init2(self.h, float(V), _ptr(x)))
+        return x
+
+    def residual2(self, V, x):
+        x = np.ascontiguousarray(x, dtype=float)
+        F = np.zeros(self.n_vec)
+        ab = ctypes.c_int32()
+        self._check(self.L.qs_residual2(self.h, float(V), self.n_vec, _ptr(x), _ptr(F), ctypes.byref(ab)))
+        return F, bool(ab.value)
+
+    def sigma(self):
+        n = self.nA * self.nA * self.prob.n_omega
+        r = np.zeros(n, complex)
+        l = np.zeros(n, complex)
+        s1 = np.zeros((self.norb, self.norb), complex, order="F")
+        self._check(self.L.qs_get_sigma(self.h, _ptr(r), _ptr(l), _ptr(s1)))
+        return r, l, s1
+
+    def final_sweep(self, V):
+        r = _Result()
+        self._check(self.L.qs_final_sweep(self.h, float(V), ctypes.byref(r)))
+        return r

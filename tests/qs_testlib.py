@@ -421,3 +421,36 @@ def assert_close_table(new, ref, tol_abs=1e-6, tol_rel=1e-5):
             worst = max(worst, err / lim)
     assert worst <= 1.0, "worst |new-ref|/tol = %g" % worst
     return worst
+
+
+# --------------------------------------------------------------------------
+# the product (HIP library) on the same problem
+# --------------------------------------------------------------------------
+def product():
+    import importlib
+    return importlib.import_module("quichotte-and-sancho_amd")
+
+
+def gpu_engine(prob, **over):
+    pkg = product()
+    kw = dict(H=prob.H, gammaL=prob.GL, gammaR=prob.GR, U_site=prob.U_site, order=prob.order,
+              n_omega=prob.n_omega, w_init=prob.w_init, delta=prob.delta, beta=prob.beta, mu=prob.mu,
+              alpha_mix=prob.alpha_mix, nhist=prob.nhist, maxscf=prob.maxscf, tolf=prob.tolf,
+              tolf_order1=prob.tolf_order1, single_history=prob.single_history)
+    kw.update(over)
+    return pkg.ScfEngine(pkg.ScfProblem(**kw))
+
+
+def random_problem(norb, n_omega, order=2, U=1.0, seed=0, gamma=0.3, **kw):
+    """Random Hermitian H with leads on the first and last site; an SOC-like
+    complex coupling makes every 2x2 spin block dense."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    H = 0.3 * (A + A.conj().T) / np.sqrt(norb)
+    GL, GR = leads(norb, gamma, 0.7 * gamma, gamma)
+    Us = np.zeros(norb)
+    Us[0::2] = U
+    ev = np.linalg.eigvalsh(H)
+    delta = (ev[-1] - ev[0] + 2.0) / n_omega
+    return Problem(H, GL, GR, Us, order, 300.0, 0.1, 0, 0, delta, [0.0], n_omega=n_omega,
+                   w_init=ev[0] - 1.0, **kw)

@@ -1,0 +1,130 @@
+"""HIP path against the CPU oracle on the same inputs (FP64 plain path).
+
+Tolerances (FP64 throughout; the oracle uses a 7-smooth FFT length and LAPACK's
+unblocked LU, the device a power-of-two FFT and Gauss-Jordan with the same
+pivot rule, so agreement is at rounding level, scaled by the data):
+  kernel seams (init2, residual2, Sigma2): max|gpu - oracle| <= 1e-11 * max|oracle|
+  converged observables: |gpu - oracle| <= 1e-10 |oracle| + 1e-12 (absolute floor
+  for J(V=0), which is pure round-off in the reference too)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from qs_testlib import (GOLDEN, Oracle, Problem, assert_close_table, gpu_engine, helix_problem, random_problem,
+                        read_table, spin_texture_rows)
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+@pytest.mark.parametrize("norb,nw", [(2, 64), (8, 200), (16, 300), (18, 128), (34, 96), (48, 64), (80, 40), (88, 24)])
+def test_init2_matches_oracle(norb, nw):
+    prob = random_problem(norb, nw, seed=norb)
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    assert g.n_vec == o.n_vec
+    xo = o.init2(0.3)
+    xg = g.init2(0.3)
+    assert rel_max(xg, xo) < 1e-11
+
+
+@pytest.mark.parametrize("norb,nw", [(4, 33), (8, 200), (16, 257), (80, 48)])
+def test_residual2_and_sigma_match_oracle(norb, nw):
+    prob = random_problem(norb, nw, seed=10 + norb, U=1.3)
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    V = -0.4
+    x = o.init2(V)
+    # a second, non-trivial input: one linear-mixing step
+    F0, _ = o.residual2(V, x)
+    x1 = x + 0.5 * F0
+    Fo, abo = o.residual2(V, x1)
+    Fg, abg = g.residual2(V, x1)
+    assert abo == abg
+    assert rel_max(Fg, Fo) < 1e-11
+    sro, slo = o.sigma2()
+    srg, slg, s1g = g.sigma()
+    assert rel_max(srg, sro) < 1e-11
+    assert rel_max(slg, slo) < 1e-11
+    s1o = o.sig1()
+    assert np.abs(s1g - s1o).max() <= 1e-12 * max(np.abs(s1o).max(), 1.0)
+
+
+def test_sigma2_nonpower_of_two_grid_and_noninteracting_site():
+    """7-smooth oracle length vs power-of-two device length, U = 0 on a contact site."""
+    prob = random_problem(8, 361, seed=5)
+    prob.U_site[2] = 0.0
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    x = o.init2(0.2)
+    Fo, _ = o.residual2(0.2, x)
+    Fg, _ = g.residual2(0.2, x)
+    assert rel_max(Fg, Fo) < 1e-11
+    sro, slo = o.sigma2()
+    srg, slg, _ = g.sigma()
+    assert rel_max(srg, sro) < 1e-11 and rel_max(slg, slo) < 1e-11
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_solve_scf_matches_oracle(order):
+    prob = helix_problem(4, 2, 1.0, order=order, n_omega=512, wlo=-4.0, whi=4.0, tolf=1e-10, tolf_order1=1e-12,
+                         nhist=8, voltages=[0.0, 0.3, 0.6])
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    fresh = True
+    for V in prob.voltages:
+        ro = o.solve(V, fresh)
+        rg = g.solve_scf(V, fresh)
+        assert ro["converged"] == rg.converged
+        assert ro["n_iter"] == rg.n_iter
+        fresh = not ro["converged"]
+        Jo, Jg = np.array(o.current()), np.array(g.current())
+        assert np.all(np.abs(Jg - Jo) <= 1e-10 * np.abs(Jo) + 1e-12), (Jg, Jo)
+        Go, Gg = o.g_less_obs(), g.g_less_obs()
+        assert np.abs(Gg - Go).max() <= 1e-10 * np.abs(Go).max()
+        assert rel_max(g.dosd(), o.dosd()) < 1e-10
+
+
+FIXTURES = ["order0_noninteracting", "order1_hartree_fock", "order2_second_order", "order2_lefthanded_siteU",
+            "order1_custom_template"]
+
+
+@pytest.mark.parametrize("case", FIXTURES)
+def test_gpu_reproduces_reference_fixture(case):
+    cd = os.path.join(GOLDEN, "cases", case)
+    prob = Problem.from_case(cd)
+    g = gpu_engine(prob)
+    o = Oracle(prob)
+    iv, pops = [], []
+    fresh = True
+    for V in prob.voltages:
+        r = g.solve_scf(V, fresh)
+        ro = o.solve(V, fresh)
+        fresh = (not prob.continuation) or (not r.converged)
+        ju, jd = g.current()
+        iv.append([V, ju, jd, ju + jd, 1 if r.converged else 0])
+        pops += spin_texture_rows(V, g.g_less_obs())
+        jo = np.array(o.current())
+        assert np.all(np.abs(np.array([ju, jd]) - jo) <= 1e-9 * np.abs(jo) + 1e-12)
+    assert_close_table(iv, read_table(os.path.join(cd, "reference", "Volt_Current_%d.dat" % prob.order)))
+    pf = os.path.join(cd, "reference", "populations.dat")
+    if os.path.exists(pf):
+        assert_close_table(pops, read_table(pf))
+
+
+def test_runaway_and_errors_are_reported():
+    import importlib
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    prob = random_problem(8, 64, seed=3)
+    g = gpu_engine(prob)
+    with pytest.raises(pkg.QsError):
+        g.residual2(0.0, np.zeros(g.n_vec + 1))
+    # a runaway input (huge occupations) aborts with F = 0, as scf_residual2
+    x = g.init2(0.0) * 1e6
+    F, ab = g.residual2(0.0, x)
+    assert ab and not F.any()
