@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the order-2 SCF path on one MI355X: converged bias points/s.
+
+One step = one bias point: solve_scf to max|F| < tolf (Broyden, FP32 history
+of 16), then the final sweep and the current (src/scf.f90:78-151,
+src/observables.f90:22-39).  Steps follow the voltage sweep with continuation,
+as the reference's voltage loop (src/quichotte.f90:125-143).
+
+Workloads (synthetic Hamiltonians from the model; no input files):
+  helix80 (default): helix N_ions = 8, N_turns = 5 -> 80 spin-orbitals,
+      hel_radius 1.4, hel_length 3.4, hand +1, E_CC 0, t_hop -1, SOC 0.01,
+      U = 2 on every site, T = 300 K, mu = 0, Gamma = 0.1 on the first and last
+      site, 2^16 energies on (-12, 12], V = -1 ... 1 in steps of 0.1, tolf 1e-8.
+  helix16: the same model with N_ions = 4, N_turns = 2 (16 spin-orbitals), U = 1,
+      2^14 energies on (-10, 10], V = 0 ... 1 in steps of 0.1.
+
+Printed: one JSON line with the metric, per-step latency percentiles, the
+roofline of the dominant kernel (algorithmic count per launch / average launch
+time from HIP events on the library's stream) and the CPU baseline (the FP64
+oracle timed on a bounded sample of the same workload on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+KB = 8.6173303e-5
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def helix_H(N_ions, N_turns, R=1.4, L=3.4, hand=1.0, E=0.0, t=-1.0, lamb=0.01):
+    """Helix model Hamiltonian (src/hamiltonian.f90:83-163)."""
+    nat = N_ions * N_turns
+    n = 2 * nat
+    H = np.zeros((n, n), complex)
+
+    def pos(i):
+        a = hand * (i - 1) * 2 * np.pi / N_ions
+        return np.array([R * np.cos(a), R * np.sin(a), L * (i - 1) / (N_turns * (N_ions - 1))])
+
+    for i in range(1, nat + 1):
+        ii = 2 * i
+        H[ii - 2, ii - 2] = E
+        H[ii - 1, ii - 1] = E
+        if i <= nat - 1:
+            H[ii - 2, ii] = t
+            H[ii - 1, ii + 1] = t
+        if i <= nat - 2:
+            r, r1, r2 = pos(i), pos(i + 1), pos(i + 2)
+            d1 = (r - r1) / np.linalg.norm(r - r1)
+            d2 = (r - r2) / np.linalg.norm(r - r2)
+            v = np.cross(d1, d2)
+            jj = 2 * (i + 2)
+            H[ii - 2, jj - 2] = 1j * lamb * v[2]
+            H[ii - 2, jj - 1] = 1j * lamb * (v[0] - 1j * v[1])
+            H[ii - 1, jj - 2] = 1j * lamb * (v[0] + 1j * v[1])
+            H[ii - 1, jj - 1] = -1j * lamb * v[2]
+    iu = np.triu_indices(n, 1)
+    H[(iu[1], iu[0])] = np.conj(H[iu])
+    return H
+
+
+def workload(name):
+    if name == "helix80":
+        w = dict(N_ions=8, N_turns=5, U=2.0, log2nw=16, wmax=12.0, volts=[round(-1.0 + 0.1 * k, 10) for k in range(21)])
+    elif name == "helix16":
+        w = dict(N_ions=4, N_turns=2, U=1.0, log2nw=14, wmax=10.0, volts=[round(0.1 * k, 10) for k in range(11)])
+    else:
+        raise SystemExit("unknown workload " + name)
+    H = helix_H(w["N_ions"], w["N_turns"])
+    n = H.shape[0]
+    GL = np.zeros((n, n), complex)
+    GR = np.zeros((n, n), complex)
+    for i in (0, 1):
+        GL[i, i] = 0.1
+        GR[n - 2 + i, n - 2 + i] = 0.1
+    U = np.zeros(n)
+    U[0::2] = w["U"]
+    nw = 1 << w["log2nw"]
+    delta = 2 * w["wmax"] / nw
+    prob = dict(H=H, gammaL=GL, gammaR=GR, U_site=U, order=2, n_omega=nw, w_init=-w["wmax"], delta=delta,
+                beta=1.0 / (KB * 300.0), mu=0.0, alpha_mix=0.5, nhist=16, maxscf=300, tolf=1e-8,
+                single_history=True)
+    desc = ("helix %dx%d (%d spin-orbitals), U=%g on every site, order 2, 2^%d energies on (-%g,%g], "
+            "V %g..%g step 0.1, tolf 1e-8, Broyden nhist 16 FP32 history"
+            % (w["N_ions"], w["N_turns"], n, w["U"], w["log2nw"], w["wmax"], w["wmax"], w["volts"][0], w["volts"][-1]))
+    return prob, w["volts"], desc
+
+
+def stage_counts(eng, nh):
+    """Algorithmic work per launch of each stage (flops or HBM bytes)."""
+    nw, norb, nA, nv = eng.prob.n_omega, eng.norb, eng.nA, eng.n_vec
+    return {
+        "dyson_keldysh": ("mfma", nw * (8.0 * norb ** 3 + 16.0 * nA ** 3), "flop"),
+        "sigma2_fft": ("hbm", 64.0 * nA * nA * nw, "byte"),
+        "broyden_history": ("hbm", nv * (32.0 + 4.0 * (nh - 1) + 8.0), "byte"),
+        "broyden_update": ("hbm", nv * (16.0 + 4.0 * nh + 8.0), "byte"),
+        "permute_in": ("hbm", nv * 8.0 + 32.0 * nA * nA * nw, "byte"),
+        "equal_time_sigma1": ("hbm", 8.0 * 4 * (nA // 2) * nw, "byte"),
+    }
+
+
+def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
+    """Oracle timed on bounded samples of one SCF iteration of the same workload."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import ctypes
+
+    from qs_testlib import _OrcProblem, oracle_lib, ptr
+
+    L = oracle_lib()
+    L.orc_bench_dyson.restype = ctypes.c_double
+    L.orc_bench_dyson.argtypes = [ctypes.c_void_p]
+    L.orc_bench_sigma2.restype = ctypes.c_double
+    L.orc_bench_sigma2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_bench_broyden.restype = ctypes.c_double
+    L.orc_bench_broyden.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+    n = prob["H"].shape[0]
+    nw = prob["n_omega"]
+    nA, nAs = n, n // 2
+    n_vec = 3 * nA * nA * nw
+    # Dyson/Keldysh: a sample of energies
+    e_sample = max(threads * 8, 64)
+    H = np.asfortranarray(prob["H"])
+    GL = np.asfortranarray(prob["gammaL"])
+    GR = np.asfortranarray(prob["gammaR"])
+    U = np.ascontiguousarray(prob["U_site"])
+    p = _OrcProblem()
+    p.norb, p.n_omega, p.w_init, p.delta = n, e_sample, prob["w_init"], prob["delta"]
+    p.H, p.gammaL, p.gammaR, p.U_site = ptr(H), ptr(GL), ptr(GR), ptr(U)
+    p.order, p.beta, p.mu = 2, prob["beta"], prob["mu"]
+    p.n_atoms, p.atom_norb = 0, None
+    p.alpha_mix, p.nhist, p.maxscf, p.tolf, p.tolf_order1 = 0.5, 16, 1, 1e-8, 1e-8
+    p.single_history, p.nthreads = 1, threads
+    t_dys = L.orc_bench_dyson(ctypes.byref(p)) / e_sample
+    # Sigma2: forward transforms of 2 nA^2 columns and nAs^2 site pairs
+    cols, pairs = max(threads * 4, 32), max(threads, 16)
+    t_s2 = L.orc_bench_sigma2(nw, cols, pairs, threads)
+    # per-column / per-pair split: time columns alone
+    t_cols = L.orc_bench_sigma2(nw, cols, 0, threads)
+    t_col = t_cols / cols
+    t_pair = max(t_s2 - t_cols, 0.0) / pairs
+    # Broyden + unpack on a sample of the vector
+    nb = 20_000_000
+    t_br = L.orc_bench_broyden(nb, 16, threads) / nb
+    t_iter = t_dys * nw + t_col * 2 * nA * nA + t_pair * nAs * nAs + t_br * n_vec
+    t_point = t_iter * n_iter_avg + t_dys * nw   # + the final sweep
+    sample = ("oracle (C, OpenMP %d threads) timed on %d energies of the Dyson/Keldysh sweep, %d zero-padded "
+              "length-%d FFT columns and %d site pairs of Sigma2, and one Broyden step on %d elements; scaled to "
+              "one SCF iteration (%.1f s) x %.1f iterations per bias point (the GPU run's mean) + the final sweep"
+              % (threads, e_sample, cols, 2 * nw, pairs, nb, t_iter, n_iter_avg))
+    return {"value": 1.0 / t_point, "unit": "bias points/s", "cores": threads, "kind": "port", "sample": sample,
+            "s_per_iteration": t_iter}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="helix80", choices=["helix80", "helix16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (one bias point is one "
+              "GPU's work; see the sharding notes in DESIGN.md)" % args.gpus, file=sys.stderr)
+        return 2
+    import importlib
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    prob, volts, desc = workload(args.workload)
+    nsteps = args.warmup + args.steps
+    if nsteps > len(volts):
+        raise SystemExit("at most %d steps for this workload" % len(volts))
+    print("bench: %s" % desc, file=sys.stderr, flush=True)
+    t0 = time.time()
+    eng = pkg.ScfEngine(pkg.ScfProblem(**prob))
+    print("bench: device memory %.1f GB, setup %.1f s" % (eng.device_bytes() / 1e9, time.time() - t0),
+          file=sys.stderr, flush=True)
+    fresh = True
+    for k in range(args.warmup):
+        t1 = time.time()
+        r = eng.solve_scf(volts[k], fresh)
+        fresh = not r.converged
+        print("bench: warmup V=%+.2f converged=%s steps=%d %.2f s" % (volts[k], r.converged, r.n_iter,
+                                                                      time.time() - t1), file=sys.stderr, flush=True)
+    eng.profile(True)
+    lat, iters, conv = [], [], []
+    tstart = time.perf_counter()
+    for k in range(args.warmup, nsteps):
+        t1 = time.perf_counter()
+        r = eng.solve_scf(volts[k], fresh)
+        J = eng.current()
+        lat.append(time.perf_counter() - t1)
+        fresh = not r.converged
+        iters.append(r.n_iter)
+        conv.append(r.converged)
+        print("bench: V=%+.2f converged=%s steps=%d J=(%.6e, %.6e) %.3f s" % (volts[k], r.converged, r.n_iter, J[0], J[1],
+                                                                              lat[-1]), file=sys.stderr, flush=True)
+    total = time.perf_counter() - tstart
+    prof = eng.profile_read()
+    eng.profile(False)
+    counts = stage_counts(eng, prob["nhist"])
+    dom = max((s for s in prof if s in counts and prof[s][1] > 0), key=lambda s: prof[s][0])
+    bound, work, kind = counts[dom]
+    ms, nl = prof[dom]
+    avg_s = ms / 1e3 / nl
+    if kind == "flop":
+        ach = work / avg_s / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+    else:
+        ach = work / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None}
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = avg_s * 1e3
+    stages = {s: {"ms": round(v[0], 3), "launches": v[1]} for s, v in prof.items() if v[1] > 0}
+    line = {
+        "metric": "converged bias points/s",
+        "value": args.steps / total,
+        "unit": "bias points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "form": "plain FP64 path, one bias point at a time",
+                   "n_spin_orbitals": eng.norb, "n_omega": eng.prob.n_omega, "fft_length": eng.Nf},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "scf_steps_per_point": float(np.mean(iters)),
+        "all_converged": bool(all(conv)),
+        "stages": stages,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline (oracle sample, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(prob, float(np.mean(iters)), args.cpu_threads)
+    eng.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

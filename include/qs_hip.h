@@ -109,6 +109,15 @@ int qs_get_sigma(const qs_handle *h, double *SigR2, double *SigL2, double *Sig1)
  * self-energies of the last residual (src/scf.f90:134-140) */
 int qs_final_sweep(qs_handle *h, double V, qs_scf_result *res);
 
+/* --- instrumentation (no counterpart in the reference, which only times the
+ * whole run in runtime_datasheet.dat, src/quichotte.f90:50-58,149-155) ------ */
+/* enable (1) or disable (0) per-stage HIP-event timing on the handle's stream;
+ * resets the totals */
+int qs_profile(qs_handle *h, int32_t enable);
+/* total milliseconds and launch counts per stage; returns the number of stages */
+int qs_profile_read(qs_handle *h, double *ms, int64_t *count, int32_t nstage);
+const char *qs_stage_name(int32_t stage);
+
 #ifdef __cplusplus
 }
 #endif

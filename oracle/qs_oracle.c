@@ -1116,3 +1116,152 @@ void orc_get_diag(const orc_state *s, double *gRd, double *gLd) {
     memcpy(gRd, s->gRd, sizeof(zc) * s->nA * s->nw);
     memcpy(gLd, s->gLd, sizeof(zc) * s->nA * s->nw);
 }
+
+/* ------------------------------------------------------------------------ */
+/* bounded samples of one order-2 SCF iteration, timed on the host cores      */
+/* (CPU baseline of the benchmark; each restates the reference's loop)        */
+/* ------------------------------------------------------------------------ */
+static double wall(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+/* sweep(M_SCF2) with Sigma present (src/scf.f90:280-355) over n_omega = p->n_omega
+ * energies; returns seconds */
+double orc_bench_dyson(const orc_problem *p) {
+    orc_state *s = orc_create(p);
+    if (!s || s->order != 2) { orc_destroy(s); return -1.0; }
+    size_t m = (size_t)s->nA * s->nA * s->nw;
+    s->SigR2 = zalloc(m);
+    s->SigL2 = zalloc(m);
+    double *F = (double *)calloc((size_t)s->n_vec, sizeof(double));
+    s->use_sigma = 1;
+    double t0 = wall();
+    sweep(s, M_SCF2, F);
+    double t = wall() - t0;
+    free(F);
+    orc_destroy(s);
+    return t;
+}
+
+/* sigma2_compute pieces (src/self_energies.f90:148-216) at FFT length
+ * good_fft_size(2 n_omega - 1): n_cols zero-padded forward transforms and
+ * n_pairs site pairs (2x2 algebra at every l + 8 backward transforms); seconds */
+double orc_bench_sigma2(int n_omega, int n_cols, int n_pairs, int nthreads) {
+    int Nf = orc_good_fft_size(2 * n_omega - 1);
+    fftplan pl; plan_make(&pl, Nf);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    zc *cols = zalloc((size_t)Nf * 16);
+    for (int q = 0; q < 16; q++)
+        for (int l = 0; l < Nf; l++) cols[(size_t)q * Nf + l] = cos(0.001 * l + q) + I * sin(0.002 * l * q);
+    double t0 = wall();
+#pragma omp parallel
+    {
+        zc *buf = zalloc(Nf), *scr = zalloc(Nf);
+#pragma omp for schedule(static)
+        for (int c = 0; c < n_cols; c++) {
+            for (int iw = 0; iw < n_omega; iw++) buf[iw] = cols[(size_t)(c % 16) * Nf + iw];
+            for (int iw = n_omega; iw < Nf; iw++) buf[iw] = 0;
+            fft_exec(&pl, -1, buf, scr);
+        }
+        zc *bufR = zalloc((size_t)Nf * 4), *bufL = zalloc((size_t)Nf * 4);
+#pragma omp for schedule(dynamic)
+        for (int pr = 0; pr < n_pairs; pr++) {
+            for (int l = 0; l < Nf; l++) {
+                int lm = (Nf - l) % Nf;
+                zc r_ij[4], L_ij[4], r_ji[4], L_ji[4], a_ij[4], a_ji[4], G_ij[4], G_ji[4];
+                for (int q = 0; q < 4; q++) {
+                    r_ij[q] = cols[(size_t)q * Nf + l]; L_ij[q] = cols[(size_t)(4 + q) * Nf + l];
+                    r_ji[q] = cols[(size_t)(8 + q) * Nf + lm]; L_ji[q] = cols[(size_t)(12 + q) * Nf + lm];
+                }
+                dag2(r_ji, a_ij); dag2(r_ij, a_ji);
+                for (int q = 0; q < 4; q++) { G_ij[q] = L_ij[q] + r_ij[q] - a_ij[q]; G_ji[q] = L_ji[q] + r_ji[q] - a_ji[q]; }
+                zc chi1[4], chi2[4], t1[4], t2[4], ad1[4], ad2[4], chiL[4], adL[4], sL[4];
+                mm2(L_ji, G_ij, chi1); mm2(L_ji, r_ij, t1); mm2(a_ji, L_ij, t2);
+                for (int q = 0; q < 4; q++) chi2[q] = t1[q] + t2[q];
+                adj2(chi1, ad1); adj2(chi2, ad2);
+                mm2(r_ij, ad1, t1); mm2(L_ij, ad2, t2);
+                for (int q = 0; q < 4; q++) bufR[(size_t)q * Nf + l] = t1[q] + t2[q];
+                mm2(G_ji, L_ij, chiL); adj2(chiL, adL); mm2(L_ij, adL, sL);
+                for (int q = 0; q < 4; q++) bufL[(size_t)q * Nf + l] = sL[q];
+            }
+            for (int q = 0; q < 4; q++) {
+                fft_exec(&pl, +1, bufR + (size_t)q * Nf, scr);
+                fft_exec(&pl, +1, bufL + (size_t)q * Nf, scr);
+            }
+        }
+        free(buf); free(scr); free(bufR); free(bufL);
+    }
+    double t = wall() - t0;
+    free(cols); free(pl.tw);
+    return t;
+}
+
+/* one steady-state step of broyden_solve with an FP32 history of nhist columns
+ * (common/broyden_mixing.f90:102-178) plus unpack_gf of the vector, on n
+ * elements; seconds */
+double orc_bench_broyden(int64_t n, int nhist, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    double *x = (double *)malloc(sizeof(double) * n), *F = (double *)malloc(sizeof(double) * n);
+    double *Fold = (double *)malloc(sizeof(double) * n), *xold = (double *)malloc(sizeof(double) * n);
+    float *dFs = (float *)malloc(sizeof(float) * n * nhist), *Us = (float *)malloc(sizeof(float) * n * nhist);
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; i++) {
+        x[i] = 1e-3 * (double)(i % 1000); F[i] = 1e-6 * (double)(i % 77); Fold[i] = 0.5 * F[i]; xold[i] = 0.9 * x[i];
+        for (int k = 0; k < nhist; k++) { dFs[(size_t)k * n + i] = 1e-3f * (float)k; Us[(size_t)k * n + i] = 1e-4f * (float)k; }
+    }
+    double cvec[64];
+    double t0 = wall();
+    /* unpack (a pure copy pass over x into two complex arrays of the same size) */
+    double *Gc = (double *)malloc(sizeof(double) * n);
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; i++) Gc[i] = x[i];
+    double fmax = 0, s2 = 0;
+#pragma omp parallel for reduction(max : fmax) reduction(+ : s2)
+    for (int64_t i = 0; i < n; i++) { double a = fabs(F[i]); if (a > fmax) fmax = a; s2 += F[i] * F[i]; }
+    double fn = 0;
+#pragma omp parallel for reduction(+ : fn)
+    for (int64_t i = 0; i < n; i++) { Fold[i] = F[i] - Fold[i]; fn += Fold[i] * Fold[i]; }
+    double rf = 1.0 / sqrt(fn + 1e-300);
+    int slot = 0;
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; i++) {
+        dFs[(size_t)slot * n + i] = (float)(Fold[i] * rf);
+        Us[(size_t)slot * n + i] = (float)(0.5 * Fold[i] * rf + (x[i] - xold[i]) * rf);
+    }
+    for (int k = 0; k < nhist; k++) {
+        double acc = 0;
+#pragma omp parallel for reduction(+ : acc)
+        for (int64_t i = 0; i < n; i++) acc += (double)dFs[(size_t)k * n + i] * (double)dFs[(size_t)slot * n + i];
+        cvec[k] = acc;
+    }
+    memcpy(xold, x, sizeof(double) * n);
+    memcpy(Fold, F, sizeof(double) * n);
+    for (int k = 0; k < nhist; k++) {
+        double acc = 0;
+#pragma omp parallel for reduction(+ : acc)
+        for (int64_t i = 0; i < n; i++) acc += (double)dFs[(size_t)k * n + i] * F[i];
+        cvec[k] = acc * 1e-9;
+    }
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; i++) {
+        double xi = x[i] + 0.5 * F[i];
+        for (int k = 0; k < nhist; k++) xi = xi - (double)Us[(size_t)k * n + i] * cvec[k];
+        x[i] = xi;
+    }
+    double t = wall() - t0;
+    free(x); free(F); free(Fold); free(xold); free(dFs); free(Us); free(Gc);
+    (void)fmax; (void)s2;
+    return t;
+}

@@ -101,6 +101,10 @@ double orc_fermi(double w, double V, double mu, double beta);         /* green_f
 
 /* broyden_solve (common/broyden_mixing.f90:51-184) on a user residual */
 typedef void (*orc_residual_fn)(void *ctx, int64_t n, const double *x, double *F, int32_t *abort_flag);
+double orc_bench_dyson(const orc_problem *p);
+double orc_bench_sigma2(int n_omega, int n_cols, int n_pairs, int nthreads);
+double orc_bench_broyden(int64_t n, int nhist, int nthreads);
+
 void orc_broyden(orc_residual_fn fn, void *ctx, double *x, int64_t n, double alpha_mix, int nhist,
                  int maxscf, double tolf, int single_history, int32_t *converged, int32_t *n_iter,
                  double *fmax, double *log_fmax, double *log_rms);

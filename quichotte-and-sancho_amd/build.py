@@ -31,6 +31,18 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for libqs_hip.so")
     os.replace(tmp, SO)
+    # the drop-in driver: plain C++ on the C ABI
+    exe = os.path.join(LIB, "quichotte_hip")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(REPO, "include"),
+           os.path.join(HERE, "host", "quichotte_hip.cpp"), "-L" + LIB, "-lqs_hip", "-Wl,-rpath,$ORIGIN",
+           "-o", exe + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for quichotte_hip")
+    os.replace(exe + ".tmp", exe)
     return SO
 
 

@@ -32,9 +32,18 @@ using zd = std::complex<double>;
 constexpr double PI = 3.14159265358979323846;
 constexpr int MAXH_DEV = 32;
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
+// stages timed with HIP events on the handle's stream (qs_profile)
+enum Stage { ST_EQT = 0, ST_PERMUTE, ST_SIGMA2, ST_DYSON, ST_REDUCE, ST_HIST, ST_UPDATE, ST_FINAL, ST_COUNT };
+const char *const kStageNames[ST_COUNT] = {"equal_time_sigma1", "permute_in",     "sigma2_fft",    "dyson_keldysh",
+                                           "reductions",        "broyden_history", "broyden_update", "final_sweep"};
+
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    struct Rec { int stage; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    double ms[ST_COUNT] = {0};
+    int64_t count[ST_COUNT] = {0};
 };
 
 }  // namespace
@@ -86,6 +95,7 @@ struct qs_handle {
     int runaway_final = 0, unphys_final = 0, iworst_final = 1;
     double nworst_final = 0.5, wworst_final = 1.0;
     bool final_valid = false;
+    Prof prof;
 };
 
 namespace {
@@ -105,6 +115,44 @@ int fail(qs_handle *h, int code, const char *fmt, ...) {
         hipError_t e_ = (call);                                                                          \
         if (e_ != hipSuccess) return fail((h), QS_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_));      \
     } while (0)
+
+hipEvent_t prof_event(qs_handle *h) {
+    if (!h->prof.pool.empty()) {
+        hipEvent_t e = h->prof.pool.back();
+        h->prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+// open a timed region on the handle's stream; returns the start event (or null)
+hipEvent_t prof_begin(qs_handle *h) {
+    if (!h->prof.on) return nullptr;
+    hipEvent_t a = prof_event(h);
+    if (a) (void)hipEventRecord(a, h->stream);
+    return a;
+}
+void prof_end(qs_handle *h, int stage, hipEvent_t a) {
+    if (!a) return;
+    hipEvent_t b = prof_event(h);
+    if (!b) return;
+    (void)hipEventRecord(b, h->stream);
+    h->prof.pending.push_back({stage, a, b});
+}
+// after a stream synchronisation: fold the pending intervals into the totals
+void prof_collect(qs_handle *h) {
+    for (auto &r : h->prof.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            h->prof.ms[r.stage] += ms;
+            h->prof.count[r.stage] += 1;
+        }
+        h->prof.pool.push_back(r.a);
+        h->prof.pool.push_back(r.b);
+    }
+    h->prof.pending.clear();
+}
 
 template <class T>
 int dalloc(qs_handle *h, T **ptr, size_t count) {
@@ -358,7 +406,9 @@ DysonArgs dyson_args(qs_handle *h, int mode, bool use_sigma, double V) {
 int run_dyson(qs_handle *h, int mode, bool use_sigma, double V) {
     DysonArgs a = dyson_args(h, mode, use_sigma, V);
     HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(int), h->stream));
+    hipEvent_t t0 = prof_begin(h);
     HIPCHK(h, qs_launch_dyson(a, h->nw, h->stream));
+    prof_end(h, mode == QS_M_FINAL ? ST_FINAL : ST_DYSON, t0);
     return QS_OK;
 }
 
@@ -366,6 +416,7 @@ int check_status(qs_handle *h) {
     int st = 0;
     HIPCHK(h, hipMemcpyAsync(&st, h->d_status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_collect(h);
     if (st & 1) return fail(h, QS_ERR_SINGULAR, "singular matrix in the Dyson inverse (zgetrf INFO != 0)");
     return QS_OK;
 }
@@ -412,17 +463,25 @@ int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, S
     const int nA = h->nA;
     h->V_scf = V;
     const double pp = h->p.delta / (2.0 * PI);
+    hipEvent_t t0 = prof_begin(h);
     HIPCHK(h, qs_launch_equal_time_sigma1(h->d_x, h->d_et_src, h->d_act_site, h->d_U_act, h->d_gt0, h->d_sig1, nA,
                                           h->nAs, h->nw, pp, h->d_scratch, h->stream));
+    prof_end(h, ST_EQT, t0);
+    t0 = prof_begin(h);
     HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, nA, h->nw, h->stream));
+    prof_end(h, ST_PERMUTE, t0);
     Sigma2Plan sp{};
     sp.nA = nA; sp.nAs = h->nAs; sp.nw = h->nw; sp.Nf = h->Nf; sp.N1 = h->N1; sp.N2 = h->N2;
     sp.log2N1 = h->log2N1; sp.log2N2 = h->log2N2; sp.delta = h->p.delta; sp.tw = h->d_tw; sp.Upair = h->d_Upair;
+    t0 = prof_begin(h);
     HIPCHK(h, qs_launch_sigma2(sp, h->d_Gt, h->d_Z, h->d_Gt, h->stream));
+    prof_end(h, ST_SIGMA2, t0);
     h->sigma_valid = true;
     int rc = run_dyson(h, QS_M_SCF2, true, V);
     if (rc) return rc;
+    t0 = prof_begin(h);
     HIPCHK(h, qs_launch_reduce_partials(h->d_partials, h->nw, h->d_red, h->stream));
+    prof_end(h, ST_REDUCE, t0);
     HIPCHK(h, hipMemcpyAsync(h->h_pin, h->d_red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if ((rc = check_status(h))) return rc;
     stats[0] = h->h_pin[0]; stats[1] = h->h_pin[1]; stats[2] = h->h_pin[2];
@@ -619,6 +678,7 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
         nh = std::min(nstored, nhist);
         if (nh > 0) need_proj = true;
         if (need_proj) {
+            hipEvent_t t0 = prof_begin(h);
             if (sp)
                 HIPCHK(h, qs_launch_broyden_hist_f32(n, h->d_F, h->d_Fold, h->d_x, h->d_xold, slot >= 0 ? 1.0 / fnorm : 0.0,
                                                      alpha, (float *)h->d_hist_dF, (float *)h->d_hist_U, slot, nh,
@@ -627,6 +687,7 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
                 HIPCHK(h, qs_launch_broyden_hist_f64(n, h->d_F, h->d_Fold, h->d_x, h->d_xold, fnorm, alpha,
                                                      (double *)h->d_hist_dF, (double *)h->d_hist_U, slot, nh,
                                                      h->d_scratch, nblocks, h->stream));
+            prof_end(h, ST_HIST, t0);
             // sum the per-block partials [nblocks][2 MAXH]
             HIPCHK(h, qs_launch_reduce_cols(h->d_scratch, 2 * MAXH_DEV, nblocks, h->d_sums,
                                             h->d_scratch + (size_t)nblocks * 2 * MAXH_DEV, h->stream));
@@ -652,12 +713,14 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
                 nh_use = 0;
             }
         }
+        hipEvent_t t1 = prof_begin(h);
         if (sp)
             HIPCHK(h, qs_launch_broyden_update_f32(n, h->d_x, h->d_F, (const float *)h->d_hist_U, gam.data(), nh_use,
                                                    alpha, h->d_xold, h->stream));
         else
             HIPCHK(h, qs_launch_broyden_update_f64(n, h->d_x, h->d_F, (const double *)h->d_hist_U, gam.data(), nh_use,
                                                    alpha, h->d_xold, h->stream));
+        prof_end(h, ST_UPDATE, t1);
         std::swap(h->d_x, h->d_xold);  // x <- x_new, xold <- x
         std::swap(h->d_F, h->d_Fold);  // Fold <- F
     }
@@ -872,6 +935,8 @@ void qs_destroy(qs_handle *h) {
     if (h->p.device >= 0) (void)hipSetDevice(h->p.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
+    for (auto &r : h->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
     if (h->h_pin) (void)hipHostFree(h->h_pin);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1098,5 +1163,25 @@ int qs_trUD(const qs_handle *hc, double *trU, double *trD) {
     }
     return QS_OK;
 }
+
+int qs_profile(qs_handle *h, int32_t enable) {
+    if (!h) return QS_ERR_ARG;
+    h->prof.on = enable != 0;
+    for (int i = 0; i < ST_COUNT; i++) { h->prof.ms[i] = 0.0; h->prof.count[i] = 0; }
+    return QS_OK;
+}
+
+int qs_profile_read(qs_handle *h, double *ms, int64_t *count, int32_t nstage) {
+    if (!h) return QS_ERR_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_collect(h);
+    for (int i = 0; i < nstage && i < ST_COUNT; i++) {
+        if (ms) ms[i] = h->prof.ms[i];
+        if (count) count[i] = h->prof.count[i];
+    }
+    return ST_COUNT;
+}
+
+const char *qs_stage_name(int32_t stage) { return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : ""; }
 
 }  // extern "C"

@@ -73,6 +73,10 @@ def load_library(path=SO):
     L.qs_residual2.argtypes = [P, D, I64, P, P, P]
     L.qs_get_sigma.argtypes = [P, P, P, P]
     L.qs_final_sweep.argtypes = [P, D, P]
+    L.qs_profile.argtypes = [P, I32]
+    L.qs_profile_read.argtypes = [P, P, P, I32]
+    L.qs_stage_name.argtypes = [I32]
+    L.qs_stage_name.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -227,6 +231,17 @@ class ScfEngine:
         s1 = np.zeros((self.norb, self.norb), complex, order="F")
         self._check(self.L.qs_get_sigma(self.h, _ptr(r), _ptr(l), _ptr(s1)))
         return r, l, s1
+
+    def profile(self, enable=True):
+        self._check(self.L.qs_profile(self.h, int(bool(enable))))
+
+    def profile_read(self):
+        ms = np.zeros(32)
+        cnt = np.zeros(32, dtype=np.int64)
+        n = self.L.qs_profile_read(self.h, _ptr(ms), _ptr(cnt), 32)
+        if n < 0:
+            raise QsError(-n, "qs_profile_read")
+        return {self.L.qs_stage_name(i).decode(): (ms[i], int(cnt[i])) for i in range(n)}
 
     def final_sweep(self, V):
         r = _Result()
