@@ -189,8 +189,122 @@ struct Params {
     bool single_history = true, continuation = true;
     int Norb = 0, n_volt_steps = 0;
     double beta = 0;
-    std::vector<int> atom_norb;
+    int N_atoms = 0;
+    std::vector<int> atom_norb, atom_first;
+    std::vector<std::string> atom_elem, elem_name;
+    std::vector<int> elem_lmax;
+    std::vector<double> elem_xi;
 };
+
+// 'BLOCK n | Atom_I: a Atom_J: b | Dist: d A | Dim: 8x8' -> a, b, 8, 8
+static bool parse_block_header(const std::string &line, int &ai, int &aj, int &di, int &dj) {
+    size_t k1 = line.find("Atom_I:"), k2 = line.find("Atom_J:"), k3 = line.find("Dim:");
+    if (k1 == std::string::npos || k2 == std::string::npos || k3 == std::string::npos) return false;
+    if (sscanf(line.c_str() + k1 + 7, "%d", &ai) != 1) return false;
+    if (sscanf(line.c_str() + k2 + 7, "%d", &aj) != 1) return false;
+    size_t kx = line.find('x', k3);
+    if (kx == std::string::npos) return false;
+    if (sscanf(line.substr(k3 + 4, kx - k3 - 4).c_str(), "%d", &di) != 1) return false;
+    if (sscanf(line.c_str() + kx + 1, "%d", &dj) != 1) return false;
+    return ai >= 1 && aj >= 1 && di >= 1 && dj >= 1;
+}
+
+// atoms and orbitals per atom from the BLOCK headers (src/hamiltonian.f90:349-396)
+static void dftb_blocks_scan(const std::string &path, Params &P) {
+    std::ifstream f(path);
+    if (!f.good()) die("cannot open ham_file %s", path.c_str());
+    std::string line;
+    std::vector<std::array<int, 4>> hdr;
+    int nat = 0;
+    while (std::getline(f, line)) {
+        std::string t = line.substr(std::min(line.find_first_not_of(" \t"), line.size()));
+        if (t.rfind("BLOCK", 0) != 0) continue;
+        int ai, aj, di, dj;
+        if (!parse_block_header(line, ai, aj, di, dj)) die("cannot read block header in %s: %s", path.c_str(), line.c_str());
+        if (di % 2 || dj % 2) die("block dimensions must be even (orbitals x 2 spins): %s", line.c_str());
+        nat = std::max({nat, ai, aj});
+        hdr.push_back({ai, aj, di, dj});
+    }
+    std::vector<int> norb(nat, 0);
+    auto setn = [&](int a, int n) {
+        if (norb[a - 1] != 0 && norb[a - 1] != n) die("atom %d has inconsistent block dimensions in %s", a, path.c_str());
+        norb[a - 1] = n;
+    };
+    for (auto &h : hdr) { setn(h[0], h[2] / 2); setn(h[1], h[3] / 2); }
+    if (nat < 1 || std::count(norb.begin(), norb.end(), 0) > 0)
+        die("%s has no BLOCK for some atom (every atom needs its on-site block)", path.c_str());
+    P.N_atoms = nat;
+    P.atom_norb = norb;
+}
+
+static std::string element_case(std::string e) {
+    e = Input::trim(e);
+    if (e.size() > 2) e = e.substr(0, 2);
+    if (!e.empty() && e[0] >= 'a' && e[0] <= 'z') e[0] = (char)(e[0] - 32);
+    if (e.size() > 1 && e[1] >= 'A' && e[1] <= 'Z') e[1] = (char)(e[1] + 32);
+    return e;
+}
+
+static std::vector<std::pair<std::string, std::string>> split_pairs(const std::string &str, const char *key) {
+    std::string s = str;
+    for (auto &c : s)
+        if (c == ',') c = ' ';
+    std::stringstream ss(s);
+    std::string tok;
+    std::vector<std::pair<std::string, std::string>> out;
+    while (ss >> tok) {
+        size_t c = tok.find(':');
+        if (c == std::string::npos || c < 1)
+            die("%s: entries must look like Element:value, got \"%s\"", key, tok.c_str());
+        out.push_back({tok.substr(0, c), tok.substr(c + 1)});
+    }
+    if (out.empty()) die("%s is empty", key);
+    return out;
+}
+
+// elements of xyz_file, shells from max_angular_momentum, xi from soc_xi (src/input_output.f90:185-238)
+static void setup_dftb_elements(Params &P) {
+    std::ifstream f(P.xyz_file);
+    if (!f.good()) die("cannot open xyz_file %s", P.xyz_file.c_str());
+    int nat = 0;
+    if (!(f >> nat) || nat < 1) die("first line of %s must be the number of atoms", P.xyz_file.c_str());
+    std::string line;
+    std::getline(f, line);
+    std::getline(f, line);
+    P.atom_elem.clear();
+    for (int a = 0; a < nat; a++) {
+        if (!std::getline(f, line)) die("%s has fewer than %d atom lines", P.xyz_file.c_str(), nat);
+        std::stringstream ls(line);
+        std::string el;
+        ls >> el;
+        P.atom_elem.push_back(element_case(el));
+    }
+    for (auto &pr : split_pairs(P.max_l, "max_angular_momentum")) {
+        P.elem_name.push_back(element_case(pr.first));
+        std::string v = Input::trim(pr.second);
+        if (v == "s" || v == "S") P.elem_lmax.push_back(0);
+        else if (v == "p" || v == "P") P.elem_lmax.push_back(1);
+        else die("max_angular_momentum: only s and p shells are supported, not \"%s\" for %s", v.c_str(), pr.first.c_str());
+        P.elem_xi.push_back(0.0);
+    }
+    if (!Input::trim(P.soc_xi).empty())
+        for (auto &pr : split_pairs(P.soc_xi, "soc_xi")) {
+            auto it = std::find(P.elem_name.begin(), P.elem_name.end(), element_case(pr.first));
+            double xv;
+            if (it == P.elem_name.end() || !Input::parse_real(pr.second, xv))
+                die("soc_xi: bad entry %s:%s (element must also appear in max_angular_momentum)", pr.first.c_str(),
+                    pr.second.c_str());
+            P.elem_xi[it - P.elem_name.begin()] = xv;
+        }
+    P.N_atoms = nat;
+    P.atom_norb.assign(nat, 1);
+    for (int a = 0; a < nat; a++) {
+        auto it = std::find(P.elem_name.begin(), P.elem_name.end(), P.atom_elem[a]);
+        if (it == P.elem_name.end())
+            die("element %s of %s is missing from max_angular_momentum", P.atom_elem[a].c_str(), P.xyz_file.c_str());
+        P.atom_norb[a] = P.elem_lmax[it - P.elem_name.begin()] == 1 ? 4 : 1;
+    }
+}
 
 static int xyz_site_count(const std::string &fn) {
     std::ifstream f(fn);
@@ -285,9 +399,15 @@ static Params read_input() {
     int n_model = -1;
     if (P.model == "helix") n_model = P.N_ions * P.N_turns;
     else if (P.model == "xyz") n_model = xyz_site_count(P.xyz_file);
-    else if (P.model == "dftb" || P.model == "dftb_hs")
-        die("hamiltonian = %s is not built into this driver (use helix, peptide, xyz or custom)", P.model.c_str());
-    else if (P.N_sites < 1) {
+    else if (P.model == "dftb") {
+        dftb_blocks_scan(P.ham_file, P);
+        n_model = 0;
+        for (int a : P.atom_norb) n_model += a;
+    } else if (P.model == "dftb_hs") {
+        setup_dftb_elements(P);
+        n_model = 0;
+        for (int a : P.atom_norb) n_model += a;
+    } else if (P.N_sites < 1) {
         fprintf(stderr, " ERROR: hamiltonian = %s needs the key N_sites\n", P.model.c_str());
         fprintf(stderr, "        (built-in models: helix, xyz, custom)\n");
         exit(1);
@@ -301,7 +421,12 @@ static Params read_input() {
     }
     P.beta = 1.0 / (KB * P.T);
     P.Norb = 2 * P.N_sites;
-    P.atom_norb.assign(P.N_sites, 1);
+    if (P.atom_norb.empty()) {  // one orbital per atom (set_atom_map, src/input_output.f90:288-308)
+        P.N_atoms = P.N_sites;
+        P.atom_norb.assign(P.N_sites, 1);
+    }
+    P.atom_first.assign(P.N_atoms, 0);
+    for (int a = 1; a < P.N_atoms; a++) P.atom_first[a] = P.atom_first[a - 1] + P.atom_norb[a - 1];
     P.n_volt_steps = (int)std::nearbyint((P.Vf - P.V) / P.delv);
 
     // check_input (src/input_output.f90:396-430)
@@ -328,7 +453,7 @@ static Params read_input() {
     if (P.tolf_order1 <= 0) complain("tolf_order1 must be > 0");
     if (badin) exit(1);
     printf(" Input read from %s\n", in.file.c_str());
-    printf(" Hamiltonian: %s   N_sites (orbitals): %d   N_atoms: %d\n", P.model.c_str(), P.N_sites, P.N_sites);
+    printf(" Hamiltonian: %s   N_sites (orbitals): %d   N_atoms: %d\n", P.model.c_str(), P.N_sites, P.N_atoms);
     return P;
 }
 
@@ -485,6 +610,171 @@ static Mat build_peptide(const Params &P) {
     return H;
 }
 
+// dftb: orthogonal spin-orbit Hamiltonian from soc_dftb blocks (src/hamiltonian.f90:398-475)
+static Mat build_dftb(const Params &P) {
+    Mat H(P.Norb);
+    std::ifstream f(P.ham_file);
+    if (!f.good()) die("cannot open ham_file %s", P.ham_file.c_str());
+    std::vector<char> onsite(P.N_atoms, 0);
+    std::string line;
+    int nblk = 0;
+    while (std::getline(f, line)) {
+        std::string t = line.substr(std::min(line.find_first_not_of(" \t"), line.size()));
+        if (t.rfind("BLOCK", 0) != 0) continue;
+        int ai, aj, di, dj;
+        if (!parse_block_header(line, ai, aj, di, dj) || ai > P.N_atoms || aj > P.N_atoms ||
+            di != 2 * P.atom_norb[ai - 1] || dj != 2 * P.atom_norb[aj - 1])
+            die("bad block header in %s: %s", P.ham_file.c_str(), line.c_str());
+        const int r0 = 2 * P.atom_first[ai - 1], c0 = 2 * P.atom_first[aj - 1];
+        for (int r = 1; r <= di; r++)
+            for (int c = 1; c <= dj; c++) {
+                std::string dl;
+                double re, imv;
+                if (!std::getline(f, dl) || sscanf(dl.c_str(), "%lf %lf", &re, &imv) != 2)
+                    die("block too short in %s: %s", P.ham_file.c_str(), line.c_str());
+                H1(H, r0 + r, c0 + c) = zd(re, imv);
+            }
+        if (ai == aj) onsite[ai - 1] = 1;
+        nblk++;
+    }
+    for (char o : onsite)
+        if (!o) die("%s is missing the on-site block of some atom", P.ham_file.c_str());
+    double herm = 0;
+    Mat Hs(P.Norb);
+    for (int i = 1; i <= P.Norb; i++)
+        for (int j = 1; j <= P.Norb; j++) {
+            herm = std::max(herm, std::abs(H1(H, i, j) - std::conj(H1(H, j, i))));
+            H1(Hs, i, j) = 0.5 * (H1(H, i, j) + std::conj(H1(H, j, i)));
+        }
+    printf(" DFTB Hamiltonian: %d blocks, %d atoms, %d orbitals\n", nblk, P.N_atoms, P.N_sites);
+    printf(" max |H - H^dagger| in file = %10.2E eV (H symmetrised)\n", herm);
+    return Hs;
+}
+
+static void hermitian_eig(const Mat &H, std::vector<double> &ev, Mat &V);
+
+// a DFTB+ square matrix file: '#' lines, 'T NALLORB NKPOINT', k-point index, n rows of n numbers
+static std::vector<double> read_dftb_square(const std::string &path, int n) {
+    std::ifstream f(path);
+    if (!f.good()) die("cannot open %s", path.c_str());
+    auto next_data = [&](std::string &l) {
+        while (std::getline(f, l)) {
+            std::string t = Input::trim(l);
+            if (!t.empty() && t[0] != '#') return;
+        }
+        die("unexpected end of %s", path.c_str());
+    };
+    std::string l;
+    next_data(l);
+    std::stringstream hs(l);
+    std::string flag;
+    int nall = -1, nk = -1;
+    hs >> flag >> nall >> nk;
+    if (nall != n) die("%s has %d orbitals; the geometry and shells give %d", path.c_str(), nall, n);
+    if (nk != 1) die("%s: only one k-point (a molecule) is supported", path.c_str());
+    next_data(l);  // k-point index
+    std::vector<double> A((size_t)n * n);
+    for (int i = 0; i < n; i++) {
+        next_data(l);
+        std::stringstream rs(l);
+        for (int j = 0; j < n; j++) {
+            std::string tok;
+            if (!(rs >> tok) || !Input::parse_real(tok, A[(size_t)j * n + i])) die("%s: cannot read row %d", path.c_str(), i + 1);
+        }
+    }
+    return A;
+}
+
+// dftb_hs: H = (S^-1/2 x 1)[H0 x 1 + H_SOC](S^-1/2 x 1) (src/hamiltonian.f90:611-708)
+static Mat build_dftb_hs(const Params &P) {
+    const double hartree = 27.211386245988;
+    const int N = P.N_sites, n = P.Norb;
+    std::vector<double> S = read_dftb_square(P.overlap_file, N), H0 = read_dftb_square(P.ham_file, N);
+    Mat Sm(N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            Sm.a[(size_t)j * N + i] = 0.5 * (S[(size_t)j * N + i] + S[(size_t)i * N + j]);
+        }
+    std::vector<double> Ssym((size_t)N * N), H0s((size_t)N * N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            Ssym[(size_t)j * N + i] = Sm.a[(size_t)j * N + i].real();
+            H0s[(size_t)j * N + i] = 0.5 * (H0[(size_t)j * N + i] + H0[(size_t)i * N + j]) * hartree;
+        }
+    std::vector<double> ev;
+    Mat U(N);
+    hermitian_eig(Sm, ev, U);
+    if (ev.empty() || *std::min_element(ev.begin(), ev.end()) <= 1e-8)
+        die("overlap matrix is not positive definite (min eigenvalue %g)", ev.empty() ? 0.0 : ev[0]);
+    std::vector<double> Smh((size_t)N * N, 0.0);  // U diag(s^-1/2) U^T (U real up to phases)
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            zd acc = 0;
+            for (int k = 0; k < N; k++) acc += U.a[(size_t)k * N + i] * std::conj(U.a[(size_t)k * N + j]) / std::sqrt(ev[k]);
+            Smh[(size_t)j * N + i] = acc.real();
+        }
+    printf(" DFTB H0/S: smallest overlap eigenvalue = %10.2E\n", ev[0]);
+    // Lambda = sum_atoms xi L.S on the p shell (L_k)_ab = -i eps_kab, DFTB+ order s, p_y, p_z, p_x
+    const zd I(0, 1);
+    auto Lk = [&](int k, int a, int b) -> zd {  // 1-based cartesian indices
+        if (k == 1) return (a == 2 && b == 3) ? -I : (a == 3 && b == 2) ? I : zd(0);
+        if (k == 2) return (a == 3 && b == 1) ? -I : (a == 1 && b == 3) ? I : zd(0);
+        return (a == 1 && b == 2) ? -I : (a == 2 && b == 1) ? I : zd(0);
+    };
+    auto sig = [&](int s1, int s2, int k) -> zd {
+        if (k == 1) return (s1 != s2) ? zd(1) : zd(0);
+        if (k == 2) return (s1 == 1 && s2 == 2) ? -I : (s1 == 2 && s2 == 1) ? I : zd(0);
+        return (s1 == s2) ? zd(s1 == 1 ? 1.0 : -1.0) : zd(0);
+    };
+    const int pidx[3] = {2, 3, 1};
+    Mat Lam(n);
+    for (int a = 0; a < P.N_atoms; a++) {
+        if (P.atom_norb[a] < 4) continue;
+        auto it = std::find(P.elem_name.begin(), P.elem_name.end(), P.atom_elem[a]);
+        const double xi = P.elem_xi[it - P.elem_name.begin()];
+        if (xi == 0.0) continue;
+        const int i0 = P.atom_first[a] + 1;  // 1-based site of the s orbital
+        for (int p = 1; p <= 3; p++)
+            for (int q = 1; q <= 3; q++)
+                for (int s1 = 1; s1 <= 2; s1++)
+                    for (int s2 = 1; s2 <= 2; s2++) {
+                        const int i = 2 * (i0 + p - 1) + s1, j = 2 * (i0 + q - 1) + s2;
+                        for (int k = 1; k <= 3; k++) H1(Lam, i, j) += xi * Lk(k, pidx[p - 1], pidx[q - 1]) * sig(s1, s2, k) / 2.0;
+                    }
+    }
+    Mat Sx2(n), Sm2(n), Hao(n);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+            for (int s1 = 1; s1 <= 2; s1++) {
+                H1(Sx2, 2 * i + s1, 2 * j + s1) = Ssym[(size_t)j * N + i];
+                H1(Sm2, 2 * i + s1, 2 * j + s1) = Smh[(size_t)j * N + i];
+                H1(Hao, 2 * i + s1, 2 * j + s1) = H0s[(size_t)j * N + i];
+            }
+    auto mm = [&](const Mat &A, const Mat &B) {
+        Mat C(n);
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++) {
+                const zd b = B.a[(size_t)j * n + k];
+                if (b == 0.0) continue;
+                for (int i = 0; i < n; i++) C.a[(size_t)j * n + i] += A.a[(size_t)k * n + i] * b;
+            }
+        return C;
+    };
+    if (P.soc_form == "dual") {
+        Mat A = mm(Sx2, Lam), B = mm(Lam, Sx2);
+        for (size_t q = 0; q < Hao.a.size(); q++) Hao.a[q] += 0.5 * (A.a[q] + B.a[q]);
+    } else if (P.soc_form == "onsite") {
+        for (size_t q = 0; q < Hao.a.size(); q++) Hao.a[q] += Lam.a[q];
+    } else {
+        die("soc_form must be dual or onsite, not %s", P.soc_form.c_str());
+    }
+    Mat Ht = mm(Sm2, mm(Hao, Sm2)), H(n);
+    for (int i = 1; i <= n; i++)
+        for (int j = 1; j <= n; j++) H1(H, i, j) = 0.5 * (H1(Ht, i, j) + std::conj(H1(Ht, j, i)));
+    printf(" DFTB H0/S Hamiltonian: %d atoms, %d orbitals, SOC form %s\n", P.N_atoms, N, P.soc_form.c_str());
+    return H;
+}
+
 // ---------------------------------------------------------------------------
 // Hermitian eigen-decomposition by cyclic complex Jacobi (setup only: the grid
 // bounds of src/quichotte.f90:81-86 and info_Hamiltonian.dat)
@@ -607,9 +897,11 @@ int main() {
     else if (P.model == "xyz") H = build_xyz(P);
     else if (P.model == "custom") H = build_custom(P);
     else if (P.model == "peptide") H = build_peptide(P);
+    else if (P.model == "dftb") H = build_dftb(P);
+    else if (P.model == "dftb_hs") H = build_dftb_hs(P);
     else {
         printf(" ERROR: unknown hamiltonian = %s\n", P.model.c_str());
-        printf("        built in: helix, peptide, xyz, custom\n");
+        printf("        built in: helix, peptide, xyz, dftb, dftb_hs, custom\n");
         exit(1);
     }
     const int n = P.Norb;
@@ -617,21 +909,32 @@ int main() {
         FILE *g = fopen("hamiltonian_matrix.dat", "w");
         fprintf(g, "# QUICHOTTE Hamiltonian [eV]; state index 2*site-1 = up, 2*site = down\n# Norb = %d\n", n);
         fprintf(g, "#   i   j   Re H_ij   Im H_ij\n");
+        // elements that vanish by symmetry (e.g. the on-site spin flip of an s orbital after the
+        // Lowdin transform) come out as rounding noise of ~1e-17 eV here; they are written as zeros
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++)
-                if (H1(H, i, j) != 0.0) fprintf(g, "%6d%6d%25.16E%25.16E\n", i, j, H1(H, i, j).real(), H1(H, i, j).imag());
+                if (std::abs(H1(H, i, j)) > 1e-12)
+                    fprintf(g, "%6d%6d%25.16E%25.16E\n", i, j, H1(H, i, j).real(), H1(H, i, j).imag());
         fclose(g);
     }
-    // leads (src/input_output.f90:310-346): one orbital per atom for these models
-    const int n_atoms = P.N_sites;
+    // leads on every orbital of the contact atoms (src/input_output.f90:310-346)
+    const int n_atoms = P.N_atoms;
     std::vector<int> left = parse_atom_list(P.lead_left, 1, "lead_left_atoms", n_atoms);
     std::vector<int> right = parse_atom_list(P.lead_right, n_atoms, "lead_right_atoms", n_atoms);
     for (int a : left)
         for (int b : right)
             if (a == b) die("atom %d is in both lead_left_atoms and lead_right_atoms", a);
     Mat GL(n), GR(n);
-    for (int a : left) { H1(GL, 2 * a - 1, 2 * a - 1) = P.GammaL_up; H1(GL, 2 * a, 2 * a) = P.GammaL_down; }
-    for (int a : right) { H1(GR, 2 * a - 1, 2 * a - 1) = P.GammaR; H1(GR, 2 * a, 2 * a) = P.GammaR; }
+    for (int a : left)
+        for (int i = P.atom_first[a - 1] + 1; i <= P.atom_first[a - 1] + P.atom_norb[a - 1]; i++) {
+            H1(GL, 2 * i - 1, 2 * i - 1) = P.GammaL_up;
+            H1(GL, 2 * i, 2 * i) = P.GammaL_down;
+        }
+    for (int a : right)
+        for (int i = P.atom_first[a - 1] + 1; i <= P.atom_first[a - 1] + P.atom_norb[a - 1]; i++) {
+            H1(GR, 2 * i - 1, 2 * i - 1) = P.GammaR;
+            H1(GR, 2 * i, 2 * i) = P.GammaR;
+        }
     // Hubbard U (src/input_output.f90:432-462)
     std::vector<double> U(n, 0.0);
     if (P.hubbard_file.empty()) {
@@ -691,8 +994,8 @@ int main() {
     qp.order = P.order;
     qp.beta = P.beta;
     qp.mu = P.mu;
-    qp.n_atoms = 0;
-    qp.atom_norb = nullptr;
+    qp.n_atoms = P.N_atoms;
+    qp.atom_norb = P.atom_norb.data();
     qp.alpha_mix = P.alpha_mix;
     qp.nhist = P.nhist;
     qp.maxscf = P.maxscf;
@@ -706,6 +1009,8 @@ int main() {
     char vfn[32];
     snprintf(vfn, sizeof vfn, "Volt_Current_%d.dat", P.order);
     FILE *fiv = fopen(vfn, "w"), *fpop = fopen("populations.dat", "w"), *fspec = fopen("spectral_function.dat", "w");
+    const bool per_atom = *std::max_element(P.atom_norb.begin(), P.atom_norb.end()) > 1;
+    FILE *fpa = per_atom ? fopen("populations_atoms.dat", "w") : nullptr;
     int n_unconv = 0;
     bool fresh = true;
     std::vector<double> lf(std::max(P.maxscf, 1)), lr(std::max(P.maxscf, 1)), dosd((size_t)n * n_omega);
@@ -759,6 +1064,41 @@ int main() {
                     fr(up.real()).c_str(), fr((up + dn).real()).c_str());
         }
         fflush(fpop);
+        // per-atom S, populations and L over the p shell (src/observables.f90:104-158)
+        if (fpa) {
+            const int cart[3] = {2, 3, 1};
+            auto Lk = [&](int k, int a, int b) -> zd {  // (L_k)_ab = -i eps_kab, 1-based
+                const int kp1 = k % 3 + 1, kp2 = (k + 1) % 3 + 1;
+                if (a == kp1 && b == kp2) return -I;
+                if (a == kp2 && b == kp1) return I;
+                return 0.0;
+            };
+            for (int at = 0; at < P.N_atoms; at++) {
+                double Sv[3] = {0, 0, 0}, Lv[3] = {0, 0, 0}, nu = 0, nd = 0;
+                for (int o = P.atom_first[at]; o < P.atom_first[at] + P.atom_norb[at]; o++) {
+                    zd rho[2][2];
+                    for (int s1 = 0; s1 < 2; s1++)
+                        for (int s2 = 0; s2 < 2; s2++) rho[s1][s2] = -I * g(2 * o + s1, 2 * o + s2);
+                    nu += rho[0][0].real();
+                    nd += rho[1][1].real();
+                    // 0.5 Re sum(sigma * transpose(rho))
+                    Sv[0] += 0.5 * (rho[1][0] + rho[0][1]).real();
+                    Sv[1] += 0.5 * (-I * rho[1][0] + I * rho[0][1]).real();
+                    Sv[2] += 0.5 * (rho[0][0] - rho[1][1]).real();
+                }
+                if (P.atom_norb[at] == 4)
+                    for (int p = 1; p <= 3; p++)
+                        for (int q = 1; q <= 3; q++) {
+                            const int ip = P.atom_first[at] + p, iq = P.atom_first[at] + q;  // 0-based sites
+                            const zd rhop = -I * (g(2 * iq, 2 * ip) + g(2 * iq + 1, 2 * ip + 1));
+                            for (int k = 1; k <= 3; k++) Lv[k - 1] += (Lk(k, cart[p - 1], cart[q - 1]) * rhop).real();
+                        }
+                fprintf(fpa, " %s %s %s %s %s %s %s %s %s %s %s\n", fr(V1).c_str(), fi(at + 1).c_str(), fr(Sv[0]).c_str(),
+                        fr(Sv[1]).c_str(), fr(Sv[2]).c_str(), fr(nd).c_str(), fr(nu).c_str(), fr(nd + nu).c_str(),
+                        fr(Lv[0]).c_str(), fr(Lv[1]).c_str(), fr(Lv[2]).c_str());
+            }
+            fflush(fpa);
+        }
         // spectral function (src/observables.f90:89-102)
         qs_dosd(h, dosd.data());
         fprintf(fspec, "# V = %10.5f   columns: V, omega, A_ii for i = 1..Norb\n", V1);
@@ -774,6 +1114,7 @@ int main() {
     fclose(fiv);
     fclose(fpop);
     fclose(fspec);
+    if (fpa) fclose(fpa);
     qs_destroy(h);
     const double mins = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() / 60.0;
     fprintf(frt, " <<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<<\n Total Runtime: %s mins\n", fr(mins).c_str());

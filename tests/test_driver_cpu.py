@@ -99,3 +99,21 @@ def test_lead_overlap_is_rejected(tmp_path):
 def test_hubbard_file_too_short(tmp_path):
     r = run_in(tmp_path, {"input.dat": BASE + "hubbard_file = U.dat\n", "U.dat": "0.8\n0.6\n"})
     assert r.returncode == 1 and "one U per site" in r.stderr
+
+
+def test_dftb_hs_hamiltonian_matches_reference(tmp_path):
+    """H = S^-1/2 (H0 + SOC) S^-1/2 built by the driver before the device is
+    used, against the reference's hamiltonian_matrix.dat fixture."""
+    from qs_testlib import assert_close_table, read_table
+    case = "order2_dftb_hs"
+    src = os.path.join(GOLDEN, "cases", case)
+    for f in os.listdir(src):
+        p = os.path.join(src, f)
+        if os.path.isfile(p):
+            shutil.copy(p, tmp_path / f)
+    subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    new = read_table(str(tmp_path / "hamiltonian_matrix.dat"))
+    ref = read_table(os.path.join(src, "reference", "hamiltonian_matrix.dat"))
+    assert_close_table(new, ref)
+    # integer index columns identical
+    assert [r[:2] for r in new] == [r[:2] for r in ref]
