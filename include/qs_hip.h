@@ -80,6 +80,12 @@ int qs_sizes(const qs_handle *h, int32_t *nA, int32_t *nS, int32_t *nCL, int32_t
              int32_t *nf);
 /* bytes of device memory held by the handle */
 int64_t qs_device_bytes(const qs_handle *h);
+/* device bytes qs_create would allocate for p (admission control of a request
+ * queue; no counterpart in the reference, whose memory estimate is printed by
+ * setup_storage, src/green_functions.f90:178-184); -1 if p is invalid */
+int64_t qs_footprint_bytes(const qs_problem *p);
+/* free and total device memory of a device (hipMemGetInfo) */
+int qs_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes);
 
 /* solve_scf (src/scf.f90:78-151): converge at voltage V (fresh start from the
  * non-interacting solution, or continue from the last input), then the final

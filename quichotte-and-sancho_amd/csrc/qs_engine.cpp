@@ -776,7 +776,9 @@ int upload_sigma1(qs_handle *h, const std::vector<zd> &Gt0) {
                 h->Sig1[(size_t)(i + b) * n + i + a] = v;
             }
     }
+    // the host vector dies at return: finish the copy before that
     HIPCHK(h, hipMemcpyAsync(h->d_sig1, s1.data(), s1.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return QS_OK;
 }
 
@@ -1183,5 +1185,40 @@ int qs_profile_read(qs_handle *h, double *ms, int64_t *count, int32_t nstage) {
 }
 
 const char *qs_stage_name(int32_t stage) { return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : ""; }
+
+int64_t qs_footprint_bytes(const qs_problem *p) {
+    std::string msg;
+    if (check_problem(p, msg) != QS_OK) return -1;
+    qs_handle tmp;
+    tmp.p = *p;
+    tmp.norb = p->norb; tmp.nw = p->n_omega; tmp.nsites = p->norb / 2; tmp.order = p->order;
+    const size_t nn = (size_t)tmp.norb * tmp.norb;
+    tmp.H.assign((const zd *)p->H, (const zd *)p->H + nn);
+    tmp.GamL.assign((const zd *)p->gammaL, (const zd *)p->gammaL + nn);
+    tmp.GamR.assign((const zd *)p->gammaR, (const zd *)p->gammaR + nn);
+    tmp.U.assign(p->U_site, p->U_site + tmp.norb);
+    setup_storage(&tmp);
+    const int64_t nw = tmp.nw, n = tmp.norb, nA = tmp.nA;
+    int m = 1;
+    while ((1 << m) < 2 * tmp.nw - 1) m++;
+    const int64_t Nf = (int64_t)1 << m;
+    int64_t b = nw * (4 * (n / 2) * 16 + n * 8 + 16 + 2 * std::max<int64_t>(nA, 1) * 8 + 32);
+    if (tmp.order == 2) {
+        const int64_t nv = tmp.n_vec;
+        b += 4 * nv * 8;
+        if (p->nhist > 0) b += 2 * nv * (int64_t)p->nhist * (p->single_history ? 4 : 8);
+        b += (nw + Nf) * 2 * nA * nA * 16;
+    }
+    return b + ((int64_t)64 << 20);
+}
+
+int qs_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes) {
+    if (hipSetDevice(device) != hipSuccess) return QS_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return QS_ERR_HIP;
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return QS_OK;
+}
 
 }  // extern "C"

@@ -73,6 +73,9 @@ def load_library(path=SO):
     L.qs_residual2.argtypes = [P, D, I64, P, P, P]
     L.qs_get_sigma.argtypes = [P, P, P, P]
     L.qs_final_sweep.argtypes = [P, D, P]
+    L.qs_footprint_bytes.argtypes = [P]
+    L.qs_footprint_bytes.restype = I64
+    L.qs_device_memory.argtypes = [I32, P, P]
     L.qs_profile.argtypes = [P, I32]
     L.qs_profile_read.argtypes = [P, P, P, I32]
     L.qs_stage_name.argtypes = [I32]
@@ -120,6 +123,56 @@ class ScfResult:
     w_worst: float
     log_fmax: np.ndarray = field(default=None, repr=False)
     log_rms: np.ndarray = field(default=None, repr=False)
+
+
+def _make_problem(prob):
+    """(_Problem, arrays that must stay alive while it is used)"""
+    H = np.asfortranarray(prob.H, dtype=complex)
+    GL = np.asfortranarray(prob.gammaL, dtype=complex)
+    GR = np.asfortranarray(prob.gammaR, dtype=complex)
+    U = np.ascontiguousarray(prob.U_site, dtype=float)
+    an = None if prob.atom_norb is None else np.ascontiguousarray(prob.atom_norb, dtype=np.int32)
+    p = _Problem()
+    p.norb = H.shape[0]
+    p.n_omega = prob.n_omega
+    p.w_init = prob.w_init
+    p.delta = prob.delta
+    p.H = _ptr(H)
+    p.gammaL = _ptr(GL)
+    p.gammaR = _ptr(GR)
+    p.U_site = _ptr(U)
+    p.order = prob.order
+    p.beta = prob.beta
+    p.mu = prob.mu
+    p.n_atoms = 0 if an is None else len(an)
+    p.atom_norb = _ptr(an)
+    p.alpha_mix = prob.alpha_mix
+    p.nhist = prob.nhist
+    p.maxscf = prob.maxscf
+    p.tolf = prob.tolf
+    p.tolf_order1 = prob.tolf_order1
+    p.single_history = int(bool(prob.single_history))
+    p.device = prob.device
+    return p, (H, GL, GR, U, an)
+
+
+def footprint_bytes(prob: ScfProblem):
+    """Device bytes a solve of `prob` needs (qs_footprint_bytes)."""
+    L = load_library()
+    p, keep = _make_problem(prob)
+    b = L.qs_footprint_bytes(ctypes.byref(p))
+    if b < 0:
+        raise QsError(2, "invalid problem")
+    return b
+
+
+def device_memory(device=0):
+    L = load_library()
+    f, t = ctypes.c_int64(), ctypes.c_int64()
+    rc = L.qs_device_memory(device, ctypes.byref(f), ctypes.byref(t))
+    if rc != 0:
+        raise QsError(rc, "hipMemGetInfo failed")
+    return f.value, t.value
 
 
 class ScfEngine:

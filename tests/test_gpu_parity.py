@@ -70,10 +70,10 @@ def test_sigma2_nonpower_of_two_grid_and_noninteracting_site():
     assert rel_max(srg, sro) < 1e-11 and rel_max(slg, slo) < 1e-11
 
 
-@pytest.mark.parametrize("order", [0, 1, 2])
-def test_solve_scf_matches_oracle(order):
+@pytest.mark.parametrize("order,single", [(0, True), (1, True), (2, True), (1, False), (2, False)])
+def test_solve_scf_matches_oracle(order, single):
     prob = helix_problem(4, 2, 1.0, order=order, n_omega=512, wlo=-4.0, whi=4.0, tolf=1e-10, tolf_order1=1e-12,
-                         nhist=8, voltages=[0.0, 0.3, 0.6])
+                         nhist=8, voltages=[0.0, 0.3, 0.6], single_history=single)
     o = Oracle(prob)
     g = gpu_engine(prob)
     fresh = True
