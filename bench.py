@@ -159,6 +159,58 @@ def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
             "s_per_iteration": t_iter}
 
 
+def serve_bench(args):
+    """Poisson arrivals of single-bias solves of helix systems with 16, 48 or 80
+    spin-orbitals, grids of 2^13 or 2^14 energies, U ~ U[0.5, 3], V ~ U[-2, 2]."""
+    import importlib
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    serving = importlib.import_module("quichotte-and-sancho_amd.serving")
+    rates = [float(r) for r in args.serve.split(",")]
+    shapes = {16: (4, 2), 48: (8, 3), 80: (8, 5)}
+    out = {"metric": "serving: converged bias points/s and latency", "unit": "bias points/s", "n_gpus": 1,
+           "dtype": "f64", "data": "synthetic", "higher_is_better": True,
+           "config": {"workload": "helix systems of 16/48/80 spin-orbitals, 2^13/2^14 energies, U~U[0.5,3], "
+                                  "V~U[-2,2], tolf 1e-8, nhist 16, Poisson arrivals", "requests_per_rate": args.requests},
+           "rates": {}}
+    for rate in rates:
+        rng = np.random.default_rng(int(rate))
+        t, reqs = 0.0, []
+        for rid in range(args.requests):
+            t += rng.exponential(1.0 / rate)
+            norb = int(rng.choice(list(shapes)))
+            lg = int(rng.choice([13, 14]))
+            U = float(rng.uniform(0.5, 3.0))
+            V = float(rng.uniform(-2.0, 2.0))
+            ni, nt = shapes[norb]
+            H = helix_H(ni, nt)
+            n = H.shape[0]
+            GL = np.zeros((n, n), complex)
+            GR = np.zeros((n, n), complex)
+            for i in (0, 1):
+                GL[i, i] = 0.1
+                GR[n - 2 + i, n - 2 + i] = 0.1
+            Us = np.zeros(n)
+            Us[0::2] = U
+            nw = 1 << lg
+            prob = pkg.ScfProblem(H=H, gammaL=GL, gammaR=GR, U_site=Us, order=2, n_omega=nw, w_init=-12.0,
+                                  delta=24.0 / nw, beta=1.0 / (KB * 300.0), mu=0.0, nhist=16, tolf=1e-8)
+            reqs.append(serving.Request(rid, prob, V, arrival=t))
+        srv = serving.Server(max_running=args.serve_streams)
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in res])
+        ok = [r for r in res if r.converged and not r.error]
+        out["rates"][str(rate)] = {"bias_points_per_s": len(ok) / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                                   "p99_ms": 1e3 * float(np.percentile(lat, 99)), "converged": len(ok),
+                                   "requests": len(res), "wall_s": wall}
+        print("bench: serve rate %g/s: %d/%d converged, %.2f points/s, p50 %.0f ms, p99 %.0f ms" %
+              (rate, len(ok), len(res), len(ok) / wall, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99)),
+              file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,11 +219,16 @@ def main():
     ap.add_argument("--workload", default="helix80", choices=["helix80", "helix16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--serve", default="", help="comma-separated Poisson arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--serve-streams", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (one bias point is one "
               "GPU's work; see the sharding notes in DESIGN.md)" % args.gpus, file=sys.stderr)
         return 2
+    if args.serve:
+        return serve_bench(args)
     import importlib
     pkg = importlib.import_module("quichotte-and-sancho_amd")
     prob, volts, desc = workload(args.workload)
@@ -181,8 +238,9 @@ def main():
     print("bench: %s" % desc, file=sys.stderr, flush=True)
     t0 = time.time()
     eng = pkg.ScfEngine(pkg.ScfProblem(**prob))
-    print("bench: device memory %.1f GB, setup %.1f s" % (eng.device_bytes() / 1e9, time.time() - t0),
-          file=sys.stderr, flush=True)
+    dv, dms = eng.dyson_choice()
+    print("bench: device memory %.1f GB, setup %.1f s, Dyson launch shape %d (tuned ms %s)" %
+          (eng.device_bytes() / 1e9, time.time() - t0, dv, dms), file=sys.stderr, flush=True)
     fresh = True
     for k in range(args.warmup):
         t1 = time.time()
@@ -236,7 +294,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "form": "plain FP64 path, one bias point at a time",
-                   "n_spin_orbitals": eng.norb, "n_omega": eng.prob.n_omega, "fft_length": eng.Nf},
+                   "n_spin_orbitals": eng.norb, "n_omega": eng.prob.n_omega, "fft_length": eng.Nf,
+                   "dyson_threads_per_energy": 256 * (dv + 1)},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "scf_steps_per_point": float(np.mean(iters)),
         "all_converged": bool(all(conv)),

@@ -6,9 +6,11 @@
 //   G^r = [ w + Hemb - Sigma1 - Sigma2^r(w) ]^-1                  (Dyson)
 //   G^< = G^r(:,sup) [ i(f_L GamL + f_R GamR) + Sigma2^<(w) ] G^r(:,sup)^dagger   (Keldysh)
 //
-// One workgroup of 256 threads (4 wavefronts) per energy.  The threads form a
-// 16 x 16 grid; thread (ti, tj) keeps the R x R elements (ti + 16a, tj + 16b) of
-// the matrix in registers ("cyclic" distribution, n <= 16 R).
+// One workgroup per energy.  Its TI x 16 threads (TI = 16 or 32: 4 or 8
+// wavefronts) form a grid; thread (ti, tj) keeps the RA x RB elements
+// (ti + TI a, tj + 16 b) of the matrix in registers ("cyclic" distribution,
+// n <= TI RA and n <= 16 RB).  The variant is chosen per size by timing
+// (qs_dyson_tune).
 //
 // Inverse: in-place Gauss-Jordan with partial row pivoting on |Re| + |Im|, the
 // pivot rule of LAPACK's zgetf2.  Rows are never moved: step k picks the unused
@@ -36,7 +38,6 @@
 
 namespace {
 
-constexpr int NT = 256;
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 __device__ __forceinline__ int sig_block_index(int a, int b, int nAs) {
@@ -53,16 +54,19 @@ struct LdsLayout {
     double2 *grcl;
 };
 
-__device__ __forceinline__ LdsLayout lds_layout(char *base, int n, int R) {
+// LDS: big (n x (n+1)), chunk (16 x 16 RB >= 16 x n), 3 int maps of 16 RB,
+// 32 pivot candidates, 16 doubles of reduction scratch, the nCL x nCL block.
+// M (48 RB ints) keeps 8-byte alignment for every RB; grcl stays 16-byte aligned.
+__device__ __forceinline__ LdsLayout lds_layout(char *base, int n, int RB) {
     LdsLayout L;
     L.big = (double2 *)base;
     L.chunk = L.big + (size_t)n * (n + 1);
-    L.piv = (int *)(L.chunk + 16 * 16 * R);
-    L.qinv = L.piv + 16 * R;
-    L.spc = L.qinv + 16 * R;
-    L.redv = (double *)(L.spc + 16 * R + ((3 * 16 * R) & 1));  // keep 8-byte alignment
-    L.redi = (int *)(L.redv + 16);
-    L.scratch = (double *)(L.redi + 16);
+    L.piv = (int *)(L.chunk + 16 * 16 * RB);
+    L.qinv = L.piv + 16 * RB;
+    L.spc = L.qinv + 16 * RB;
+    L.redv = (double *)(L.spc + 16 * RB);
+    L.redi = (int *)(L.redv + 32);
+    L.scratch = (double *)(L.redi + 32);
     L.grcl = (double2 *)(L.scratch + 16);
     return L;
 }
@@ -77,25 +81,26 @@ __device__ __forceinline__ double2 sl_elem(const DysonArgs &A, int b1, int b2, d
     return v;
 }
 
-template <int R>
-__global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
+template <int TI, int RA, int RB>
+__global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
+    constexpr int NT = TI * 16;
     extern __shared__ __align__(16) char lds_raw[];
     const int n = A.norb, pitch = n + 1;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
     const int iw = A.iw0 + blockIdx.x;
-    LdsLayout L = lds_layout(lds_raw, n, R);
+    LdsLayout L = lds_layout(lds_raw, n, RB);
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const int nA = A.nA, nA2 = nA * nA;
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
 
     // ---- 1. A = w + Hemb - Sigma1 - Sigma2^r, in registers -------------------------
-    double2 M[R][R];
+    double2 M[RA][RB];
 #pragma unroll
-    for (int a = 0; a < R; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < R; b++) {
-            const int i = ti + 16 * a, j = tj + 16 * b;
+        for (int b = 0; b < RB; b++) {
+            const int i = ti + TI * a, j = tj + 16 * b;
             double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
             if (i < n && j < n) {
                 v = A.Hemb[i * n + j];
@@ -110,73 +115,85 @@ __global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
             }
             M[a][b] = v;
         }
-    unsigned used = 0;  // bit a: my row ti + 16a already pivoted
+    unsigned used = 0;  // bit a: my row ti + TI a already pivoted
 #pragma unroll
-    for (int a = 0; a < R; a++)
-        if (ti + 16 * a >= n) used |= 1u << a;
+    for (int a = 0; a < RA; a++)
+        if (ti + TI * a >= n) used |= 1u << a;
 
-    double2 *prow = L.chunk, *pcol = L.chunk + 16 * R;
+    double2 *prow = L.chunk, *pcol = L.chunk + 16 * RB;
 
     // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
+    const int lane = tid & 63;
     for (int k = 0; k < n; k++) {
         const int bk = k >> 4, tk = k & 15;
         if (tj == tk) {
             double best = -1.0;
             int bi = -1;
 #pragma unroll
-            for (int a = 0; a < R; a++)
+            for (int a = 0; a < RA; a++)
 #pragma unroll
-                for (int b = 0; b < R; b++)
+                for (int b = 0; b < RB; b++)
                     if (b == bk && !(used & (1u << a))) {
                         double v = fabs(M[a][b].x) + fabs(M[a][b].y);
                         if (v != v) v = -0.5;  // NaN never wins over a number
-                        if (bi < 0 || v > best) { best = v; bi = ti + 16 * a; }
+                        if (bi < 0 || v > best) { best = v; bi = ti + TI * a; }
                     }
             L.redv[ti] = best;
             L.redi[ti] = bi;
         }
         __syncthreads();
+        // every wavefront reduces the TI candidates (largest value, then smallest row)
         double best = -2.0;
         int p = -1;
-        for (int t = 0; t < 16; t++) {
-            const int ii = L.redi[t];
-            const double v = L.redv[t];
-            if (ii >= 0 && (p < 0 || v > best || (v == best && ii < p))) { best = v; p = ii; }
+        {
+            const int t = lane & (TI - 1);
+            int ii = L.redi[t];
+            double v = L.redv[t];
+            if (ii < 0) { v = -3.0; ii = 1 << 30; }
+#pragma unroll
+            for (int o = TI / 2; o > 0; o >>= 1) {
+                const double vo = __shfl_xor(v, o, 64);
+                const int io = __shfl_xor(ii, o, 64);
+                if (vo > v || (vo == v && io < ii)) { v = vo; ii = io; }
+            }
+            best = v;
+            p = ii;
         }
-        const int ap = p >> 4, tp = p & 15;
+        (void)best;
+        const int tp = p % TI, ap = p / TI;
         if (ti == tp) {
 #pragma unroll
-            for (int a = 0; a < R; a++)
+            for (int a = 0; a < RA; a++)
                 if (a == ap)
 #pragma unroll
-                    for (int b = 0; b < R; b++) prow[tj + 16 * b] = M[a][b];
+                    for (int b = 0; b < RB; b++) prow[tj + 16 * b] = M[a][b];
         }
         if (tj == tk) {
 #pragma unroll
-            for (int b = 0; b < R; b++)
+            for (int b = 0; b < RB; b++)
                 if (b == bk)
 #pragma unroll
-                    for (int a = 0; a < R; a++) pcol[ti + 16 * a] = M[a][b];
+                    for (int a = 0; a < RA; a++) pcol[ti + TI * a] = M[a][b];
         }
         if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
         __syncthreads();
         const double2 pv = pcol[p];
         if (tid == 0 && pv.x == 0.0 && pv.y == 0.0) atomicOr(A.status, 1);
         const double2 rinv = cinv(pv);
-        double2 sp[R], fr[R];
+        double2 sp[RB], fr[RA];
 #pragma unroll
-        for (int b = 0; b < R; b++) sp[b] = cmul(prow[tj + 16 * b], rinv);
+        for (int b = 0; b < RB; b++) sp[b] = cmul(prow[tj + 16 * b], rinv);
 #pragma unroll
-        for (int a = 0; a < R; a++) fr[a] = pcol[ti + 16 * a];
+        for (int a = 0; a < RA; a++) fr[a] = pcol[ti + TI * a];
 #pragma unroll
-        for (int a = 0; a < R; a++)
+        for (int a = 0; a < RA; a++)
 #pragma unroll
-            for (int b = 0; b < R; b++) cfms(M[a][b], fr[a], sp[b]);
+            for (int b = 0; b < RB; b++) cfms(M[a][b], fr[a], sp[b]);
         if (tj == tk) {
 #pragma unroll
-            for (int a = 0; a < R; a++)
+            for (int a = 0; a < RA; a++)
 #pragma unroll
-                for (int b = 0; b < R; b++)
+                for (int b = 0; b < RB; b++)
                     if (b == bk) {
                         double2 t = cmul(fr[a], rinv);
                         M[a][b] = make_double2(-t.x, -t.y);
@@ -184,11 +201,11 @@ __global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
         }
         if (ti == tp) {
 #pragma unroll
-            for (int a = 0; a < R; a++)
+            for (int a = 0; a < RA; a++)
                 if (a == ap) {
                     used |= 1u << a;
 #pragma unroll
-                    for (int b = 0; b < R; b++) M[a][b] = (b == bk && tj == tk) ? rinv : sp[b];
+                    for (int b = 0; b < RB; b++) M[a][b] = (b == bk && tj == tk) ? rinv : sp[b];
                 }
         }
     }
@@ -196,10 +213,10 @@ __global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
 
     // ---- 3. G^r = permuted M staged in LDS; mode extraction -----------------------
 #pragma unroll
-    for (int a = 0; a < R; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < R; b++) {
-            const int r = ti + 16 * a, c = tj + 16 * b;
+        for (int b = 0; b < RB; b++) {
+            const int r = ti + TI * a, c = tj + 16 * b;
             if (r < n && c < n) L.big[L.qinv[r] * pitch + L.piv[c]] = M[a][b];
         }
     for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
@@ -239,17 +256,17 @@ __global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
 
     // ---- 4. phase 1: T = M S'' --------------------------------------------------
 #pragma unroll
-    for (int a = 0; a < R; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < R; b++) {
-            const int r = ti + 16 * a, c = tj + 16 * b;
+        for (int b = 0; b < RB; b++) {
+            const int r = ti + TI * a, c = tj + 16 * b;
             if (r < n && c < n) L.big[r * pitch + c] = M[a][b];
         }
-    double2 acc[R][R];
+    double2 acc[RA][RB];
 #pragma unroll
-    for (int a = 0; a < R; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < R; b++) acc[a][b] = make_double2(0.0, 0.0);
+        for (int b = 0; b < RB; b++) acc[a][b] = make_double2(0.0, 0.0);
     for (int k0 = 0; k0 < n; k0 += 16) {
         __syncthreads();
         for (int e = tid; e < 16 * n; e += NT) {
@@ -264,71 +281,71 @@ __global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
         __syncthreads();
         const int kmax = min(16, n - k0);
         for (int kk = 0; kk < kmax; kk++) {
-            double2 av[R], bv[R];
+            double2 av[RA], bv[RB];
 #pragma unroll
-            for (int a = 0; a < R; a++) {
-                const int r = ti + 16 * a;
+            for (int a = 0; a < RA; a++) {
+                const int r = ti + TI * a;
                 av[a] = r < n ? L.big[r * pitch + k0 + kk] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for (int b = 0; b < R; b++) {
+            for (int b = 0; b < RB; b++) {
                 const int c = tj + 16 * b;
                 bv[b] = c < n ? L.chunk[kk * n + c] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for (int a = 0; a < R; a++)
+            for (int a = 0; a < RA; a++)
 #pragma unroll
-                for (int b = 0; b < R; b++) cfma(acc[a][b], av[a], bv[b]);
+                for (int b = 0; b < RB; b++) cfma(acc[a][b], av[a], bv[b]);
         }
     }
     __syncthreads();
     // ---- 5. phase 2: P = T M^dagger ---------------------------------------------
 #pragma unroll
-    for (int a = 0; a < R; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < R; b++) {
-            const int r = ti + 16 * a, c = tj + 16 * b;
+        for (int b = 0; b < RB; b++) {
+            const int r = ti + TI * a, c = tj + 16 * b;
             if (r < n && c < n) L.big[r * pitch + c] = acc[a][b];
             acc[a][b] = make_double2(0.0, 0.0);
         }
 #pragma unroll
-    for (int bb = 0; bb < R; bb++) {
+    for (int bb = 0; bb < RB; bb++) {
         const int k0 = 16 * bb;
         if (k0 >= n) break;
         __syncthreads();
         // chunk[kk][j] = conj(M[j][k0 + kk]); thread (ti, tj) owns column k0 + tj
 #pragma unroll
-        for (int a = 0; a < R; a++) {
-            const int j = ti + 16 * a;
+        for (int a = 0; a < RA; a++) {
+            const int j = ti + TI * a;
             if (j < n) L.chunk[tj * n + j] = cconj(M[a][bb]);
         }
         __syncthreads();
         const int kmax = min(16, n - k0);
         for (int kk = 0; kk < kmax; kk++) {
-            double2 av[R], bv[R];
+            double2 av[RA], bv[RB];
 #pragma unroll
-            for (int a = 0; a < R; a++) {
-                const int r = ti + 16 * a;
+            for (int a = 0; a < RA; a++) {
+                const int r = ti + TI * a;
                 av[a] = r < n ? L.big[r * pitch + k0 + kk] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for (int b = 0; b < R; b++) {
+            for (int b = 0; b < RB; b++) {
                 const int c = tj + 16 * b;
                 bv[b] = c < n ? L.chunk[kk * n + c] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for (int a = 0; a < R; a++)
+            for (int a = 0; a < RA; a++)
 #pragma unroll
-                for (int b = 0; b < R; b++) cfma(acc[a][b], av[a], bv[b]);
+                for (int b = 0; b < RB; b++) cfma(acc[a][b], av[a], bv[b]);
         }
     }
     __syncthreads();
     // ---- 6. G^< = permuted P staged in LDS; mode extraction ------------------------
 #pragma unroll
-    for (int a = 0; a < R; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < R; b++) {
-            const int r = ti + 16 * a, c = tj + 16 * b;
+        for (int b = 0; b < RB; b++) {
+            const int r = ti + TI * a, c = tj + 16 * b;
             if (r < n && c < n) L.big[L.qinv[r] * pitch + L.qinv[c]] = acc[a][b];
         }
     __syncthreads();
@@ -401,26 +418,26 @@ __global__ __launch_bounds__(NT) void k_dyson(DysonArgs A) {
     }
 }
 
-template <int R>
-hipError_t launch_r(const DysonArgs &a, int count, hipStream_t s) {
+template <int TI, int RA, int RB>
+hipError_t launch_v(const DysonArgs &a, int count, hipStream_t s) {
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_dyson<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)k_dyson<TI, RA, RB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         attr = true;
     }
     const size_t lds = qs_dyson_lds_bytes(a.norb, a.nCL);
-    hipLaunchKernelGGL(k_dyson<R>, dim3(count), dim3(NT), lds, s, a);
+    hipLaunchKernelGGL((k_dyson<TI, RA, RB>), dim3(count), dim3(TI * 16), lds, s, a);
     return hipGetLastError();
 }
 
 }  // namespace
 
 size_t qs_dyson_lds_bytes(int n, int nCL) {
-    const int R = (n + 15) / 16;
-    size_t b = (size_t)n * (n + 1) * 16 + (size_t)16 * 16 * R * 16;
-    b += (size_t)(3 * 16 * R + 1) * 4 + 16 * 8 + 16 * 4 + 16 * 8;
+    const int RB = (n + 15) / 16;
+    size_t b = (size_t)n * (n + 1) * 16 + (size_t)16 * 16 * RB * 16;
+    b += (size_t)3 * 16 * RB * 4 + 32 * 8 + 32 * 4 + 16 * 8;
     b = (b + 15) & ~(size_t)15;
     b += (size_t)nCL * nCL * 16;
     return b;
@@ -428,17 +445,35 @@ size_t qs_dyson_lds_bytes(int n, int nCL) {
 
 int qs_dyson_max_norb() { return 88; }
 
-hipError_t qs_launch_dyson(const DysonArgs &a, int count, hipStream_t s) {
+// variant 0: 256 threads (16 x 16), variant 1: 512 threads (32 x 16)
+int qs_dyson_variants(int norb) { return norb <= 16 ? 1 : 2; }
+
+hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    const int R = (a.norb + 15) / 16;
     if (qs_dyson_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
-    switch (R) {
-        case 1: return launch_r<1>(a, count, s);
-        case 2: return launch_r<2>(a, count, s);
-        case 3: return launch_r<3>(a, count, s);
-        case 4: return launch_r<4>(a, count, s);
-        case 5: return launch_r<5>(a, count, s);
-        case 6: return launch_r<6>(a, count, s);
+    const int RB = (a.norb + 15) / 16;
+    if (variant == 0 || a.norb <= 16) {
+        switch (RB) {
+            case 1: return launch_v<16, 1, 1>(a, count, s);
+            case 2: return launch_v<16, 2, 2>(a, count, s);
+            case 3: return launch_v<16, 3, 3>(a, count, s);
+            case 4: return launch_v<16, 4, 4>(a, count, s);
+            case 5: return launch_v<16, 5, 5>(a, count, s);
+            case 6: return launch_v<16, 6, 6>(a, count, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    const int RA = (a.norb + 31) / 32;
+    switch (RB * 4 + RA) {
+        case 2 * 4 + 1: return launch_v<32, 1, 2>(a, count, s);
+        case 3 * 4 + 2: return launch_v<32, 2, 3>(a, count, s);
+        case 4 * 4 + 2: return launch_v<32, 2, 4>(a, count, s);
+        case 5 * 4 + 3: return launch_v<32, 3, 5>(a, count, s);
+        case 6 * 4 + 3: return launch_v<32, 3, 6>(a, count, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t qs_launch_dyson(const DysonArgs &a, int count, hipStream_t s) {
+    return qs_launch_dyson_variant(a, count, 0, s);
 }

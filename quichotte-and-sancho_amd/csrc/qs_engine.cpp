@@ -19,6 +19,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -96,6 +97,8 @@ struct qs_handle {
     double nworst_final = 0.5, wworst_final = 1.0;
     bool final_valid = false;
     Prof prof;
+    int dyson_variant = 0;
+    double dyson_tune_ms[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -407,7 +410,7 @@ int run_dyson(qs_handle *h, int mode, bool use_sigma, double V) {
     DysonArgs a = dyson_args(h, mode, use_sigma, V);
     HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(int), h->stream));
     hipEvent_t t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_dyson(a, h->nw, h->stream));
+    HIPCHK(h, qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream));
     prof_end(h, mode == QS_M_FINAL ? ST_FINAL : ST_DYSON, t0);
     return QS_OK;
 }
@@ -456,6 +459,40 @@ Sanity diag_sanity(const double *sumImGL, const double *sumImGR, int nb, double 
     s.runaway = ra;
     s.unphysical = (ra || un);
     return s;
+}
+
+// Pick the Dyson launch shape for this size: time each variant on a slice of
+// the grid (mode FINAL, which every order runs) and keep the fastest.
+// QS_DYSON_VARIANT=<k> in the environment forces variant k.
+int tune_dyson(qs_handle *h) {
+    const int nv = qs_dyson_variants(h->norb);
+    h->dyson_variant = 0;
+    if (const char *e = getenv("QS_DYSON_VARIANT")) {
+        const int v = atoi(e);
+        if (v >= 0 && v < nv) h->dyson_variant = v;
+        return QS_OK;
+    }
+    if (nv <= 1) return QS_OK;
+    const int count = std::min(h->nw, 2048);
+    DysonArgs a = dyson_args(h, QS_M_FINAL, false, 0.0);
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    double best = 1e300;
+    for (int v = 0; v < nv; v++) {
+        HIPCHK(h, qs_launch_dyson_variant(a, count, v, h->stream));  // warm-up (code load)
+        HIPCHK(h, hipEventRecord(e0, h->stream));
+        for (int r = 0; r < 2; r++) HIPCHK(h, qs_launch_dyson_variant(a, count, v, h->stream));
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+        if (v < 4) h->dyson_tune_ms[v] = ms / 2;
+        if (ms < best) { best = ms; h->dyson_variant = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return QS_OK;
 }
 
 // scf_residual2 on the device: d_x -> d_F; stats[3] = max|F|, |F|^2, |F-Fold|^2
@@ -920,6 +957,7 @@ int qs_create(const qs_problem *p, qs_handle **out) {
     if (h->nCL > 64) { fprintf(stderr, "qs_create: more than 64 left-contact states\n"); delete h; return QS_ERR_UNSUPPORTED; }
     if (h->order == 2 && (h->log2N1 > 10 || h->log2N2 < 1)) { delete h; return QS_ERR_UNSUPPORTED; }
     rc = setup_device(h);
+    if (!rc) rc = tune_dyson(h);
     if (rc) {
         fprintf(stderr, "qs_create: %s\n", h->err.c_str());
         qs_destroy(h);
@@ -1219,6 +1257,14 @@ int qs_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes) 
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
     return QS_OK;
+}
+
+int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax) {
+    if (!h) return QS_ERR_ARG;
+    if (variant) *variant = h->dyson_variant;
+    for (int i = 0; i < nmax && i < 4; i++)
+        if (ms_per_variant) ms_per_variant[i] = h->dyson_tune_ms[i];
+    return qs_dyson_variants(h->norb);
 }
 
 }  // extern "C"

@@ -39,6 +39,9 @@ struct DysonArgs {
 
 // Dyson + Keldysh at every energy of [iw0, iw0 + count).  Returns the hip error.
 hipError_t qs_launch_dyson(const DysonArgs &a, int count, hipStream_t s);
+// launch shapes of the Dyson kernel for a size: qs_dyson_variants(norb) choices
+int qs_dyson_variants(int norb);
+hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, hipStream_t s);
 size_t qs_dyson_lds_bytes(int norb, int nCL);
 int qs_dyson_max_norb();
 

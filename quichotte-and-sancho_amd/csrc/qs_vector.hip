@@ -105,15 +105,16 @@ struct Gam { double g[MAXH]; };
 // One pass: store the new (dF, u) pair (slot >= 0) and accumulate, for k < nh,
 //   gram_k = <dF_k, dF_slot>,  proj_k = <dF_k, F>   (double sums of FP32 history)
 // partials[block][2 MAXH].
+template <int MH>
 __global__ __launch_bounds__(NT) void k_hist_f32(int64_t n, const double *__restrict__ F,
                                                  const double *__restrict__ Fold, const double *__restrict__ x,
                                                  const double *__restrict__ xold, double rf, double alpha,
                                                  float *__restrict__ dFs, float *__restrict__ Us, int slot, int nh,
                                                  double *__restrict__ part) {
     __shared__ double red[NT / QS_WAVE][2 * MAXH];
-    double g[MAXH], pr[MAXH];
+    double g[MH], pr[MH];
 #pragma unroll
-    for (int k = 0; k < MAXH; k++) { g[k] = 0.0; pr[k] = 0.0; }
+    for (int k = 0; k < MH; k++) { g[k] = 0.0; pr[k] = 0.0; }
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
         const double f = F[i];
         float df = 0.0f;
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(NT) void k_hist_f32(int64_t n, const double *__rest
             Us[(size_t)slot * n + i] = u;
         }
 #pragma unroll
-        for (int k = 0; k < MAXH; k++)
+        for (int k = 0; k < MH; k++)
             if (k < nh) {
                 const double hk = (k == slot) ? (double)df : (double)dFs[(size_t)k * n + i];
                 g[k] = fma(hk, (double)df, g[k]);
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(NT) void k_hist_f32(int64_t n, const double *__rest
     }
     const int w = threadIdx.x / QS_WAVE, lane = threadIdx.x & (QS_WAVE - 1);
 #pragma unroll
-    for (int k = 0; k < MAXH; k++)
+    for (int k = 0; k < MH; k++)
         if (k < nh) {
             const double a = wave_sum(g[k]), b = wave_sum(pr[k]);
             if (lane == 0) { red[w][k] = a; red[w][MAXH + k] = b; }
@@ -147,15 +148,16 @@ __global__ __launch_bounds__(NT) void k_hist_f32(int64_t n, const double *__rest
     }
 }
 
+template <int MH>
 __global__ __launch_bounds__(NT) void k_hist_f64(int64_t n, const double *__restrict__ F,
                                                  const double *__restrict__ Fold, const double *__restrict__ x,
                                                  const double *__restrict__ xold, double fnorm, double alpha,
                                                  double *__restrict__ dF, double *__restrict__ U, int slot, int nh,
                                                  double *__restrict__ part) {
     __shared__ double red[NT / QS_WAVE][2 * MAXH];
-    double g[MAXH], pr[MAXH];
+    double g[MH], pr[MH];
 #pragma unroll
-    for (int k = 0; k < MAXH; k++) { g[k] = 0.0; pr[k] = 0.0; }
+    for (int k = 0; k < MH; k++) { g[k] = 0.0; pr[k] = 0.0; }
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
         const double f = F[i];
         double dfn = 0.0;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(NT) void k_hist_f64(int64_t n, const double *__rest
             U[(size_t)slot * n + i] = alpha * dfn + (x[i] - xold[i]) / fnorm;
         }
 #pragma unroll
-        for (int k = 0; k < MAXH; k++)
+        for (int k = 0; k < MH; k++)
             if (k < nh) {
                 const double hk = (k == slot) ? dfn : dF[(size_t)k * n + i];
                 g[k] = fma(hk, dfn, g[k]);
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(NT) void k_hist_f64(int64_t n, const double *__rest
     }
     const int w = threadIdx.x / QS_WAVE, lane = threadIdx.x & (QS_WAVE - 1);
 #pragma unroll
-    for (int k = 0; k < MAXH; k++)
+    for (int k = 0; k < MH; k++)
         if (k < nh) {
             const double a = wave_sum(g[k]), b = wave_sum(pr[k]);
             if (lane == 0) { red[w][k] = a; red[w][MAXH + k] = b; }
@@ -252,8 +254,12 @@ hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *
                                       const double *xold, double rf, double alpha, float *dFs, float *Us, int slot,
                                       int nh, double *partials, int nblocks, hipStream_t s) {
     if (nh > MAXH) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_hist_f32, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us, slot, nh,
-                       partials);
+    if (nh <= 16)
+        hipLaunchKernelGGL(k_hist_f32<16>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us, slot,
+                           nh, partials);
+    else
+        hipLaunchKernelGGL(k_hist_f32<MAXH>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us,
+                           slot, nh, partials);
     return hipGetLastError();
 }
 
@@ -261,8 +267,12 @@ hipError_t qs_launch_broyden_hist_f64(int64_t n, const double *F, const double *
                                       const double *xold, double fnorm, double alpha, double *dF, double *U, int slot,
                                       int nh, double *partials, int nblocks, hipStream_t s) {
     if (nh > MAXH) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_hist_f64, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, fnorm, alpha, dF, U, slot, nh,
-                       partials);
+    if (nh <= 16)
+        hipLaunchKernelGGL(k_hist_f64<16>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, fnorm, alpha, dF, U, slot,
+                           nh, partials);
+    else
+        hipLaunchKernelGGL(k_hist_f64<MAXH>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, fnorm, alpha, dF, U,
+                           slot, nh, partials);
     return hipGetLastError();
 }
 

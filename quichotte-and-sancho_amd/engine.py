@@ -78,6 +78,7 @@ def load_library(path=SO):
     L.qs_device_memory.argtypes = [I32, P, P]
     L.qs_profile.argtypes = [P, I32]
     L.qs_profile_read.argtypes = [P, P, P, I32]
+    L.qs_dyson_choice.argtypes = [P, P, P, I32]
     L.qs_stage_name.argtypes = [I32]
     L.qs_stage_name.restype = ctypes.c_char_p
     _lib = L
@@ -284,6 +285,12 @@ class ScfEngine:
         s1 = np.zeros((self.norb, self.norb), complex, order="F")
         self._check(self.L.qs_get_sigma(self.h, _ptr(r), _ptr(l), _ptr(s1)))
         return r, l, s1
+
+    def dyson_choice(self):
+        v = ctypes.c_int32()
+        ms = np.zeros(4)
+        n = self.L.qs_dyson_choice(self.h, ctypes.byref(v), _ptr(ms), 4)
+        return v.value, ms[:n].tolist()
 
     def profile(self, enable=True):
         self._check(self.L.qs_profile(self.h, int(bool(enable))))

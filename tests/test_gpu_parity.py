@@ -128,3 +128,21 @@ def test_runaway_and_errors_are_reported():
     x = g.init2(0.0) * 1e6
     F, ab = g.residual2(0.0, x)
     assert ab and not F.any()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("norb", [18, 48, 80, 88])
+def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
+    """Both launch shapes of the Dyson kernel (256 and 512 threads per energy)."""
+    monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
+    prob = random_problem(norb, 48, seed=100 + norb, U=0.9)
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    assert g.dyson_choice()[0] == variant
+    x = o.init2(0.1)
+    assert rel_max(g.init2(0.1), x) < 1e-11
+    F0, _ = o.residual2(0.1, x)
+    x1 = x + 0.5 * F0
+    Fo, _ = o.residual2(0.1, x1)
+    Fg, _ = g.residual2(0.1, x1)
+    assert rel_max(Fg, Fo) < 1e-11

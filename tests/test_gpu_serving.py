@@ -1,0 +1,65 @@
+"""Serving loop: requests of different sizes admitted by footprint and run
+concurrently give the same result as alone (bit-for-bit: each request's
+kernels run with the same launch shapes in its own handle), and one of them
+matches the oracle."""
+import importlib
+
+import numpy as np
+import pytest
+
+from qs_testlib import Oracle, helix_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def make_requests():
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    serving = importlib.import_module("quichotte-and-sancho_amd.serving")
+    rng = np.random.default_rng(11)
+    reqs, probs = [], []
+    shapes = [(2, 2, 256), (4, 2, 512), (4, 1, 1024), (6, 4, 256), (4, 2, 256), (2, 3, 512), (4, 3, 384), (2, 2, 1024)]
+    for rid, (ni, nt, nw) in enumerate(shapes):
+        U = float(rng.uniform(0.5, 2.0))
+        V = float(rng.uniform(-1.0, 1.0))
+        p = helix_problem(ni, nt, U, order=2, n_omega=nw, wlo=-5.0, whi=5.0, tolf=1e-10, nhist=8, voltages=[V])
+        probs.append(p)
+        sp = pkg.ScfProblem(H=p.H, gammaL=p.GL, gammaR=p.GR, U_site=p.U_site, order=2, n_omega=p.n_omega,
+                            w_init=p.w_init, delta=p.delta, beta=p.beta, mu=p.mu, nhist=8, tolf=1e-10)
+        reqs.append(serving.Request(rid, sp, V, arrival=0.002 * rid))
+    return pkg, serving, reqs, probs
+
+
+def test_mix_equals_alone_and_oracle():
+    pkg, serving, reqs, probs = make_requests()
+    alone = []
+    for r in reqs:
+        e = pkg.ScfEngine(r.problem)
+        res = e.solve_scf(r.V, True)
+        alone.append((e.current(), res.n_iter))
+        e.close()
+    srv = serving.Server(max_running=4)
+    out = srv.run(reqs)
+    assert len(out) == len(reqs)
+    for r, (J, nit) in zip(out, alone):
+        assert r.error == ""
+        assert r.converged and r.n_iter == nit
+        assert r.J_up == J[0] and r.J_down == J[1]
+    o = Oracle(probs[1])
+    ro = o.solve(reqs[1].V, True)
+    Jo = np.array(o.current())
+    assert ro["n_iter"] == out[1].n_iter
+    assert np.all(np.abs(np.array([out[1].J_up, out[1].J_down]) - Jo) <= 1e-10 * np.abs(Jo) + 1e-13)
+
+
+def test_admission_respects_budget():
+    pkg, serving, reqs, probs = make_requests()
+    srv = serving.Server(max_running=8)
+    fp = [pkg.footprint_bytes(r.problem) for r in reqs]
+    srv.budget = max(fp) + min(fp) // 2      # at most ~one large request at a time
+    out = srv.run(reqs[:4])
+    assert all(r.error == "" and r.converged for r in out)
+    # intervals of requests whose footprints cannot coexist do not overlap
+    for a in out:
+        for b in out:
+            if a.rid < b.rid and a.footprint + b.footprint > srv.budget:
+                assert a.done <= b.start + 1e-6 or b.done <= a.start + 1e-6
