@@ -34,28 +34,52 @@ constexpr int CB = 8;  // columns per site-pair block
 
 __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned)v) >> (32 - bits); }
 
-// In-place radix-2 DIT on `nseq` sequences of length 2^lg, stored with stride `ld`
-// (complex), input already in bit-reversed order.  sign = -1 forward, +1 inverse.
-// twstep = Nf / 2^lg; tw[k] = e^{-2 pi i k / Nf}.
-__device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const double2 *tw, int twstep) {
-    const int L = 1 << lg, half_total = nseq * (L >> 1);
-    for (int s = 1; s <= lg; s++) {
-        const int half = 1 << (s - 1);
-        const int tws = twstep * (L >> s);  // e^{-2 pi i j / 2^s} = tw[j * Nf / 2^s]
-        for (int t = threadIdx.x; t < half_total; t += blockDim.x) {
-            const int seq = t >> (lg - 1);
-            const int bf = t & ((L >> 1) - 1);
-            const int j = bf & (half - 1), g = bf >> (s - 1);
-            const int i1 = g * (half << 1) + j, i2 = i1 + half;
-            double2 w = tw[j * tws];
-            if (sign > 0) w.y = -w.y;
-            double2 *b = buf + (size_t)seq * ld;
-            const double2 u = b[i1], v = cmul(w, b[i2]);
-            b[i1] = cadd(u, v);
-            b[i2] = csub(u, v);
+// In-place decimation-in-time DFT on `nseq` sequences of length 2^lg stored with
+// stride `ld` (complex), input in bit-reversed order; sign = -1 forward, +1
+// inverse; tw[k] = e^{-2 pi i k / Nf}, twstep = Nf / 2^lg.
+// The lg radix-2 levels are done in groups of up to three: one task loads the
+// 8 elements j + t h (t < 8) of a block, runs the group's levels in registers
+// and stores them back, so a length-512 transform takes 3 LDS sweeps and 3
+// barriers.  The arithmetic is exactly that of the radix-2 levels.
+template <int G>
+__device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg, int s0, int sign,
+                                         const double2 *tw, int twstep) {
+    constexpr int T = 1 << G;
+    const int L = 1 << lg, h = 1 << (s0 - 1), span = h * T;
+    const int tasks_per_seq = L / T, total = nseq * tasks_per_seq;
+    for (int task = threadIdx.x; task < total; task += blockDim.x) {
+        const int seq = task / tasks_per_seq, r = task - seq * tasks_per_seq;
+        const int blk = r / h, j = r - blk * h;
+        double2 *b = buf + (size_t)seq * ld + blk * span + j;
+        double2 v[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) v[t] = b[t * h];
+#pragma unroll
+        for (int e = 0; e < G; e++) {
+            const int half = h << e;                        // radix-2 level s0 + e
+            const int twmul = twstep * (L / (2 * half));    // e^{-2 pi i m/(2 half)} = tw[m * twmul]
+#pragma unroll
+            for (int t = 0; t < T; t++)
+                if (!(t & (1 << e))) {
+                    const int m = j + (t & ((1 << e) - 1)) * h;   // position inside the level's group
+                    double2 w = tw[m * twmul];
+                    if (sign > 0) w.y = -w.y;
+                    const double2 u = v[t], x = cmul(w, v[t + (1 << e)]);
+                    v[t] = cadd(u, x);
+                    v[t + (1 << e)] = csub(u, x);
+                }
         }
-        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < T; t++) b[t * h] = v[t];
     }
+    __syncthreads();
+}
+
+__device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const double2 *tw, int twstep) {
+    int s = 1;
+    while (lg - s + 1 >= 3) { fft_group<3>(buf, nseq, ld, lg, s, sign, tw, twstep); s += 3; }
+    if (lg - s + 1 == 2) fft_group<2>(buf, nseq, ld, lg, s, sign, tw, twstep);
+    else if (lg - s + 1 == 1) fft_group<1>(buf, nseq, ld, lg, s, sign, tw, twstep);
 }
 
 // ---- x (reference packed layout) -> Gt[omega][c] (blocked) ----------------------
@@ -84,12 +108,23 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
     double2 *buf = (double2 *)lds_raw;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
     const int N1 = p.N1;
-    for (int t = threadIdx.x; t < CB * N1; t += NT) {
-        const int col = t & (CB - 1), n1 = t >> 3;
-        const int n = n2 + p.N2 * n1;
-        double2 v = make_double2(0.0, 0.0);
-        if (n < p.nw) v = Gt[(size_t)n * ncol + cb * CB + col];
-        buf[col * N1 + bitrev(n1, p.log2N1)] = v;
+    // loads staged through registers in rounds of 8 per thread
+    for (int base = 0; base < CB * N1; base += 8 * NT) {
+        double2 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = base + threadIdx.x + q * NT;
+            v[q] = make_double2(0.0, 0.0);
+            if (t < CB * N1) {
+                const int n = n2 + p.N2 * (t >> 3);
+                if (n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & (CB - 1))];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = base + threadIdx.x + q * NT;
+            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = v[q];
+        }
     }
     __syncthreads();
     lds_fft(buf, CB, N1, p.log2N1, -1, p.tw, p.Nf / N1);
@@ -178,11 +213,27 @@ __global__ __launch_bounds__(NT) void k_fft_mid(double2 *__restrict__ Z, Sigma2P
         }
         return;
     }
-    for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
-        const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
-        const int row = rr ? k1p : k1;
-        const int cb = s < 8 ? cb_ij : cb_ji;
-        buf[s * LD + rr * N2 + bitrev(n2, lg2)] = Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)];
+    for (int base = 0; base < 16 * nrow * N2; base += 8 * NT) {
+        double2 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = base + threadIdx.x + q * NT;
+            v[q] = make_double2(0.0, 0.0);
+            if (t < 16 * nrow * N2) {
+                const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+                const int row = rr ? k1p : k1;
+                const int cb = s < 8 ? cb_ij : cb_ji;
+                v[q] = Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = base + threadIdx.x + q * NT;
+            if (t < 16 * nrow * N2) {
+                const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+                buf[s * LD + rr * N2 + bitrev(n2, lg2)] = v[q];
+            }
+        }
     }
     __syncthreads();
     // forward DFT over n2: 16 * nrow sequences of length N2 (rows packed with stride N2)
@@ -239,9 +290,19 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
     double2 *buf = (double2 *)lds_raw;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
     const int N1 = p.N1;
-    for (int t = threadIdx.x; t < CB * N1; t += NT) {
-        const int col = t & (CB - 1), k1 = t >> 3;
-        buf[col * N1 + bitrev(k1, p.log2N1)] = Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col];
+    for (int base = 0; base < CB * N1; base += 8 * NT) {
+        double2 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = base + threadIdx.x + q * NT;
+            v[q] = make_double2(0.0, 0.0);
+            if (t < CB * N1) v[q] = Z[((size_t)(t >> 3) * p.N2 + n2) * ncol + cb * CB + (t & (CB - 1))];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = base + threadIdx.x + q * NT;
+            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = v[q];
+        }
     }
     __syncthreads();
     lds_fft(buf, CB, N1, p.log2N1, +1, p.tw, p.Nf / N1);

@@ -213,6 +213,95 @@ __global__ __launch_bounds__(NT) void k_update_f64(int64_t n, const double *__re
     }
 }
 
+
+// FP32 history, 4 consecutive elements per thread: every column is read with
+// 16-byte loads, so each thread keeps ~30 independent loads in flight.
+// n4 = n / 4 quads; the n % 4 tail is handled by the scalar kernel.
+template <int MH>
+__global__ __launch_bounds__(NT) void k_hist_f32x4(int64_t n, int64_t n4, const double *__restrict__ F,
+                                                   const double *__restrict__ Fold, const double *__restrict__ x,
+                                                   const double *__restrict__ xold, double rf, double alpha,
+                                                   float *__restrict__ dFs, float *__restrict__ Us, int slot, int nh,
+                                                   double *__restrict__ part) {
+    __shared__ double red[NT / QS_WAVE][2 * MAXH];
+    double g[MH], pr[MH];
+#pragma unroll
+    for (int k = 0; k < MH; k++) { g[k] = 0.0; pr[k] = 0.0; }
+    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n4; q += (int64_t)gridDim.x * NT) {
+        const int64_t i = 4 * q;
+        const double4 f = *(const double4 *)(F + i);
+        float4 df = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (slot >= 0) {
+            const double4 fo = *(const double4 *)(Fold + i);
+            const double4 xv = *(const double4 *)(x + i);
+            const double4 xo = *(const double4 *)(xold + i);
+            const double d0 = f.x - fo.x, d1 = f.y - fo.y, d2 = f.z - fo.z, d3 = f.w - fo.w;
+            df = make_float4((float)(d0 * rf), (float)(d1 * rf), (float)(d2 * rf), (float)(d3 * rf));
+            const float4 u = make_float4((float)(alpha * d0 * rf + (xv.x - xo.x) * rf),
+                                         (float)(alpha * d1 * rf + (xv.y - xo.y) * rf),
+                                         (float)(alpha * d2 * rf + (xv.z - xo.z) * rf),
+                                         (float)(alpha * d3 * rf + (xv.w - xo.w) * rf));
+            *(float4 *)(dFs + (size_t)slot * n + i) = df;
+            *(float4 *)(Us + (size_t)slot * n + i) = u;
+        }
+        float4 hv[MH];
+#pragma unroll
+        for (int k = 0; k < MH; k++)
+            if (k < nh && k != slot) hv[k] = *(const float4 *)(dFs + (size_t)k * n + i);
+#pragma unroll
+        for (int k = 0; k < MH; k++)
+            if (k < nh) {
+                const float4 h = (k == slot) ? df : hv[k];
+                g[k] = fma((double)h.x, (double)df.x, g[k]);
+                g[k] = fma((double)h.y, (double)df.y, g[k]);
+                g[k] = fma((double)h.z, (double)df.z, g[k]);
+                g[k] = fma((double)h.w, (double)df.w, g[k]);
+                pr[k] = fma((double)h.x, f.x, pr[k]);
+                pr[k] = fma((double)h.y, f.y, pr[k]);
+                pr[k] = fma((double)h.z, f.z, pr[k]);
+                pr[k] = fma((double)h.w, f.w, pr[k]);
+            }
+    }
+    const int w = threadIdx.x / QS_WAVE, lane = threadIdx.x & (QS_WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < MH; k++)
+        if (k < nh) {
+            const double a = wave_sum(g[k]), b = wave_sum(pr[k]);
+            if (lane == 0) { red[w][k] = a; red[w][MAXH + k] = b; }
+        }
+    __syncthreads();
+    if (threadIdx.x < 2 * MAXH) {
+        double s = 0.0;
+        for (int ww = 0; ww < NT / QS_WAVE; ww++) s += ((threadIdx.x & (MAXH - 1)) < nh) ? red[ww][threadIdx.x] : 0.0;
+        part[(size_t)blockIdx.x * 2 * MAXH + threadIdx.x] = s;
+    }
+}
+
+template <int MH>
+__global__ __launch_bounds__(NT) void k_update_f32x4(int64_t n, int64_t n4, const double *__restrict__ x,
+                                                     const double *__restrict__ F, const float *__restrict__ Us,
+                                                     Gam gm, int nh, double alpha, double *__restrict__ xnew) {
+    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n4; q += (int64_t)gridDim.x * NT) {
+        const int64_t i = 4 * q;
+        const double4 xv = *(const double4 *)(x + i);
+        const double4 f = *(const double4 *)(F + i);
+        float4 hv[MH];
+#pragma unroll
+        for (int k = 0; k < MH; k++)
+            if (k < nh) hv[k] = *(const float4 *)(Us + (size_t)k * n + i);
+        double v0 = xv.x + alpha * f.x, v1 = xv.y + alpha * f.y, v2 = xv.z + alpha * f.z, v3 = xv.w + alpha * f.w;
+#pragma unroll
+        for (int k = 0; k < MH; k++)
+            if (k < nh) {
+                v0 = v0 - (double)hv[k].x * gm.g[k];
+                v1 = v1 - (double)hv[k].y * gm.g[k];
+                v2 = v2 - (double)hv[k].z * gm.g[k];
+                v3 = v3 - (double)hv[k].w * gm.g[k];
+            }
+        *(double4 *)(xnew + i) = make_double4(v0, v1, v2, v3);
+    }
+}
+
 __global__ __launch_bounds__(NT) void k_fill(double *p, int64_t n, double v) {
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) p[i] = v;
 }
@@ -254,7 +343,10 @@ hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *
                                       const double *xold, double rf, double alpha, float *dFs, float *Us, int slot,
                                       int nh, double *partials, int nblocks, hipStream_t s) {
     if (nh > MAXH) return hipErrorInvalidValue;
-    if (nh <= 16)
+    if (n % 4 == 0 && nh <= 16)
+        hipLaunchKernelGGL(k_hist_f32x4<16>, dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold, rf, alpha, dFs,
+                           Us, slot, nh, partials);
+    else if (nh <= 16)
         hipLaunchKernelGGL(k_hist_f32<16>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us, slot,
                            nh, partials);
     else
@@ -281,7 +373,11 @@ hipError_t qs_launch_broyden_update_f32(int64_t n, const double *x, const double
     if (nh > MAXH) return hipErrorInvalidValue;
     Gam gm;
     for (int k = 0; k < MAXH; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
-    hipLaunchKernelGGL(k_update_f32, dim3(qs_vec_blocks(n)), dim3(NT), 0, s, n, x, F, Us, gm, nh, alpha, xnew);
+    if (n % 4 == 0 && nh <= 16)
+        hipLaunchKernelGGL(k_update_f32x4<16>, dim3(qs_vec_blocks(n / 4)), dim3(NT), 0, s, n, n / 4, x, F, Us, gm, nh,
+                           alpha, xnew);
+    else
+        hipLaunchKernelGGL(k_update_f32, dim3(qs_vec_blocks(n)), dim3(NT), 0, s, n, x, F, Us, gm, nh, alpha, xnew);
     return hipGetLastError();
 }
 

@@ -273,9 +273,9 @@ class ScfEngine:
 
     def residual2(self, V, x):
         x = np.ascontiguousarray(x, dtype=float)
-        F = np.zeros(self.n_vec)
+        F = np.zeros(len(x))
         ab = ctypes.c_int32()
-        self._check(self.L.qs_residual2(self.h, float(V), self.n_vec, _ptr(x), _ptr(F), ctypes.byref(ab)))
+        self._check(self.L.qs_residual2(self.h, float(V), len(x), _ptr(x), _ptr(F), ctypes.byref(ab)))
         return F, bool(ab.value)
 
     def sigma(self):

@@ -17,18 +17,18 @@ program qs_fortran_demo
 
    H = (0.d0, 0.d0); GL = (0.d0, 0.d0); GR = (0.d0, 0.d0); U = 0.d0
    do i = 1, nsite
-      H(2*i - 1, 2*i - 1) = 0.2d0; H(2*i, 2*i) = 0.2d0
+      H(2*i - 1, 2*i - 1) = 2.d0; H(2*i, 2*i) = 2.d0
       if (i < nsite) then
-         H(2*i - 1, 2*i + 1) = -1.d0; H(2*i + 1, 2*i - 1) = -1.d0
-         H(2*i, 2*i + 2) = -1.d0; H(2*i + 2, 2*i) = -1.d0
+         H(2*i - 1, 2*i + 1) = -0.4d0; H(2*i + 1, 2*i - 1) = -0.4d0
+         H(2*i, 2*i + 2) = -0.4d0; H(2*i + 2, 2*i) = -0.4d0
       end if
-      U(2*i - 1) = 1.d0
+      U(2*i - 1) = 0.8d0
    end do
-   GL(1, 1) = 0.2d0; GL(2, 2) = 0.15d0; GR(norb - 1, norb - 1) = 0.2d0; GR(norb, norb) = 0.2d0
+   GL(1, 1) = 1.5d0; GL(2, 2) = 1.d0; GR(norb - 1, norb - 1) = 1.d0; GR(norb, norb) = 1.d0
 
-   p%norb = norb; p%n_omega = nw; p%w_init = -4.d0; p%delta = 8.d0/nw
+   p%norb = norb; p%n_omega = nw; p%w_init = 0.d0; p%delta = 4.d0/nw
    p%H = c_loc(H); p%gammaL = c_loc(GL); p%gammaR = c_loc(GR); p%U_site = c_loc(U)
-   p%order = 2; p%beta = 1.d0/(8.6173303d-5*300.d0); p%mu = 0.d0
+   p%order = 2; p%beta = 1.d0/(8.6173303d-5*300.d0); p%mu = 2.d0
    p%n_atoms = 0; p%atom_norb = c_null_ptr
    p%alpha_mix = 0.5d0; p%nhist = 8; p%maxscf = 200; p%tolf = 1.d-10; p%tolf_order1 = 1.d-8
    p%single_history = 1; p%device = 0

@@ -19,9 +19,10 @@ def make_requests():
     reqs, probs = [], []
     shapes = [(2, 2, 256), (4, 2, 512), (4, 1, 1024), (6, 4, 256), (4, 2, 256), (2, 3, 512), (4, 3, 384), (2, 2, 1024)]
     for rid, (ni, nt, nw) in enumerate(shapes):
-        U = float(rng.uniform(0.5, 2.0))
+        U = float(rng.uniform(0.3, 1.0))
         V = float(rng.uniform(-1.0, 1.0))
-        p = helix_problem(ni, nt, U, order=2, n_omega=nw, wlo=-5.0, whi=5.0, tolf=1e-10, nhist=8, voltages=[V])
+        p = helix_problem(ni, nt, U, order=2, n_omega=nw, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
+                          gamma=1.0, mu=2.0, tolf=1e-10, nhist=8, voltages=[V])
         probs.append(p)
         sp = pkg.ScfProblem(H=p.H, gammaL=p.GL, gammaR=p.GR, U_site=p.U_site, order=2, n_omega=p.n_omega,
                             w_init=p.w_init, delta=p.delta, beta=p.beta, mu=p.mu, nhist=8, tolf=1e-10)
