@@ -96,10 +96,14 @@ def workload(name):
 
 def stage_counts(eng, nh):
     """Algorithmic work per launch of each stage (flops or HBM bytes)."""
-    nw, norb, nA, nv = eng.prob.n_omega, eng.norb, eng.nA, eng.n_vec
+    nw, norb, nA, nv, Nf = eng.prob.n_omega, eng.norb, eng.nA, eng.n_vec, eng.Nf
+    ncol = 2.0 * nA * nA          # complex columns of the blocked Sigma2 layout
     return {
         "dyson_keldysh": ("mfma", nw * (8.0 * norb ** 3 + 16.0 * nA ** 3), "flop"),
-        "sigma2_fft": ("hbm", 64.0 * nA * nA * nw, "byte"),
+        # each pass reads its input once and writes its output once
+        "sigma2_fft_f1": ("hbm", 16.0 * ncol * (nw + Nf), "byte"),
+        "sigma2_fft_mid": ("hbm", 32.0 * ncol * Nf, "byte"),
+        "sigma2_fft_i2": ("hbm", 16.0 * ncol * (Nf + nw), "byte"),
         "broyden_history": ("hbm", nv * (32.0 + 4.0 * (nh - 1) + 8.0), "byte"),
         "broyden_update": ("hbm", nv * (16.0 + 4.0 * nh + 8.0), "byte"),
         "permute_in": ("hbm", nv * 8.0 + 32.0 * nA * nA * nw, "byte"),
@@ -211,6 +215,21 @@ def serve_bench(args):
     return 0
 
 
+def copy_calibration():
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch scale
+    the same counters read on the kernels of the path to HBM bytes."""
+    import torch
+    n = 1 << 28                      # 2^28 doubles = 2 GiB each way
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +241,10 @@ def main():
     ap.add_argument("--serve", default="", help="comma-separated Poisson arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--maxscf", type=int, default=0,
+                    help="cap on SCF steps per bias point (counter-collection passes; 0 = the workload's 300)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain device copy of 2 GiB five times (scale for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (one bias point is one "
@@ -229,9 +252,13 @@ def main():
         return 2
     if args.serve:
         return serve_bench(args)
+    if args.calibrate_copy:
+        return copy_calibration()
     import importlib
     pkg = importlib.import_module("quichotte-and-sancho_amd")
     prob, volts, desc = workload(args.workload)
+    if args.maxscf > 0:
+        prob["maxscf"] = args.maxscf
     nsteps = args.warmup + args.steps
     if nsteps > len(volts):
         raise SystemExit("at most %d steps for this workload" % len(volts))

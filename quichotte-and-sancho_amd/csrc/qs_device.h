@@ -40,8 +40,8 @@ __device__ __forceinline__ void cfmac(double2 &acc, double2 a, double2 b) {
     acc.y = fma(a.y, b.x, acc.y);
 }
 __device__ __forceinline__ double2 cinv(double2 a) {
-    double d = a.x * a.x + a.y * a.y;
-    return make_double2(a.x / d, -a.y / d);
+    const double r = 1.0 / (a.x * a.x + a.y * a.y);
+    return make_double2(a.x * r, -a.y * r);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -47,16 +47,16 @@ __device__ __forceinline__ int sig_block_index(int a, int b, int nAs) {
 struct LdsLayout {
     double2 *big;   // n x (n+1)
     double2 *chunk; // 16 x 16R
-    int *piv, *qinv, *spc;
+    int *piv, *qinv, *spc, *actl;
     double *redv;
     int *redi;
     double *scratch;
     double2 *grcl;
 };
 
-// LDS: big (n x (n+1)), chunk (16 x 16 RB >= 16 x n), 3 int maps of 16 RB,
+// LDS: big (n x (n+1)), chunk (16 x 16 RB >= 16 x n), 4 int maps of 16 RB,
 // 32 pivot candidates, 16 doubles of reduction scratch, the nCL x nCL block.
-// M (48 RB ints) keeps 8-byte alignment for every RB; grcl stays 16-byte aligned.
+// The maps (64 RB ints) keep 8-byte alignment for every RB; grcl stays 16-byte aligned.
 __device__ __forceinline__ LdsLayout lds_layout(char *base, int n, int RB) {
     LdsLayout L;
     L.big = (double2 *)base;
@@ -64,21 +64,12 @@ __device__ __forceinline__ LdsLayout lds_layout(char *base, int n, int RB) {
     L.piv = (int *)(L.chunk + 16 * 16 * RB);
     L.qinv = L.piv + 16 * RB;
     L.spc = L.qinv + 16 * RB;
-    L.redv = (double *)(L.spc + 16 * RB);
+    L.actl = L.spc + 16 * RB;
+    L.redv = (double *)(L.actl + 16 * RB);
     L.redi = (int *)(L.redv + 32);
     L.scratch = (double *)(L.redi + 32);
     L.grcl = (double2 *)(L.scratch + 16);
     return L;
-}
-
-// SL(b1, b2) on the support: i (f_L GamL + f_R GamR) + Sigma2^<  (src/scf.f90:333-336)
-__device__ __forceinline__ double2 sl_elem(const DysonArgs &A, int b1, int b2, double fL, double fR,
-                                           const double2 *sigw) {
-    double2 gl = A.gamL_s[b1 * A.nS + b2], gr = A.gamR_s[b1 * A.nS + b2];
-    double re = fL * gl.x + fR * gr.x, im = fL * gl.y + fR * gr.y;
-    double2 v = make_double2(-im, re);
-    if (sigw) v = cadd(v, sigw[sig_block_index(b1, b2, A.nAs) + 4]);
-    return v;
 }
 
 template <int TI, int RA, int RB>
@@ -95,24 +86,50 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
 
     // ---- 1. A = w + Hemb - Sigma1 - Sigma2^r, in registers -------------------------
+    // Every load of the tile is issued before any is used (clamped in-bounds
+    // addresses and selects, no branches), so the tile costs two memory round
+    // trips (pos, then Sigma2) rather than one per element.
+    int pr[RA], pc[RB];
+#pragma unroll
+    for (int a = 0; a < RA; a++) pr[a] = A.pos[min(ti + TI * a, n - 1)];
+#pragma unroll
+    for (int b = 0; b < RB; b++) pc[b] = A.pos[min(tj + 16 * b, n - 1)];
     double2 M[RA][RB];
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++) M[a][b] = A.Hemb[min(ti + TI * a, n - 1) * n + min(tj + 16 * b, n - 1)];
+    const bool use_s1 = A.use_sigma && A.sig1, use_s2 = A.use_sigma && sigw;
+    double2 s1v[RA][RB];
+    if (use_s1) {
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const int i = min(ti + TI * a, n - 1), j = min(tj + 16 * b, n - 1);
+                s1v[a][b] = A.sig1[((i >> 1) == (j >> 1)) ? (i >> 1) * 4 + (i & 1) * 2 + (j & 1) : 0];
+            }
+    }
+    double2 s2v[RA][RB];
+    if (use_s2) {
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const bool ok = pr[a] >= 0 && pc[b] >= 0;
+                s2v[a][b] = sigw[ok ? sig_block_index(pr[a], pc[b], A.nAs) : 0];
+            }
+    }
 #pragma unroll
     for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < RB; b++) {
             const int i = ti + TI * a, j = tj + 16 * b;
-            double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
-            if (i < n && j < n) {
-                v = A.Hemb[i * n + j];
-                if (i == j) v.x += w;
-                if (A.use_sigma) {
-                    if ((i >> 1) == (j >> 1) && A.sig1) v = csub(v, A.sig1[(i >> 1) * 4 + (i & 1) * 2 + (j & 1)]);
-                    if (sigw) {
-                        const int pa = A.pos[i], pb = A.pos[j];
-                        if (pa >= 0 && pb >= 0) v = csub(v, sigw[sig_block_index(pa, pb, A.nAs)]);
-                    }
-                }
-            }
+            double2 v = M[a][b];
+            if (i == j) v.x += w;
+            if (use_s1 && (i >> 1) == (j >> 1)) v = csub(v, s1v[a][b]);
+            if (use_s2 && pr[a] >= 0 && pc[b] >= 0) v = csub(v, s2v[a][b]);
+            if (i >= n || j >= n) v = make_double2(i == j ? 1.0 : 0.0, 0.0);
             M[a][b] = v;
         }
     unsigned used = 0;  // bit a: my row ti + TI a already pivoted
@@ -123,6 +140,11 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
     double2 *prow = L.chunk, *pcol = L.chunk + 16 * RB;
 
     // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
+    // Step k: the owners of column k post their best unused row, every wave
+    // reduces the TI candidates, the pivot row and column go through LDS, and
+    // each thread applies M -= M(:,k) (M(p,:) / M(p,k)) to its tile with
+    // column k zeroed first, so that it becomes -M(:,k) / M(p,k); row p is then
+    // replaced by M(p,:) / M(p,k) with 1 / M(p,k) at column k.
     const int lane = tid & 63;
     for (int k = 0; k < n; k++) {
         const int bk = k >> 4, tk = k & 15;
@@ -130,21 +152,21 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
             double best = -1.0;
             int bi = -1;
 #pragma unroll
-            for (int a = 0; a < RA; a++)
+            for (int b = 0; b < RB; b++)
+                if (b == bk)
 #pragma unroll
-                for (int b = 0; b < RB; b++)
-                    if (b == bk && !(used & (1u << a))) {
-                        double v = fabs(M[a][b].x) + fabs(M[a][b].y);
-                        if (v != v) v = -0.5;  // NaN never wins over a number
-                        if (bi < 0 || v > best) { best = v; bi = ti + TI * a; }
-                    }
+                    for (int a = 0; a < RA; a++)
+                        if (!(used & (1u << a))) {
+                            double v = fabs(M[a][b].x) + fabs(M[a][b].y);
+                            if (v != v) v = -0.5;  // NaN never wins over a number
+                            if (bi < 0 || v > best) { best = v; bi = ti + TI * a; }
+                        }
             L.redv[ti] = best;
             L.redi[ti] = bi;
         }
         __syncthreads();
         // every wavefront reduces the TI candidates (largest value, then smallest row)
-        double best = -2.0;
-        int p = -1;
+        int p;
         {
             const int t = lane & (TI - 1);
             int ii = L.redi[t];
@@ -156,10 +178,8 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
                 const int io = __shfl_xor(ii, o, 64);
                 if (vo > v || (vo == v && io < ii)) { v = vo; ii = io; }
             }
-            best = v;
-            p = ii;
+            p = __builtin_amdgcn_readfirstlane(ii);
         }
-        (void)best;
         const int tp = p % TI, ap = p / TI;
         if (ti == tp) {
 #pragma unroll
@@ -185,27 +205,26 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
         for (int b = 0; b < RB; b++) sp[b] = cmul(prow[tj + 16 * b], rinv);
 #pragma unroll
         for (int a = 0; a < RA; a++) fr[a] = pcol[ti + TI * a];
+        if (tj == tk) {
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+                if (b == bk) {
+                    sp[b] = rinv;
+#pragma unroll
+                    for (int a = 0; a < RA; a++) M[a][b] = make_double2(0.0, 0.0);
+                }
+        }
 #pragma unroll
         for (int a = 0; a < RA; a++)
 #pragma unroll
             for (int b = 0; b < RB; b++) cfms(M[a][b], fr[a], sp[b]);
-        if (tj == tk) {
-#pragma unroll
-            for (int a = 0; a < RA; a++)
-#pragma unroll
-                for (int b = 0; b < RB; b++)
-                    if (b == bk) {
-                        double2 t = cmul(fr[a], rinv);
-                        M[a][b] = make_double2(-t.x, -t.y);
-                    }
-        }
         if (ti == tp) {
 #pragma unroll
             for (int a = 0; a < RA; a++)
                 if (a == ap) {
                     used |= 1u << a;
 #pragma unroll
-                    for (int b = 0; b < RB; b++) M[a][b] = (b == bk && tj == tk) ? rinv : sp[b];
+                    for (int b = 0; b < RB; b++) M[a][b] = sp[b];
                 }
         }
     }
@@ -220,31 +239,50 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
             if (r < n && c < n) L.big[L.qinv[r] * pitch + L.piv[c]] = M[a][b];
         }
     for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
+    for (int c = tid; c < nA; c += NT) L.actl[c] = A.act[c];
     __syncthreads();
 
     const size_t nblk = 3 * (size_t)nA2, off = (size_t)iw * nblk;
     double st_max = 0.0, st_s2 = 0.0, st_d2 = 0.0;
-    if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
+    // Packing epilogues read x and Fold in batches of EU elements per thread,
+    // all loads of a batch ahead of its stores.
+    constexpr int EU = 8;
+    if (A.mode == QS_M_INIT2) {
         for (int e = tid; e < nA2; e += NT) {
             const int a = e / nA, b = e - a * nA;
-            const double2 g = L.big[A.act[a] * pitch + A.act[b]];
-            if (A.mode == QS_M_INIT2) {
-                A.xout[off + e] = g.x;
-                A.xout[off + nA2 + e] = g.y;
-            } else {
-                const size_t d0 = off + e, d1 = off + nA2 + e;
-                const double f0 = g.x - A.x[d0], f1 = g.y - A.x[d1];
-                A.F[d0] = f0;
-                A.F[d1] = f1;
-                st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
-                st_s2 += f0 * f0 + f1 * f1;
-                const double e0 = f0 - A.Fold[d0], e1 = f1 - A.Fold[d1];
-                st_d2 += e0 * e0 + e1 * e1;
-                if (a == b) A.diagR[(size_t)iw * nA + a] = g.y;
+            const double2 g = L.big[L.actl[a] * pitch + L.actl[b]];
+            A.xout[off + e] = g.x;
+            A.xout[off + nA2 + e] = g.y;
+        }
+    } else if (A.mode == QS_M_SCF2) {
+        for (int e0 = tid; e0 < nA2; e0 += EU * NT) {
+            double x0[EU], x1[EU], o0[EU], o1[EU];
+#pragma unroll
+            for (int u = 0; u < EU; u++) {
+                const size_t d0 = off + min(e0 + u * NT, nA2 - 1), d1 = d0 + nA2;
+                x0[u] = A.x[d0]; x1[u] = A.x[d1];
+                o0[u] = A.Fold[d0]; o1[u] = A.Fold[d1];
+            }
+#pragma unroll
+            for (int u = 0; u < EU; u++) {
+                const int e = e0 + u * NT;
+                if (e < nA2) {
+                    const int a = e / nA, b = e - a * nA;
+                    const double2 g = L.big[L.actl[a] * pitch + L.actl[b]];
+                    const size_t d0 = off + e, d1 = off + nA2 + e;
+                    const double f0 = g.x - x0[u], f1 = g.y - x1[u];
+                    A.F[d0] = f0;
+                    A.F[d1] = f1;
+                    st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
+                    st_s2 += f0 * f0 + f1 * f1;
+                    const double e0v = f0 - o0[u], e1v = f1 - o1[u];
+                    st_d2 += e0v * e0v + e1v * e1v;
+                    if (a == b) A.diagR[(size_t)iw * nA + a] = g.y;
+                }
             }
         }
     } else if (A.mode == QS_M_SCF1) {
-        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = L.big[A.act[k2] * pitch + A.act[k2]].y;
+        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = L.big[L.actl[k2] * pitch + L.actl[k2]].y;
     } else {
         for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = L.big[i * pitch + i].y;
         for (int e = tid; e < A.nCL * A.nCL; e += NT) {
@@ -267,18 +305,45 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
     for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < RB; b++) acc[a][b] = make_double2(0.0, 0.0);
+    // S'' rows k0 .. k0+15 are fetched into registers one chunk ahead, so their
+    // memory latency hides behind the previous chunk's FMA loop.
+    constexpr int QF = (16 * 16 * RB + NT - 1) / NT;
+    double2 fgl[QF], fgr[QF], fsg[QF];
+    unsigned fok = 0;  // bit q: element q lies on the support (else S'' is zero there)
+    auto fetch = [&](int k0) {
+        fok = 0;
+#pragma unroll
+        for (int q = 0; q < QF; q++) {
+            const int e = tid + q * NT, kk = e / n, c = e - kk * n, k = k0 + kk;
+            const bool in = e < 16 * n && k < n;
+            const int b1 = in ? L.spc[k] : -1, b2 = in ? L.spc[c] : -1;
+            const bool ok = b1 >= 0 && b2 >= 0;
+            const int gi = ok ? b1 * A.nS + b2 : 0;
+            fgl[q] = A.gamL_s[gi];
+            fgr[q] = A.gamR_s[gi];
+            fsg[q] = sigw ? sigw[ok ? sig_block_index(b1, b2, A.nAs) + 4 : 0] : make_double2(0.0, 0.0);
+            if (ok) fok |= 1u << q;
+        }
+    };
+    fetch(0);
     for (int k0 = 0; k0 < n; k0 += 16) {
         __syncthreads();
-        for (int e = tid; e < 16 * n; e += NT) {
-            const int kk = e / n, c = e - kk * n, k = k0 + kk;
-            double2 v = make_double2(0.0, 0.0);
-            if (k < n) {
-                const int b1 = L.spc[k], b2 = L.spc[c];
-                if (b1 >= 0 && b2 >= 0) v = sl_elem(A, b1, b2, fL, fR, sigw);
+#pragma unroll
+        for (int q = 0; q < QF; q++) {
+            const int e = tid + q * NT;
+            if (e < 16 * n) {
+                double2 v = make_double2(0.0, 0.0);
+                if (fok & (1u << q)) {
+                    // i (f_L GamL + f_R GamR) + Sigma2^<  (src/scf.f90:333-336)
+                    const double re = fL * fgl[q].x + fR * fgr[q].x, im = fL * fgl[q].y + fR * fgr[q].y;
+                    v = make_double2(-im, re);
+                    if (sigw) v = cadd(v, fsg[q]);
+                }
+                L.chunk[e] = v;
             }
-            L.chunk[kk * n + c] = v;
         }
         __syncthreads();
+        if (k0 + 16 < n) fetch(k0 + 16);
         const int kmax = min(16, n - k0);
         for (int kk = 0; kk < kmax; kk++) {
             double2 av[RA], bv[RB];
@@ -352,21 +417,40 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
 
     if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
         const int nsu = (nA2 - nA) / 2;
-        for (int e = tid; e < nA2; e += NT) {
-            const int code = A.tri_map[e], i = code >> 16, j = code & 0xffff;
-            const double2 g = L.big[A.act[i] * pitch + A.act[j]];
-            const double val = e < nsu ? g.x : g.y;
-            const size_t d = off + 2 * (size_t)nA2 + e;
-            if (A.mode == QS_M_INIT2) {
-                A.xout[d] = val;
-            } else {
-                const double f = val - A.x[d];
-                A.F[d] = f;
-                st_max = fmax(st_max, fabs(f));
-                st_s2 += f * f;
-                const double e0 = f - A.Fold[d];
-                st_d2 += e0 * e0;
-                if (e >= nsu && i == j) A.diagL[(size_t)iw * nA + i] = g.y;
+        const bool scf2 = A.mode == QS_M_SCF2;
+        for (int e0 = tid; e0 < nA2; e0 += EU * NT) {
+            int code[EU];
+            double xv[EU], ov[EU];
+#pragma unroll
+            for (int u = 0; u < EU; u++) {
+                const int ec = min(e0 + u * NT, nA2 - 1);
+                code[u] = A.tri_map[ec];
+                if (scf2) {
+                    const size_t d = off + 2 * (size_t)nA2 + ec;
+                    xv[u] = A.x[d];
+                    ov[u] = A.Fold[d];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < EU; u++) {
+                const int e = e0 + u * NT;
+                if (e < nA2) {
+                    const int i = code[u] >> 16, j = code[u] & 0xffff;
+                    const double2 g = L.big[L.actl[i] * pitch + L.actl[j]];
+                    const double val = e < nsu ? g.x : g.y;
+                    const size_t d = off + 2 * (size_t)nA2 + e;
+                    if (!scf2) {
+                        A.xout[d] = val;
+                    } else {
+                        const double f = val - xv[u];
+                        A.F[d] = f;
+                        st_max = fmax(st_max, fabs(f));
+                        st_s2 += f * f;
+                        const double e0v = f - ov[u];
+                        st_d2 += e0v * e0v;
+                        if (e >= nsu && i == j) A.diagL[(size_t)iw * nA + i] = g.y;
+                    }
+                }
             }
         }
         if (A.mode == QS_M_SCF2) {
@@ -380,7 +464,7 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
         }
     } else {
         if (A.mode == QS_M_SCF1)
-            for (int k2 = tid; k2 < nA; k2 += NT) A.diagL[(size_t)iw * nA + k2] = L.big[A.act[k2] * pitch + A.act[k2]].y;
+            for (int k2 = tid; k2 < nA; k2 += NT) A.diagL[(size_t)iw * nA + k2] = L.big[L.actl[k2] * pitch + L.actl[k2]].y;
         for (int e = tid; e < 4 * A.nsites; e += NT) {
             const int si = e >> 2, a = e & 1, b = (e >> 1) & 1;
             A.onsL[(size_t)iw * 4 * A.nsites + e] = L.big[(2 * si + a) * pitch + 2 * si + b];
@@ -437,7 +521,7 @@ hipError_t launch_v(const DysonArgs &a, int count, hipStream_t s) {
 size_t qs_dyson_lds_bytes(int n, int nCL) {
     const int RB = (n + 15) / 16;
     size_t b = (size_t)n * (n + 1) * 16 + (size_t)16 * 16 * RB * 16;
-    b += (size_t)3 * 16 * RB * 4 + 32 * 8 + 32 * 4 + 16 * 8;
+    b += (size_t)4 * 16 * RB * 4 + 32 * 8 + 32 * 4 + 16 * 8;
     b = (b + 15) & ~(size_t)15;
     b += (size_t)nCL * nCL * 16;
     return b;

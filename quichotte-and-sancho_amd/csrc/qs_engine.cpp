@@ -34,9 +34,13 @@ constexpr double PI = 3.14159265358979323846;
 constexpr int MAXH_DEV = 32;
 
 // stages timed with HIP events on the handle's stream (qs_profile)
-enum Stage { ST_EQT = 0, ST_PERMUTE, ST_SIGMA2, ST_DYSON, ST_REDUCE, ST_HIST, ST_UPDATE, ST_FINAL, ST_COUNT };
-const char *const kStageNames[ST_COUNT] = {"equal_time_sigma1", "permute_in",     "sigma2_fft",    "dyson_keldysh",
-                                           "reductions",        "broyden_history", "broyden_update", "final_sweep"};
+enum Stage {
+    ST_EQT = 0, ST_PERMUTE, ST_FFT_F1, ST_FFT_MID, ST_FFT_I2, ST_DYSON, ST_REDUCE, ST_HIST, ST_UPDATE, ST_FINAL,
+    ST_COUNT
+};
+const char *const kStageNames[ST_COUNT] = {"equal_time_sigma1", "permute_in",      "sigma2_fft_f1",  "sigma2_fft_mid",
+                                           "sigma2_fft_i2",     "dyson_keldysh",   "reductions",     "broyden_history",
+                                           "broyden_update",    "final_sweep"};
 
 struct Prof {
     bool on = false;
@@ -510,9 +514,11 @@ int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, S
     Sigma2Plan sp{};
     sp.nA = nA; sp.nAs = h->nAs; sp.nw = h->nw; sp.Nf = h->Nf; sp.N1 = h->N1; sp.N2 = h->N2;
     sp.log2N1 = h->log2N1; sp.log2N2 = h->log2N2; sp.delta = h->p.delta; sp.tw = h->d_tw; sp.Upair = h->d_Upair;
-    t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_sigma2(sp, h->d_Gt, h->d_Z, h->d_Gt, h->stream));
-    prof_end(h, ST_SIGMA2, t0);
+    for (int part = 0; part < 3; part++) {
+        t0 = prof_begin(h);
+        HIPCHK(h, qs_launch_sigma2(sp, h->d_Gt, h->d_Z, h->d_Gt, 1 << part, h->stream));
+        prof_end(h, ST_FFT_F1 + part, t0);
+    }
     h->sigma_valid = true;
     int rc = run_dyson(h, QS_M_SCF2, true, V);
     if (rc) return rc;

@@ -54,7 +54,10 @@ struct Sigma2Plan {
 };
 // x (packed, reference layout) -> Gt [nw][2 nA^2] blocked layout
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int nw, hipStream_t s);
-hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, hipStream_t s);
+// parts: bit 0 forward pass over n1, bit 1 middle pass (row DFTs + 2x2 algebra),
+// bit 2 inverse pass over k1; the three in order make one Sigma2 evaluation.
+hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
+                            hipStream_t s);
 
 // small vector kernels (qs_vector.hip)
 // part: >= 256 * 4 nAs doubles of scratch

@@ -24,6 +24,8 @@
 //   I2: inverse DFT over k1 (length N1) for every (c, n2), keep n < n_omega.
 // Each pass reads and writes HBM once in 128-byte segments; the DFTs run in LDS
 // (radix-2, decimation in time, bit-reversed load).
+#include <cstdlib>
+
 #include "qs_device.h"
 #include "qs_internal.h"
 
@@ -283,6 +285,123 @@ __global__ __launch_bounds__(NT) void k_fft_mid(double2 *__restrict__ Z, Sigma2P
     }
 }
 
+
+// Middle pass with one row of the (k1, -k1) pair held in registers: LDS holds a
+// single row of the 16 columns (16 N2 complex, 64 KB at N2 = 256), so two
+// workgroups share a CU.  Thread t owns position t of row k1 and, for the
+// spin algebra, the partner position (-k1, N2-1-t); both outputs are computed
+// by the same thread from the same two inputs.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid2(double2 *__restrict__ Z, Sigma2Plan p) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double2 *buf = (double2 *)lds_raw;  // [s (16)][N2]
+    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA;
+    int pid = blockIdx.x, ia = 0;
+    while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
+    const int ja = ia + pid;
+    const int k1 = blockIdx.y, k1p = (N1 - k1) & (N1 - 1);
+    const int cb_ij = (ia * nAs + ja) * CB, cb_ji = (ja * nAs + ia) * CB;
+    const double c = p.Upair[ia * nAs + ja];
+    if (c == 0.0) {
+        const int nrow = (k1 == k1p) ? 1 : 2;
+        for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
+            const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+            const int row = rr ? k1p : k1;
+            Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = make_double2(0.0, 0.0);
+        }
+        return;
+    }
+    const int t = threadIdx.x;
+    auto load_row = [&](int row) {
+        for (int base = 0; base < 16 * N2; base += 8 * NT) {
+            double2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = base + threadIdx.x + q * NT;
+                v[q] = make_double2(0.0, 0.0);
+                if (e < 16 * N2) {
+                    const int s = e & 15, n2 = e >> 4;
+                    v[q] = Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = base + threadIdx.x + q * NT;
+                if (e < 16 * N2) buf[(e & 15) * N2 + bitrev(e >> 4, lg2)] = v[q];
+            }
+        }
+        __syncthreads();
+    };
+    auto store_row = [&](int row) {
+        for (int e = threadIdx.x; e < 16 * N2; e += NT) {
+            const int s = e & 15, n2 = e >> 4;
+            const double2 w = cconj(p.tw[row * n2]);
+            Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * N2 + n2], w);
+        }
+    };
+    load_row(k1);
+    lds_fft(buf, 16, N2, lg2, -1, p.tw, p.Nf / N2);
+    if (k1 != k1p) {
+        double2 A[16];
+        if (t < N2)
+#pragma unroll
+            for (int s = 0; s < 16; s++) A[s] = buf[s * N2 + t];
+        __syncthreads();
+        load_row(k1p);
+        lds_fft(buf, 16, N2, lg2, -1, p.tw, p.Nf / N2);
+        const int m2 = N2 - 1 - t;  // partner of (k1, t) is (k1p, N2 - 1 - t)
+        double2 Q[16];
+        if (t < N2)
+#pragma unroll
+            for (int s = 0; s < 16; s++) Q[s] = buf[s * N2 + m2];
+        __syncthreads();
+        if (t < N2) {
+            // Two independent groups keep the live set at A, Q and 8 outputs:
+            // {A0-7, Q8-15} -> (k1, t) blocks 0-7 and (k1p, m2) blocks 8-15,
+            // {A8-15, Q0-7} -> (k1, t) blocks 8-15 and (k1p, m2) blocks 0-7.
+            const int bm = bitrev(m2, lg2);
+            double2 O[8];
+            pair_sigma(Q + 8, Q + 12, A + 0, A + 4, c, O + 0, O + 4);
+#pragma unroll
+            for (int s = 0; s < 8; s++) buf[(8 + s) * N2 + bm] = O[s];
+            pair_sigma(A + 0, A + 4, Q + 8, Q + 12, c, O + 0, O + 4);
+#pragma unroll
+            for (int s = 0; s < 8; s++) A[s] = O[s];
+            pair_sigma(Q + 0, Q + 4, A + 8, A + 12, c, O + 0, O + 4);
+#pragma unroll
+            for (int s = 0; s < 8; s++) buf[s * N2 + bm] = O[s];
+            pair_sigma(A + 8, A + 12, Q + 0, Q + 4, c, O + 0, O + 4);
+#pragma unroll
+            for (int s = 0; s < 8; s++) A[8 + s] = O[s];
+        }
+        __syncthreads();
+        lds_fft(buf, 16, N2, lg2, +1, p.tw, p.Nf / N2);
+        store_row(k1p);
+        __syncthreads();
+        if (t < N2)
+#pragma unroll
+            for (int s = 0; s < 16; s++) buf[s * N2 + bitrev(t, lg2)] = A[s];
+        __syncthreads();
+    } else {
+        // k1 = 0 or N1/2: the partner position is in the same row
+        const int m2 = (k1 == 0) ? ((N2 - t) & (N2 - 1)) : (N2 - 1 - t);
+        double2 O[16];
+        if (t < N2) {
+            double2 P[16], Q[16];
+#pragma unroll
+            for (int s = 0; s < 16; s++) { P[s] = buf[s * N2 + t]; Q[s] = buf[s * N2 + m2]; }
+            pair_sigma(P + 0, P + 4, Q + 8, Q + 12, c, O + 0, O + 4);
+            pair_sigma(P + 8, P + 12, Q + 0, Q + 4, c, O + 8, O + 12);
+        }
+        __syncthreads();
+        if (t < N2)
+#pragma unroll
+            for (int s = 0; s < 16; s++) buf[s * N2 + bitrev(t, lg2)] = O[s];
+        __syncthreads();
+    }
+    lds_fft(buf, 16, N2, lg2, +1, p.tw, p.Nf / N2);
+    store_row(k1);
+}
+
 // ---- I2 -----------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
@@ -323,24 +442,40 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
     return hipGetLastError();
 }
 
-hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, hipStream_t s) {
+hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
+                            hipStream_t s) {
     const int ncol = 2 * p.nA * p.nA, nblk = ncol / CB;
     const size_t lds1 = (size_t)CB * p.N1 * sizeof(double2);
     const size_t ldsm = (size_t)16 * 2 * p.N2 * sizeof(double2);
+    static int mid_variant = -1;
+    if (mid_variant < 0) {
+        const char *e = getenv("QS_FFT_MID");
+        mid_variant = (e && atoi(e) == 1) ? 1 : 2;
+    }
     static bool attr = false;
     if (!attr) {
-        hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_mid2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL(k_fft_f1, dim3(nblk, p.N2), dim3(NT), lds1, s, Gt, Z, p);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int npairs = p.nAs * (p.nAs + 1) / 2;
-    hipLaunchKernelGGL(k_fft_mid, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fft_i2, dim3(nblk, p.N2), dim3(NT), lds1, s, Z, Sig, p);
-    return hipGetLastError();
+    hipError_t e = hipSuccess;
+    if (parts & 1) {
+        hipLaunchKernelGGL(k_fft_f1, dim3(nblk, p.N2), dim3(NT), lds1, s, Gt, Z, p);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (parts & 2) {
+        const int npairs = p.nAs * (p.nAs + 1) / 2;
+        if (mid_variant == 1)
+            hipLaunchKernelGGL(k_fft_mid, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
+        else
+            hipLaunchKernelGGL(k_fft_mid2, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm / 2, s, Z, p);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (parts & 4) {
+        hipLaunchKernelGGL(k_fft_i2, dim3(nblk, p.N2), dim3(NT), lds1, s, Z, Sig, p);
+        e = hipGetLastError();
+    }
+    return e;
 }

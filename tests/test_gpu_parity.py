@@ -128,8 +128,9 @@ def test_runaway_and_errors_are_reported():
     g = gpu_engine(prob)
     with pytest.raises(pkg.QsError):
         g.residual2(0.0, np.zeros(g.n_vec + 1))
-    # a runaway input (huge occupations) aborts with F = 0, as scf_residual2
-    x = g.init2(0.0) * 1e6
+    # a non-finite input makes the output occupations non-finite: runaway, F = 0 (scf_residual2)
+    x = g.init2(0.0)
+    x[5] = np.nan
     F, ab = g.residual2(0.0, x)
     assert ab and not F.any()
 
