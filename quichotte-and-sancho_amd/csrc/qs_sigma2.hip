@@ -85,20 +85,34 @@ __device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const 
 }
 
 // ---- x (reference packed layout) -> Gt[omega][c] (blocked) ----------------------
+// One workgroup per energy: the gather of an energy's packed slice (3 nA^2
+// doubles) stays inside one XCD's L2, so each line of x is fetched once.
+constexpr int PU = 8;  // gathers in flight per thread
 __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x, double2 *__restrict__ Gt,
                                                    const int2 *__restrict__ src, int nA, int nw) {
     const int ncol = 2 * nA * nA;
     const size_t nblk = 3 * (size_t)nA * nA;
-    const size_t total = (size_t)nw * ncol;
-    for (size_t e = (size_t)blockIdx.x * NT + threadIdx.x; e < total; e += (size_t)gridDim.x * NT) {
-        const size_t iw = e / ncol;
-        const int c = (int)(e - iw * ncol);
-        const int2 s = src[c];
-        const double *xb = x + iw * nblk;
-        double re = 0.0;
-        if (s.x > 0) re = xb[s.x - 1];
-        else if (s.x < 0) re = -xb[-s.x - 1];
-        Gt[e] = make_double2(re, xb[s.y]);
+    for (int iw = blockIdx.x; iw < nw; iw += gridDim.x) {
+        const double *xb = x + (size_t)iw * nblk;
+        double2 *g = Gt + (size_t)iw * ncol;
+        for (int c0 = threadIdx.x; c0 < ncol; c0 += PU * NT) {
+            double re[PU], im[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                const int c = min(c0 + u * NT, ncol - 1);
+                const int2 sc = src[c];
+                const int ir = sc.x > 0 ? sc.x - 1 : (sc.x < 0 ? -sc.x - 1 : 0);
+                re[u] = xb[ir];
+                im[u] = xb[sc.y];
+                if (sc.x < 0) re[u] = -re[u];
+                if (sc.x == 0) re[u] = 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                const int c = c0 + u * NT;
+                if (c < ncol) g[c] = make_double2(re[u], im[u]);
+            }
+        }
     }
 }
 
@@ -134,6 +148,88 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
         const int col = t & (CB - 1), k1 = t >> 3;
         const double2 v = cmul(buf[col * N1 + k1], p.tw[k1 * n2]);
         Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = v;
+    }
+}
+
+// Persistent forms of the two outer passes: each workgroup walks tiles
+// (column block, n2) with the next tile's loads in registers while the current
+// tile is transformed and stored, so HBM latency overlaps the LDS FFT.
+// Tile t: column block t % nblk (fastest, so neighbouring workgroups read
+// neighbouring 128-byte segments), n2 = t / nblk.
+constexpr int QF1 = 8;   // nonzero F1 inputs per thread and tile: CB N1 / 2 / NT <= 8 (N1 <= 512)
+constexpr int QI2 = 16;  // I2 inputs per thread and tile: CB N1 / NT <= 16
+
+__global__ __launch_bounds__(NT) void k_fft_f1p(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
+                                                Sigma2Plan p, int nblk) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double2 *buf = (double2 *)lds_raw;
+    const int ncol = 2 * p.nA * p.nA, N1 = p.N1, N2 = p.N2, half = CB * N1 / 2;
+    const int ntiles = nblk * N2;
+    double2 nxt[QF1];
+    auto fetch = [&](int tile) {
+        const int cb = tile % nblk, n2 = tile / nblk;
+#pragma unroll
+        for (int q = 0; q < QF1; q++) {
+            const int t = threadIdx.x + q * NT, n = n2 + N2 * (t >> 3);
+            nxt[q] = make_double2(0.0, 0.0);
+            if (t < half && n < p.nw) nxt[q] = Gt[(size_t)n * ncol + cb * CB + (t & (CB - 1))];
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int cb = tile % nblk, n2 = tile / nblk;
+        // rows n1 >= N1 / 2 are the zero padding (t >= half; nxt is zero there)
+#pragma unroll
+        for (int q = 0; q < QI2; q++) {
+            const int t = threadIdx.x + q * NT;
+            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = q < QF1 ? nxt[q] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        lds_fft(buf, CB, N1, p.log2N1, -1, p.tw, p.Nf / N1);
+        for (int t = threadIdx.x; t < CB * N1; t += NT) {
+            const int col = t & (CB - 1), k1 = t >> 3;
+            Z[((size_t)k1 * N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * N1 + k1], p.tw[k1 * n2]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_fft_i2p(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
+                                                Sigma2Plan p, int nblk) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double2 *buf = (double2 *)lds_raw;
+    const int ncol = 2 * p.nA * p.nA, N1 = p.N1, N2 = p.N2;
+    const int ntiles = nblk * N2;
+    double2 nxt[QI2];
+    auto fetch = [&](int tile) {
+        const int cb = tile % nblk, n2 = tile / nblk;
+#pragma unroll
+        for (int q = 0; q < QI2; q++) {
+            const int t = threadIdx.x + q * NT;
+            nxt[q] = make_double2(0.0, 0.0);
+            if (t < CB * N1) nxt[q] = Z[((size_t)(t >> 3) * N2 + n2) * ncol + cb * CB + (t & (CB - 1))];
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int cb = tile % nblk, n2 = tile / nblk;
+#pragma unroll
+        for (int q = 0; q < QI2; q++) {
+            const int t = threadIdx.x + q * NT;
+            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = nxt[q];
+        }
+        __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        lds_fft(buf, CB, N1, p.log2N1, +1, p.tw, p.Nf / N1);
+        for (int t = threadIdx.x; t < CB * N1; t += NT) {
+            const int col = t & (CB - 1), n1 = t >> 3;
+            const int n = n2 + N2 * n1;
+            if (n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * N1 + n1];
+        }
+        __syncthreads();
     }
 }
 
@@ -435,10 +531,8 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 }  // namespace
 
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int nw, hipStream_t s) {
-    const size_t total = (size_t)nw * 2 * nA * nA;
-    size_t blocks = (total + NT - 1) / NT;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_permute_in, dim3((unsigned)blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, nw);
+    const int blocks = nw < 65536 ? nw : 65536;
+    hipLaunchKernelGGL(k_permute_in, dim3(blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, nw);
     return hipGetLastError();
 }
 
@@ -452,8 +546,19 @@ hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, 
         const char *e = getenv("QS_FFT_MID");
         mid_variant = (e && atoi(e) == 1) ? 1 : 2;
     }
+    static int pass_variant = -1, resident = 0;
+    if (pass_variant < 0) {
+        const char *e = getenv("QS_FFT_PASS");
+        pass_variant = (e && atoi(e) == 1) ? 1 : 2;
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+        resident = 2 * ncu;  // 64 KB of LDS per workgroup: two per CU
+    }
     static bool attr = false;
     if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft_f1p, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_i2p, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_mid2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -461,7 +566,11 @@ hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, 
         attr = true;
     }
     hipError_t e = hipSuccess;
-    if (parts & 1) {
+    const int ntiles = nblk * p.N2, pgrid = ntiles < resident ? ntiles : resident;
+    if ((parts & 1) && pass_variant == 2) {
+        hipLaunchKernelGGL(k_fft_f1p, dim3(pgrid), dim3(NT), lds1, s, Gt, Z, p, nblk);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else if (parts & 1) {
         hipLaunchKernelGGL(k_fft_f1, dim3(nblk, p.N2), dim3(NT), lds1, s, Gt, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
@@ -473,7 +582,10 @@ hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, 
             hipLaunchKernelGGL(k_fft_mid2, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm / 2, s, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (parts & 4) {
+    if ((parts & 4) && pass_variant == 2) {
+        hipLaunchKernelGGL(k_fft_i2p, dim3(pgrid), dim3(NT), lds1, s, Z, Sig, p, nblk);
+        e = hipGetLastError();
+    } else if (parts & 4) {
         hipLaunchKernelGGL(k_fft_i2, dim3(nblk, p.N2), dim3(NT), lds1, s, Z, Sig, p);
         e = hipGetLastError();
     }
