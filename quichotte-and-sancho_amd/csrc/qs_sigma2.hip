@@ -117,119 +117,40 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 }
 
 // ---- F1 -----------------------------------------------------------------------
-// grid: (ncol / CB, N2).  LDS: CB * N1 complex.
+// grid: (ncol / CBF, N2).  LDS: CBF * N1 complex.  CBF = 4 (64-byte column
+// segments, 32 KB of LDS at N1 = 512: four workgroups per CU) or 8.
+template <int CBF>
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
+    constexpr int LCB = CBF == 8 ? 3 : 2;
     extern __shared__ __align__(16) char lds_raw[];
     double2 *buf = (double2 *)lds_raw;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
-    const int N1 = p.N1;
-    // loads staged through registers in rounds of 8 per thread
-    for (int base = 0; base < CB * N1; base += 8 * NT) {
+    const int N1 = p.N1, half = CBF * N1 / 2;
+    // rows n1 >= N1 / 2 are the zero padding: one round of loads covers the rest
+    for (int base = 0; base < half; base += 8 * NT) {
         double2 v[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int t = base + threadIdx.x + q * NT;
+            const int n = n2 + p.N2 * (t >> LCB);
             v[q] = make_double2(0.0, 0.0);
-            if (t < CB * N1) {
-                const int n = n2 + p.N2 * (t >> 3);
-                if (n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & (CB - 1))];
-            }
+            if (t < half && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CBF + (t & (CBF - 1))];
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int t = base + threadIdx.x + q * NT;
-            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = v[q];
+            if (t < half) buf[(t & (CBF - 1)) * N1 + bitrev(t >> LCB, p.log2N1)] = v[q];
         }
     }
+    for (int t = half + threadIdx.x; t < CBF * N1; t += NT)
+        buf[(t & (CBF - 1)) * N1 + bitrev(t >> LCB, p.log2N1)] = make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft(buf, CB, N1, p.log2N1, -1, p.tw, p.Nf / N1);
-    for (int t = threadIdx.x; t < CB * N1; t += NT) {
-        const int col = t & (CB - 1), k1 = t >> 3;
+    lds_fft(buf, CBF, N1, p.log2N1, -1, p.tw, p.Nf / N1);
+    for (int t = threadIdx.x; t < CBF * N1; t += NT) {
+        const int col = t & (CBF - 1), k1 = t >> LCB;
         const double2 v = cmul(buf[col * N1 + k1], p.tw[k1 * n2]);
-        Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = v;
-    }
-}
-
-// Persistent forms of the two outer passes: each workgroup walks tiles
-// (column block, n2) with the next tile's loads in registers while the current
-// tile is transformed and stored, so HBM latency overlaps the LDS FFT.
-// Tile t: column block t % nblk (fastest, so neighbouring workgroups read
-// neighbouring 128-byte segments), n2 = t / nblk.
-constexpr int QF1 = 8;   // nonzero F1 inputs per thread and tile: CB N1 / 2 / NT <= 8 (N1 <= 512)
-constexpr int QI2 = 16;  // I2 inputs per thread and tile: CB N1 / NT <= 16
-
-__global__ __launch_bounds__(NT) void k_fft_f1p(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
-                                                Sigma2Plan p, int nblk) {
-    extern __shared__ __align__(16) char lds_raw[];
-    double2 *buf = (double2 *)lds_raw;
-    const int ncol = 2 * p.nA * p.nA, N1 = p.N1, N2 = p.N2, half = CB * N1 / 2;
-    const int ntiles = nblk * N2;
-    double2 nxt[QF1];
-    auto fetch = [&](int tile) {
-        const int cb = tile % nblk, n2 = tile / nblk;
-#pragma unroll
-        for (int q = 0; q < QF1; q++) {
-            const int t = threadIdx.x + q * NT, n = n2 + N2 * (t >> 3);
-            nxt[q] = make_double2(0.0, 0.0);
-            if (t < half && n < p.nw) nxt[q] = Gt[(size_t)n * ncol + cb * CB + (t & (CB - 1))];
-        }
-    };
-    int tile = blockIdx.x;
-    if (tile < ntiles) fetch(tile);
-    for (; tile < ntiles; tile += gridDim.x) {
-        const int cb = tile % nblk, n2 = tile / nblk;
-        // rows n1 >= N1 / 2 are the zero padding (t >= half; nxt is zero there)
-#pragma unroll
-        for (int q = 0; q < QI2; q++) {
-            const int t = threadIdx.x + q * NT;
-            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = q < QF1 ? nxt[q] : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-        lds_fft(buf, CB, N1, p.log2N1, -1, p.tw, p.Nf / N1);
-        for (int t = threadIdx.x; t < CB * N1; t += NT) {
-            const int col = t & (CB - 1), k1 = t >> 3;
-            Z[((size_t)k1 * N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * N1 + k1], p.tw[k1 * n2]);
-        }
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(NT) void k_fft_i2p(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
-                                                Sigma2Plan p, int nblk) {
-    extern __shared__ __align__(16) char lds_raw[];
-    double2 *buf = (double2 *)lds_raw;
-    const int ncol = 2 * p.nA * p.nA, N1 = p.N1, N2 = p.N2;
-    const int ntiles = nblk * N2;
-    double2 nxt[QI2];
-    auto fetch = [&](int tile) {
-        const int cb = tile % nblk, n2 = tile / nblk;
-#pragma unroll
-        for (int q = 0; q < QI2; q++) {
-            const int t = threadIdx.x + q * NT;
-            nxt[q] = make_double2(0.0, 0.0);
-            if (t < CB * N1) nxt[q] = Z[((size_t)(t >> 3) * N2 + n2) * ncol + cb * CB + (t & (CB - 1))];
-        }
-    };
-    int tile = blockIdx.x;
-    if (tile < ntiles) fetch(tile);
-    for (; tile < ntiles; tile += gridDim.x) {
-        const int cb = tile % nblk, n2 = tile / nblk;
-#pragma unroll
-        for (int q = 0; q < QI2; q++) {
-            const int t = threadIdx.x + q * NT;
-            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = nxt[q];
-        }
-        __syncthreads();
-        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-        lds_fft(buf, CB, N1, p.log2N1, +1, p.tw, p.Nf / N1);
-        for (int t = threadIdx.x; t < CB * N1; t += NT) {
-            const int col = t & (CB - 1), n1 = t >> 3;
-            const int n = n2 + N2 * n1;
-            if (n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * N1 + n1];
-        }
-        __syncthreads();
+        Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CBF + col] = v;
     }
 }
 
@@ -284,104 +205,6 @@ __device__ __forceinline__ void pair_sigma(const double2 *r_ij, const double2 *L
     for (int q = 0; q < 4; q++) sL[q] = cscale(t1[q], c);
 }
 
-// grid: (nAs (nAs+1)/2 unordered pairs, N1/2 + 1 row pairs).  LDS: 16 * 2 * N2 complex.
-// Thread work per position: the 16 outputs stay in registers across the barrier
-// that protects the in-place overwrite.
-constexpr int MAXPOS = 2;  // positions per thread: 2 * N2 / NT <= 2  (N2 <= 256)
-
-__global__ __launch_bounds__(NT) void k_fft_mid(double2 *__restrict__ Z, Sigma2Plan p) {
-    extern __shared__ __align__(16) char lds_raw[];
-    double2 *buf = (double2 *)lds_raw;  // [s (16)][row (2)][N2]
-    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA;
-    // unordered pair index -> (ia <= ja)
-    int pid = blockIdx.x, ia = 0;
-    while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
-    const int ja = ia + pid;
-    const int k1 = blockIdx.y, k1p = (N1 - k1) & (N1 - 1);
-    const int nrow = (k1 == k1p) ? 1 : 2;
-    const int cb_ij = (ia * nAs + ja) * CB, cb_ji = (ja * nAs + ia) * CB;
-    const double c = p.Upair[ia * nAs + ja];
-    const int LD = 2 * N2;  // stride between the 16 sequences (rows packed)
-    if (c == 0.0) {
-        for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
-            const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
-            const int row = rr ? k1p : k1;
-            const int cb = s < 8 ? cb_ij : cb_ji;
-            Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)] = make_double2(0.0, 0.0);
-        }
-        return;
-    }
-    for (int base = 0; base < 16 * nrow * N2; base += 8 * NT) {
-        double2 v[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = base + threadIdx.x + q * NT;
-            v[q] = make_double2(0.0, 0.0);
-            if (t < 16 * nrow * N2) {
-                const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
-                const int row = rr ? k1p : k1;
-                const int cb = s < 8 ? cb_ij : cb_ji;
-                v[q] = Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = base + threadIdx.x + q * NT;
-            if (t < 16 * nrow * N2) {
-                const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
-                buf[s * LD + rr * N2 + bitrev(n2, lg2)] = v[q];
-            }
-        }
-    }
-    __syncthreads();
-    // forward DFT over n2: 16 * nrow sequences of length N2 (rows packed with stride N2)
-    lds_fft(buf, 16 * 2, N2, lg2, -1, p.tw, p.Nf / N2);
-    // pointwise at l = (row, k2) and -l
-    double2 outv[MAXPOS][16];
-#pragma unroll
-    for (int q = 0; q < MAXPOS; q++) {
-        const int t = threadIdx.x + q * NT;
-        if (t < nrow * N2) {
-            const int rr = t / N2, k2 = t - rr * N2;
-            const int row = rr ? k1p : k1;
-            int rq, m2;
-            if (row == 0) { rq = 0; m2 = (N2 - k2) & (N2 - 1); }
-            else { rq = (N1 - row) & (N1 - 1); m2 = N2 - 1 - k2; }
-            const int rrq = (rq == k1) ? 0 : 1;
-            double2 P[16], Q[16];
-#pragma unroll
-            for (int s = 0; s < 16; s++) {
-                P[s] = buf[s * LD + rr * N2 + k2];
-                Q[s] = buf[s * LD + rrq * N2 + m2];
-            }
-            // (ij): r_ij, L_ij at +l = P[0..3], P[4..7]; r_ji, L_ji at -l = Q[8..11], Q[12..15]
-            pair_sigma(P + 0, P + 4, Q + 8, Q + 12, c, outv[q] + 0, outv[q] + 4);
-            // (ji): r_ji, L_ji at +l = P[8..11], P[12..15]; r_ij, L_ij at -l = Q[0..3], Q[4..7]
-            pair_sigma(P + 8, P + 12, Q + 0, Q + 4, c, outv[q] + 8, outv[q] + 12);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < MAXPOS; q++) {
-        const int t = threadIdx.x + q * NT;
-        if (t < nrow * N2) {
-            const int rr = t / N2, k2 = t - rr * N2;
-#pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * LD + rr * N2 + bitrev(k2, lg2)] = outv[q][s];
-        }
-    }
-    __syncthreads();
-    lds_fft(buf, 16 * 2, N2, lg2, +1, p.tw, p.Nf / N2);
-    for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
-        const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
-        const int row = rr ? k1p : k1;
-        const int cb = s < 8 ? cb_ij : cb_ji;
-        const double2 w = cconj(p.tw[row * n2]);
-        Z[((size_t)row * N2 + n2) * ncol + cb + (s & 7)] = cmul(buf[s * LD + rr * N2 + n2], w);
-    }
-}
-
-
 // Middle pass with one row of the (k1, -k1) pair held in registers: LDS holds a
 // single row of the 16 columns (16 N2 complex, 64 KB at N2 = 256), so two
 // workgroups share a CU.  Thread t owns position t of row k1 and, for the
@@ -407,25 +230,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         return;
     }
     const int t = threadIdx.x;
-    auto load_row = [&](int row) {
-        for (int base = 0; base < 16 * N2; base += 8 * NT) {
-            double2 v[8];
+    // A row is 16 N2 <= 16 NT values: one round of loads into registers, then
+    // into LDS at bit-reversed positions.  Row k1p is fetched while row k1 is
+    // transformed.
+    constexpr int QM = 16;
+    double2 pre[QM];
+    auto fetch_row = [&](int row) {
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int e = base + threadIdx.x + q * NT;
-                v[q] = make_double2(0.0, 0.0);
-                if (e < 16 * N2) {
-                    const int s = e & 15, n2 = e >> 4;
-                    v[q] = Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int e = base + threadIdx.x + q * NT;
-                if (e < 16 * N2) buf[(e & 15) * N2 + bitrev(e >> 4, lg2)] = v[q];
+        for (int q = 0; q < QM; q++) {
+            const int e = threadIdx.x + q * NT;
+            pre[q] = make_double2(0.0, 0.0);
+            if (e < 16 * N2) {
+                const int s = e & 15, n2 = e >> 4;
+                pre[q] = Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)];
             }
         }
-        __syncthreads();
+    };
+    auto put_row = [&]() {
+#pragma unroll
+        for (int q = 0; q < QM; q++) {
+            const int e = threadIdx.x + q * NT;
+            if (e < 16 * N2) buf[(e & 15) * N2 + bitrev(e >> 4, lg2)] = pre[q];
+        }
     };
     auto store_row = [&](int row) {
         for (int e = threadIdx.x; e < 16 * N2; e += NT) {
@@ -434,7 +260,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * N2 + n2], w);
         }
     };
-    load_row(k1);
+    fetch_row(k1);
+    put_row();
+    __syncthreads();
+    if (k1 != k1p) fetch_row(k1p);
     lds_fft(buf, 16, N2, lg2, -1, p.tw, p.Nf / N2);
     if (k1 != k1p) {
         double2 A[16];
@@ -442,7 +271,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 #pragma unroll
             for (int s = 0; s < 16; s++) A[s] = buf[s * N2 + t];
         __syncthreads();
-        load_row(k1p);
+        put_row();
+        __syncthreads();
         lds_fft(buf, 16, N2, lg2, -1, p.tw, p.Nf / N2);
         const int m2 = N2 - 1 - t;  // partner of (k1, t) is (k1p, N2 - 1 - t)
         double2 Q[16];
@@ -499,32 +329,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 }
 
 // ---- I2 -----------------------------------------------------------------------
+template <int CBF>
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
+    constexpr int LCB = CBF == 8 ? 3 : 2;
     extern __shared__ __align__(16) char lds_raw[];
     double2 *buf = (double2 *)lds_raw;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
     const int N1 = p.N1;
-    for (int base = 0; base < CB * N1; base += 8 * NT) {
-        double2 v[8];
+    for (int base = 0; base < CBF * N1; base += 16 * NT) {
+        double2 v[16];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
+        for (int q = 0; q < 16; q++) {
             const int t = base + threadIdx.x + q * NT;
             v[q] = make_double2(0.0, 0.0);
-            if (t < CB * N1) v[q] = Z[((size_t)(t >> 3) * p.N2 + n2) * ncol + cb * CB + (t & (CB - 1))];
+            if (t < CBF * N1) v[q] = Z[((size_t)(t >> LCB) * p.N2 + n2) * ncol + cb * CBF + (t & (CBF - 1))];
         }
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
+        for (int q = 0; q < 16; q++) {
             const int t = base + threadIdx.x + q * NT;
-            if (t < CB * N1) buf[(t & (CB - 1)) * N1 + bitrev(t >> 3, p.log2N1)] = v[q];
+            if (t < CBF * N1) buf[(t & (CBF - 1)) * N1 + bitrev(t >> LCB, p.log2N1)] = v[q];
         }
     }
     __syncthreads();
-    lds_fft(buf, CB, N1, p.log2N1, +1, p.tw, p.Nf / N1);
-    for (int t = threadIdx.x; t < CB * N1; t += NT) {
-        const int col = t & (CB - 1), n1 = t >> 3;
+    lds_fft(buf, CBF, N1, p.log2N1, +1, p.tw, p.Nf / N1);
+    for (int t = threadIdx.x; t < CBF * N1; t += NT) {
+        const int col = t & (CBF - 1), n1 = t >> LCB;
         const int n = n2 + p.N2 * n1;
-        if (n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * N1 + n1];
+        if (n < p.nw) Sig[(size_t)n * ncol + cb * CBF + col] = buf[col * N1 + n1];
     }
 }
 
@@ -538,55 +370,34 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
 
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
                             hipStream_t s) {
-    const int ncol = 2 * p.nA * p.nA, nblk = ncol / CB;
-    const size_t lds1 = (size_t)CB * p.N1 * sizeof(double2);
-    const size_t ldsm = (size_t)16 * 2 * p.N2 * sizeof(double2);
-    static int mid_variant = -1;
-    if (mid_variant < 0) {
-        const char *e = getenv("QS_FFT_MID");
-        mid_variant = (e && atoi(e) == 1) ? 1 : 2;
-    }
-    static int pass_variant = -1, resident = 0;
-    if (pass_variant < 0) {
-        const char *e = getenv("QS_FFT_PASS");
-        pass_variant = (e && atoi(e) == 1) ? 1 : 2;
-        int dev = 0, ncu = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
-        resident = 2 * ncu;  // 64 KB of LDS per workgroup: two per CU
-    }
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_fft_f1p, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2p, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const int ncol = 2 * p.nA * p.nA;
+    const size_t ldsm = (size_t)16 * p.N2 * sizeof(double2);
+    static int cbf = 0;
+    if (cbf == 0) {
+        const char *e = getenv("QS_FFT_CB");
+        cbf = (e && atoi(e) == 8) ? 8 : 4;
+        (void)hipFuncSetAttribute((const void *)k_fft_f1<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_i2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_f1<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_i2<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_mid2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
     }
+    const size_t lds1 = (size_t)cbf * p.N1 * sizeof(double2);
+    const dim3 g1(ncol / cbf, p.N2);
     hipError_t e = hipSuccess;
-    const int ntiles = nblk * p.N2, pgrid = ntiles < resident ? ntiles : resident;
-    if ((parts & 1) && pass_variant == 2) {
-        hipLaunchKernelGGL(k_fft_f1p, dim3(pgrid), dim3(NT), lds1, s, Gt, Z, p, nblk);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    } else if (parts & 1) {
-        hipLaunchKernelGGL(k_fft_f1, dim3(nblk, p.N2), dim3(NT), lds1, s, Gt, Z, p);
+    if (parts & 1) {
+        if (cbf == 8) hipLaunchKernelGGL(k_fft_f1<8>, g1, dim3(NT), lds1, s, Gt, Z, p);
+        else hipLaunchKernelGGL(k_fft_f1<4>, g1, dim3(NT), lds1, s, Gt, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (parts & 2) {
         const int npairs = p.nAs * (p.nAs + 1) / 2;
-        if (mid_variant == 1)
-            hipLaunchKernelGGL(k_fft_mid, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
-        else
-            hipLaunchKernelGGL(k_fft_mid2, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm / 2, s, Z, p);
+        hipLaunchKernelGGL(k_fft_mid2, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if ((parts & 4) && pass_variant == 2) {
-        hipLaunchKernelGGL(k_fft_i2p, dim3(pgrid), dim3(NT), lds1, s, Z, Sig, p, nblk);
-        e = hipGetLastError();
-    } else if (parts & 4) {
-        hipLaunchKernelGGL(k_fft_i2, dim3(nblk, p.N2), dim3(NT), lds1, s, Z, Sig, p);
+    if (parts & 4) {
+        if (cbf == 8) hipLaunchKernelGGL(k_fft_i2<8>, g1, dim3(NT), lds1, s, Z, Sig, p);
+        else hipLaunchKernelGGL(k_fft_i2<4>, g1, dim3(NT), lds1, s, Z, Sig, p);
         e = hipGetLastError();
     }
     return e;
