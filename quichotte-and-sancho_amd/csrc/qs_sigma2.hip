@@ -117,8 +117,8 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 }
 
 // ---- F1 -----------------------------------------------------------------------
-// grid: (ncol / CBF, N2).  LDS: CBF * N1 complex.  CBF = 4 (64-byte column
-// segments, 32 KB of LDS at N1 = 512: four workgroups per CU) or 8.
+// grid: (ncol / CBF, N2).  LDS: CBF * N1 complex.  CBF = 8 (128-byte column
+// segments; the default) or 4 (64-byte segments, half the LDS).
 template <int CBF>
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
@@ -375,7 +375,7 @@ hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, 
     static int cbf = 0;
     if (cbf == 0) {
         const char *e = getenv("QS_FFT_CB");
-        cbf = (e && atoi(e) == 8) ? 8 : 4;
+        cbf = (e && atoi(e) == 4) ? 4 : 8;
         (void)hipFuncSetAttribute((const void *)k_fft_f1<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_i2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_f1<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
