@@ -137,114 +137,98 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
     for (int a = 0; a < RA; a++)
         if (ti + TI * a >= n) used |= 1u << a;
 
+    double2 *prow = L.chunk, *pcol = L.chunk + 16 * RB;
+
     // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
-    // One barrier per step.  After the update of step k, the owners of column
-    // k+1 find their best unused row; each wavefront reduces its four owner
-    // lanes and posts its candidate (value, row), the candidate's whole row and
-    // column k+1 to LDS (double-buffered on the step parity).  After the
-    // barrier every thread picks the pivot among the NW posted candidates and
-    // already has the pivot row and column.  The step itself applies
-    // M -= M(:,k) (M(p,:) / M(p,k)) with column k zeroed first, so that it
-    // becomes -M(:,k) / M(p,k), and replaces row p by M(p,:) / M(p,k) with
-    // 1 / M(p,k) at column k.  Pivot order: largest |Re| + |Im|, then smallest
-    // row (zgetf2's rule); a NaN never wins over a number.
-    constexpr int NW = NT / 64;
-    const int lane = tid & 63, wv = tid >> 6;
-    double2 *crow = L.big;                        // [2][NW][16 RB] candidate rows
-    double2 *pcolb = L.big + 2 * NW * 16 * RB;    // [2][TI RA] pivot column
-    // step k = -1 only posts the candidates of column 0
-    for (int k = -1; k < n; k++) {
-        if (k >= 0) {
-            const int buf = k & 1, bk = k >> 4, tk = k & 15;
-            double bv = -1.0;
-            int p = -1, pw = 0;
-#pragma unroll
-            for (int w2 = 0; w2 < NW; w2++) {
-                const int i = L.redi[buf * NW + w2];
-                const double v = L.redv[buf * NW + w2];
-                if (i >= 0 && (p < 0 || v > bv || (v == bv && i < p))) { bv = v; p = i; pw = w2; }
-            }
-            if (p < 0) p = 0;  // unreachable: n - k unused rows remain at step k
-            p = __builtin_amdgcn_readfirstlane(p);
-            pw = __builtin_amdgcn_readfirstlane(pw);
-            if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
-            const double2 *prow = crow + (buf * NW + pw) * 16 * RB;
-            const double2 *pcol = pcolb + buf * TI * RA;
-            const int tp = p % TI, ap = p / TI;
-            const double2 pv = pcol[p];
-            if (tid == 0 && pv.x == 0.0 && pv.y == 0.0) atomicOr(A.status, 1);
-            const double2 rinv = cinv(pv);
-            double2 sp[RB], fr[RA];
-#pragma unroll
-            for (int b = 0; b < RB; b++) sp[b] = cmul(prow[tj + 16 * b], rinv);
-#pragma unroll
-            for (int a = 0; a < RA; a++) fr[a] = pcol[ti + TI * a];
-            if (tj == tk) {
-#pragma unroll
-                for (int b = 0; b < RB; b++)
-                    if (b == bk) {
-                        sp[b] = rinv;
-#pragma unroll
-                        for (int a = 0; a < RA; a++) M[a][b] = make_double2(0.0, 0.0);
-                    }
-            }
-#pragma unroll
-            for (int a = 0; a < RA; a++)
-#pragma unroll
-                for (int b = 0; b < RB; b++) cfms(M[a][b], fr[a], sp[b]);
-            if (ti == tp) {
-#pragma unroll
-                for (int a = 0; a < RA; a++)
-                    if (a == ap) {
-                        used |= 1u << a;
-#pragma unroll
-                        for (int b = 0; b < RB; b++) M[a][b] = sp[b];
-                    }
-            }
-        }
-        if (k + 1 < n) {
-            const int kk = k + 1, buf = kk & 1;
-            const int bkk = kk >> 4, tkk = kk & 15;
+    // Step k: the owners of column k post their best unused row, every wave
+    // reduces the TI candidates, the pivot row and column go through LDS, and
+    // each thread applies M -= M(:,k) (M(p,:) / M(p,k)) to its tile with
+    // column k zeroed first, so that it becomes -M(:,k) / M(p,k); row p is then
+    // replaced by M(p,:) / M(p,k) with 1 / M(p,k) at column k.
+    const int lane = tid & 63;
+    for (int k = 0; k < n; k++) {
+        const int bk = k >> 4, tk = k & 15;
+        if (tj == tk) {
             double best = -1.0;
             int bi = -1;
-            if (tj == tkk) {
 #pragma unroll
-                for (int b = 0; b < RB; b++)
-                    if (b == bkk)
+            for (int b = 0; b < RB; b++)
+                if (b == bk)
 #pragma unroll
-                        for (int a = 0; a < RA; a++) {
-                            pcolb[buf * TI * RA + ti + TI * a] = M[a][b];
-                            if (!(used & (1u << a))) {
-                                double v = fabs(M[a][b].x) + fabs(M[a][b].y);
-                                if (v != v) v = -0.5;
-                                if (bi < 0 || v > best) { best = v; bi = ti + TI * a; }
-                            }
+                    for (int a = 0; a < RA; a++)
+                        if (!(used & (1u << a))) {
+                            double v = fabs(M[a][b].x) + fabs(M[a][b].y);
+                            if (v != v) v = -0.5;  // NaN never wins over a number
+                            if (bi < 0 || v > best) { best = v; bi = ti + TI * a; }
                         }
-            }
-            // the wavefront's owner lanes are tkk, tkk + 16, tkk + 32, tkk + 48
-#pragma unroll
-            for (int o = 16; o <= 32; o <<= 1) {
-                const double vo = __shfl_xor(best, o, 64);
-                const int io = __shfl_xor(bi, o, 64);
-                if (io >= 0 && (bi < 0 || vo > best || (vo == best && io < bi))) { best = vo; bi = io; }
-            }
-            const int rw = __builtin_amdgcn_readlane(bi, tkk);
-            if (lane == tkk) {
-                L.redv[buf * NW + wv] = best;
-                L.redi[buf * NW + wv] = bi;
-            }
-            if (rw >= 0 && ti == rw % TI) {
-                const int aw = rw / TI;
-                double2 *dst = crow + (buf * NW + wv) * 16 * RB;
-#pragma unroll
-                for (int a = 0; a < RA; a++)
-                    if (a == aw)
-#pragma unroll
-                        for (int b = 0; b < RB; b++) dst[tj + 16 * b] = M[a][b];
-            }
+            L.redv[ti] = best;
+            L.redi[ti] = bi;
         }
         __syncthreads();
+        // every wavefront reduces the TI candidates (largest value, then smallest row)
+        int p;
+        {
+            const int t = lane & (TI - 1);
+            int ii = L.redi[t];
+            double v = L.redv[t];
+            if (ii < 0) { v = -3.0; ii = 1 << 30; }
+#pragma unroll
+            for (int o = TI / 2; o > 0; o >>= 1) {
+                const double vo = __shfl_xor(v, o, 64);
+                const int io = __shfl_xor(ii, o, 64);
+                if (vo > v || (vo == v && io < ii)) { v = vo; ii = io; }
+            }
+            p = __builtin_amdgcn_readfirstlane(ii);
+        }
+        const int tp = p % TI, ap = p / TI;
+        if (ti == tp) {
+#pragma unroll
+            for (int a = 0; a < RA; a++)
+                if (a == ap)
+#pragma unroll
+                    for (int b = 0; b < RB; b++) prow[tj + 16 * b] = M[a][b];
+        }
+        if (tj == tk) {
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+                if (b == bk)
+#pragma unroll
+                    for (int a = 0; a < RA; a++) pcol[ti + TI * a] = M[a][b];
+        }
+        if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
+        __syncthreads();
+        const double2 pv = pcol[p];
+        if (tid == 0 && pv.x == 0.0 && pv.y == 0.0) atomicOr(A.status, 1);
+        const double2 rinv = cinv(pv);
+        double2 sp[RB], fr[RA];
+#pragma unroll
+        for (int b = 0; b < RB; b++) sp[b] = cmul(prow[tj + 16 * b], rinv);
+#pragma unroll
+        for (int a = 0; a < RA; a++) fr[a] = pcol[ti + TI * a];
+        if (tj == tk) {
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+                if (b == bk) {
+                    sp[b] = rinv;
+#pragma unroll
+                    for (int a = 0; a < RA; a++) M[a][b] = make_double2(0.0, 0.0);
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+#pragma unroll
+            for (int b = 0; b < RB; b++) cfms(M[a][b], fr[a], sp[b]);
+        if (ti == tp) {
+#pragma unroll
+            for (int a = 0; a < RA; a++)
+                if (a == ap) {
+                    used |= 1u << a;
+#pragma unroll
+                    for (int b = 0; b < RB; b++) M[a][b] = sp[b];
+                }
+        }
     }
+    __syncthreads();
 
     // ---- 3. G^r = permuted M staged in LDS; mode extraction -----------------------
 #pragma unroll

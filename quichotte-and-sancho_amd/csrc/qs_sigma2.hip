@@ -24,7 +24,6 @@
 //   I2: inverse DFT over k1 (length N1) for every (c, n2), keep n < n_omega.
 // Each pass reads and writes HBM once in 128-byte segments; the DFTs run in LDS
 // (radix-2, decimation in time, bit-reversed load).
-#include <cstdlib>
 
 #include "qs_device.h"
 #include "qs_internal.h"
@@ -36,16 +35,18 @@ constexpr int CB = 8;  // columns per site-pair block
 
 __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned)v) >> (32 - bits); }
 
-// In-place decimation-in-time DFT on `nseq` sequences of length 2^lg stored with
-// stride `ld` (complex), input in bit-reversed order; sign = -1 forward, +1
-// inverse; tw[k] = e^{-2 pi i k / Nf}, twstep = Nf / 2^lg.
+// In-place decimation-in-time DFT on `nseq` sequences of length L = 2^lg stored
+// with stride `ld` (complex), input in bit-reversed order; sign = -1 forward,
+// +1 inverse; twl[m] = e^{-2 pi i m / L} (m < L / 2), a table in LDS.
 // The lg radix-2 levels are done in groups of up to three: one task loads the
 // 8 elements j + t h (t < 8) of a block, runs the group's levels in registers
 // and stores them back, so a length-512 transform takes 3 LDS sweeps and 3
 // barriers.  The arithmetic is exactly that of the radix-2 levels.
+// Sequences are stored with stride L + 1: the column-interleaved staging
+// accesses (8 or 16 sequences at one position) then fall in distinct banks.
 template <int G>
 __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg, int s0, int sign,
-                                         const double2 *tw, int twstep) {
+                                         const double2 *twl) {
     constexpr int T = 1 << G;
     const int L = 1 << lg, h = 1 << (s0 - 1), span = h * T;
     const int tasks_per_seq = L / T, total = nseq * tasks_per_seq;
@@ -58,13 +59,13 @@ __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg
         for (int t = 0; t < T; t++) v[t] = b[t * h];
 #pragma unroll
         for (int e = 0; e < G; e++) {
-            const int half = h << e;                        // radix-2 level s0 + e
-            const int twmul = twstep * (L / (2 * half));    // e^{-2 pi i m/(2 half)} = tw[m * twmul]
+            const int half = h << e;                 // radix-2 level s0 + e
+            const int twmul = L / (2 * half);        // e^{-2 pi i m/(2 half)} = twl[m * twmul]
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (!(t & (1 << e))) {
                     const int m = j + (t & ((1 << e) - 1)) * h;   // position inside the level's group
-                    double2 w = tw[m * twmul];
+                    double2 w = twl[m * twmul];
                     if (sign > 0) w.y = -w.y;
                     const double2 u = v[t], x = cmul(w, v[t + (1 << e)]);
                     v[t] = cadd(u, x);
@@ -77,11 +78,16 @@ __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg
     __syncthreads();
 }
 
-__device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const double2 *tw, int twstep) {
+__device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const double2 *twl) {
     int s = 1;
-    while (lg - s + 1 >= 3) { fft_group<3>(buf, nseq, ld, lg, s, sign, tw, twstep); s += 3; }
-    if (lg - s + 1 == 2) fft_group<2>(buf, nseq, ld, lg, s, sign, tw, twstep);
-    else if (lg - s + 1 == 1) fft_group<1>(buf, nseq, ld, lg, s, sign, tw, twstep);
+    while (lg - s + 1 >= 3) { fft_group<3>(buf, nseq, ld, lg, s, sign, twl); s += 3; }
+    if (lg - s + 1 == 2) fft_group<2>(buf, nseq, ld, lg, s, sign, twl);
+    else if (lg - s + 1 == 1) fft_group<1>(buf, nseq, ld, lg, s, sign, twl);
+}
+
+// twl[m] = e^{-2 pi i m / L} for m < L / 2, from the length-Nf table
+__device__ __forceinline__ void load_twl(double2 *twl, const double2 *tw, int L, int Nf) {
+    for (int m = threadIdx.x; m < L / 2; m += NT) twl[m] = tw[m * (Nf / L)];
 }
 
 // ---- x (reference packed layout) -> Gt[omega][c] (blocked) ----------------------
@@ -117,40 +123,47 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 }
 
 // ---- F1 -----------------------------------------------------------------------
-// grid: (ncol / CBF, N2).  LDS: CBF * N1 complex.  CBF = 8 (128-byte column
-// segments; the default) or 4 (64-byte segments, half the LDS).
-template <int CBF>
+// grid: (ncol / CB, N2).  LDS: twiddles (N1 / 2) | CB sequences of stride N1 + 1.
+// Thread work is unrolled over constant bounds (CB N1 <= 16 NT) so that every
+// global load of a round is in flight at once.
+constexpr int QS = 16;  // staging rounds: CB N1 / NT <= 16 (N1 <= 512)
+
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
-    constexpr int LCB = CBF == 8 ? 3 : 2;
     extern __shared__ __align__(16) char lds_raw[];
-    double2 *buf = (double2 *)lds_raw;
-    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
-    const int N1 = p.N1, half = CBF * N1 / 2;
-    // rows n1 >= N1 / 2 are the zero padding: one round of loads covers the rest
-    for (int base = 0; base < half; base += 8 * NT) {
-        double2 v[8];
+    const int N1 = p.N1, ld = N1 + 1, lg1 = p.log2N1;
+    double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
+    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y, half = CB * N1 / 2;
+    load_twl(twl, p.tw, N1, p.Nf);
+    // rows n1 >= N1 / 2 are the zero padding: the inputs are one round of 8
+    double2 v[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = base + threadIdx.x + q * NT;
-            const int n = n2 + p.N2 * (t >> LCB);
-            v[q] = make_double2(0.0, 0.0);
-            if (t < half && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CBF + (t & (CBF - 1))];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = base + threadIdx.x + q * NT;
-            if (t < half) buf[(t & (CBF - 1)) * N1 + bitrev(t >> LCB, p.log2N1)] = v[q];
-        }
+    for (int q = 0; q < 8; q++) {
+        const int t = threadIdx.x + q * NT, n = n2 + p.N2 * (t >> 3);
+        v[q] = make_double2(0.0, 0.0);
+        if (t < half && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & 7)];
     }
-    for (int t = half + threadIdx.x; t < CBF * N1; t += NT)
-        buf[(t & (CBF - 1)) * N1 + bitrev(t >> LCB, p.log2N1)] = make_double2(0.0, 0.0);
+    // output twiddles e^{-2 pi i k1 n2 / Nf}, k1 = (tid >> 3) + (NT / 8) i
+    double2 w[QS];
+#pragma unroll
+    for (int i = 0; i < QS; i++) {
+        const int k1 = (threadIdx.x >> 3) + (NT / 8) * i;
+        w[i] = k1 < N1 ? p.tw[k1 * n2] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < QS; q++) {
+        const int t = threadIdx.x + q * NT;
+        if (t < CB * N1) buf[(t & 7) * ld + bitrev(t >> 3, lg1)] = q < 8 ? v[q] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
-    lds_fft(buf, CBF, N1, p.log2N1, -1, p.tw, p.Nf / N1);
-    for (int t = threadIdx.x; t < CBF * N1; t += NT) {
-        const int col = t & (CBF - 1), k1 = t >> LCB;
-        const double2 v = cmul(buf[col * N1 + k1], p.tw[k1 * n2]);
-        Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CBF + col] = v;
+    lds_fft(buf, CB, ld, lg1, -1, twl);
+#pragma unroll
+    for (int i = 0; i < QS; i++) {
+        const int t = threadIdx.x + i * NT;
+        if (t < CB * N1) {
+            const int col = t & 7, k1 = t >> 3;
+            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * ld + k1], w[i]);
+        }
     }
 }
 
@@ -205,15 +218,17 @@ __device__ __forceinline__ void pair_sigma(const double2 *r_ij, const double2 *L
     for (int q = 0; q < 4; q++) sL[q] = cscale(t1[q], c);
 }
 
-// Middle pass with one row of the (k1, -k1) pair held in registers: LDS holds a
-// single row of the 16 columns (16 N2 complex, 64 KB at N2 = 256), so two
-// workgroups share a CU.  Thread t owns position t of row k1 and, for the
-// spin algebra, the partner position (-k1, N2-1-t); both outputs are computed
-// by the same thread from the same two inputs.
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid2(double2 *__restrict__ Z, Sigma2Plan p) {
+// Middle pass with one row of the (k1, -k1) pair held in registers: LDS holds
+// a single row of the 16 columns (16 sequences of stride N2 + 1, 66 KB at
+// N2 = 256) and the row-DFT twiddles, so two workgroups share a CU.  Thread t
+// owns position t of row k1 and, for the spin algebra, the partner position
+// (-k1, N2-1-t); both outputs are computed by the same thread from the same
+// two inputs.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid(double2 *__restrict__ Z,
+                                                                                          Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    double2 *buf = (double2 *)lds_raw;  // [s (16)][N2]
-    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA;
+    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA, ld = N2 + 1;
+    double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (16)][ld]
     int pid = blockIdx.x, ia = 0;
     while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
     const int ja = ia + pid;
@@ -230,6 +245,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         return;
     }
     const int t = threadIdx.x;
+    load_twl(twl, p.tw, N2, p.Nf);
     // A row is 16 N2 <= 16 NT values: one round of loads into registers, then
     // into LDS at bit-reversed positions.  Row k1p is fetched while row k1 is
     // transformed.
@@ -250,35 +266,46 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 #pragma unroll
         for (int q = 0; q < QM; q++) {
             const int e = threadIdx.x + q * NT;
-            if (e < 16 * N2) buf[(e & 15) * N2 + bitrev(e >> 4, lg2)] = pre[q];
+            if (e < 16 * N2) buf[(e & 15) * ld + bitrev(e >> 4, lg2)] = pre[q];
         }
     };
+    // twiddle e^{+2 pi i row n2 / Nf} between the inverse row DFT and the
+    // inverse pass over k1; position n2 = (tid >> 4) + (NT / 16) q
     auto store_row = [&](int row) {
-        for (int e = threadIdx.x; e < 16 * N2; e += NT) {
-            const int s = e & 15, n2 = e >> 4;
-            const double2 w = cconj(p.tw[row * n2]);
-            Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * N2 + n2], w);
+        double2 w[QM];
+#pragma unroll
+        for (int q = 0; q < QM; q++) {
+            const int n2 = (threadIdx.x >> 4) + (NT / 16) * q;
+            w[q] = n2 < N2 ? cconj(p.tw[row * n2]) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < QM; q++) {
+            const int e = threadIdx.x + q * NT;
+            if (e < 16 * N2) {
+                const int s = e & 15, n2 = e >> 4;
+                Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * ld + n2], w[q]);
+            }
         }
     };
     fetch_row(k1);
     put_row();
     __syncthreads();
     if (k1 != k1p) fetch_row(k1p);
-    lds_fft(buf, 16, N2, lg2, -1, p.tw, p.Nf / N2);
+    lds_fft(buf, 16, ld, lg2, -1, twl);
     if (k1 != k1p) {
         double2 A[16];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) A[s] = buf[s * N2 + t];
+            for (int s = 0; s < 16; s++) A[s] = buf[s * ld + t];
         __syncthreads();
         put_row();
         __syncthreads();
-        lds_fft(buf, 16, N2, lg2, -1, p.tw, p.Nf / N2);
+        lds_fft(buf, 16, ld, lg2, -1, twl);
         const int m2 = N2 - 1 - t;  // partner of (k1, t) is (k1p, N2 - 1 - t)
         double2 Q[16];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) Q[s] = buf[s * N2 + m2];
+            for (int s = 0; s < 16; s++) Q[s] = buf[s * ld + m2];
         __syncthreads();
         if (t < N2) {
             // Two independent groups keep the live set at A, Q and 8 outputs:
@@ -288,24 +315,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             double2 O[8];
             pair_sigma(Q + 8, Q + 12, A + 0, A + 4, c, O + 0, O + 4);
 #pragma unroll
-            for (int s = 0; s < 8; s++) buf[(8 + s) * N2 + bm] = O[s];
+            for (int s = 0; s < 8; s++) buf[(8 + s) * ld + bm] = O[s];
             pair_sigma(A + 0, A + 4, Q + 8, Q + 12, c, O + 0, O + 4);
 #pragma unroll
             for (int s = 0; s < 8; s++) A[s] = O[s];
             pair_sigma(Q + 0, Q + 4, A + 8, A + 12, c, O + 0, O + 4);
 #pragma unroll
-            for (int s = 0; s < 8; s++) buf[s * N2 + bm] = O[s];
+            for (int s = 0; s < 8; s++) buf[s * ld + bm] = O[s];
             pair_sigma(A + 8, A + 12, Q + 0, Q + 4, c, O + 0, O + 4);
 #pragma unroll
             for (int s = 0; s < 8; s++) A[8 + s] = O[s];
         }
         __syncthreads();
-        lds_fft(buf, 16, N2, lg2, +1, p.tw, p.Nf / N2);
+        lds_fft(buf, 16, ld, lg2, +1, twl);
         store_row(k1p);
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * N2 + bitrev(t, lg2)] = A[s];
+            for (int s = 0; s < 16; s++) buf[s * ld + bitrev(t, lg2)] = A[s];
         __syncthreads();
     } else {
         // k1 = 0 or N1/2: the partner position is in the same row
@@ -314,49 +341,48 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         if (t < N2) {
             double2 P[16], Q[16];
 #pragma unroll
-            for (int s = 0; s < 16; s++) { P[s] = buf[s * N2 + t]; Q[s] = buf[s * N2 + m2]; }
+            for (int s = 0; s < 16; s++) { P[s] = buf[s * ld + t]; Q[s] = buf[s * ld + m2]; }
             pair_sigma(P + 0, P + 4, Q + 8, Q + 12, c, O + 0, O + 4);
             pair_sigma(P + 8, P + 12, Q + 0, Q + 4, c, O + 8, O + 12);
         }
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * N2 + bitrev(t, lg2)] = O[s];
+            for (int s = 0; s < 16; s++) buf[s * ld + bitrev(t, lg2)] = O[s];
         __syncthreads();
     }
-    lds_fft(buf, 16, N2, lg2, +1, p.tw, p.Nf / N2);
+    lds_fft(buf, 16, ld, lg2, +1, twl);
     store_row(k1);
 }
 
 // ---- I2 -----------------------------------------------------------------------
-template <int CBF>
+// grid: (ncol / CB, N2).  LDS as in F1.  Only the outputs n < nw are kept.
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
-    constexpr int LCB = CBF == 8 ? 3 : 2;
     extern __shared__ __align__(16) char lds_raw[];
-    double2 *buf = (double2 *)lds_raw;
+    const int N1 = p.N1, ld = N1 + 1, lg1 = p.log2N1;
+    double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
-    const int N1 = p.N1;
-    for (int base = 0; base < CBF * N1; base += 16 * NT) {
-        double2 v[16];
+    load_twl(twl, p.tw, N1, p.Nf);
+    double2 v[QS];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int t = base + threadIdx.x + q * NT;
-            v[q] = make_double2(0.0, 0.0);
-            if (t < CBF * N1) v[q] = Z[((size_t)(t >> LCB) * p.N2 + n2) * ncol + cb * CBF + (t & (CBF - 1))];
-        }
+    for (int q = 0; q < QS; q++) {
+        const int t = threadIdx.x + q * NT;
+        v[q] = make_double2(0.0, 0.0);
+        if (t < CB * N1) v[q] = Z[((size_t)(t >> 3) * p.N2 + n2) * ncol + cb * CB + (t & 7)];
+    }
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int t = base + threadIdx.x + q * NT;
-            if (t < CBF * N1) buf[(t & (CBF - 1)) * N1 + bitrev(t >> LCB, p.log2N1)] = v[q];
-        }
+    for (int q = 0; q < QS; q++) {
+        const int t = threadIdx.x + q * NT;
+        if (t < CB * N1) buf[(t & 7) * ld + bitrev(t >> 3, lg1)] = v[q];
     }
     __syncthreads();
-    lds_fft(buf, CBF, N1, p.log2N1, +1, p.tw, p.Nf / N1);
-    for (int t = threadIdx.x; t < CBF * N1; t += NT) {
-        const int col = t & (CBF - 1), n1 = t >> LCB;
-        const int n = n2 + p.N2 * n1;
-        if (n < p.nw) Sig[(size_t)n * ncol + cb * CBF + col] = buf[col * N1 + n1];
+    lds_fft(buf, CB, ld, lg1, +1, twl);
+#pragma unroll
+    for (int i = 0; i < QS; i++) {
+        const int t = threadIdx.x + i * NT;
+        const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
+        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * ld + n1];
     }
 }
 
@@ -371,33 +397,28 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
                             hipStream_t s) {
     const int ncol = 2 * p.nA * p.nA;
-    const size_t ldsm = (size_t)16 * p.N2 * sizeof(double2);
-    static int cbf = 0;
-    if (cbf == 0) {
-        const char *e = getenv("QS_FFT_CB");
-        cbf = (e && atoi(e) == 4) ? 4 : 8;
-        (void)hipFuncSetAttribute((const void *)k_fft_f1<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_f1<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_mid2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)CB * (p.N1 + 1)) * sizeof(double2);
+    const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)16 * (p.N2 + 1)) * sizeof(double2);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
     }
-    const size_t lds1 = (size_t)cbf * p.N1 * sizeof(double2);
-    const dim3 g1(ncol / cbf, p.N2);
+    const dim3 g1(ncol / CB, p.N2);
     hipError_t e = hipSuccess;
     if (parts & 1) {
-        if (cbf == 8) hipLaunchKernelGGL(k_fft_f1<8>, g1, dim3(NT), lds1, s, Gt, Z, p);
-        else hipLaunchKernelGGL(k_fft_f1<4>, g1, dim3(NT), lds1, s, Gt, Z, p);
+        hipLaunchKernelGGL(k_fft_f1, g1, dim3(NT), lds1, s, Gt, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (parts & 2) {
         const int npairs = p.nAs * (p.nAs + 1) / 2;
-        hipLaunchKernelGGL(k_fft_mid2, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
+        hipLaunchKernelGGL(k_fft_mid, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (parts & 4) {
-        if (cbf == 8) hipLaunchKernelGGL(k_fft_i2<8>, g1, dim3(NT), lds1, s, Z, Sig, p);
-        else hipLaunchKernelGGL(k_fft_i2<4>, g1, dim3(NT), lds1, s, Z, Sig, p);
+        hipLaunchKernelGGL(k_fft_i2, g1, dim3(NT), lds1, s, Z, Sig, p);
         e = hipGetLastError();
     }
     return e;
