@@ -42,8 +42,13 @@ __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned
 // 8 elements j + t h (t < 8) of a block, runs the group's levels in registers
 // and stores them back, so a length-512 transform takes 3 LDS sweeps and 3
 // barriers.  The arithmetic is exactly that of the radix-2 levels.
-// Sequences are stored with stride L + 1: the column-interleaved staging
-// accesses (8 or 16 sequences at one position) then fall in distinct banks.
+// Layout: element x of sequence q sits at q * ld + px(x), px(x) = x + x / 8,
+// ld = px(L) + 1.  The pad after every 8 elements puts the 8 lanes of the
+// first group (stride-8 blocks) in distinct banks, and the odd sequence stride
+// does the same for the column-interleaved staging accesses.
+__device__ __forceinline__ int px(int x) { return x + (x >> 3); }
+__device__ __forceinline__ int seq_stride(int L) { return L + (L >> 3) + 1; }
+
 template <int G>
 __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg, int s0, int sign,
                                          const double2 *twl) {
@@ -53,10 +58,11 @@ __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg
     for (int task = threadIdx.x; task < total; task += blockDim.x) {
         const int seq = task / tasks_per_seq, r = task - seq * tasks_per_seq;
         const int blk = r / h, j = r - blk * h;
-        double2 *b = buf + (size_t)seq * ld + blk * span + j;
+        double2 *b = buf + (size_t)seq * ld;
+        const int x0 = blk * span + j;
         double2 v[T];
 #pragma unroll
-        for (int t = 0; t < T; t++) v[t] = b[t * h];
+        for (int t = 0; t < T; t++) v[t] = b[px(x0 + t * h)];
 #pragma unroll
         for (int e = 0; e < G; e++) {
             const int half = h << e;                 // radix-2 level s0 + e
@@ -73,7 +79,7 @@ __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg
                 }
         }
 #pragma unroll
-        for (int t = 0; t < T; t++) b[t * h] = v[t];
+        for (int t = 0; t < T; t++) b[px(x0 + t * h)] = v[t];
     }
     __syncthreads();
 }
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 }
 
 // ---- F1 -----------------------------------------------------------------------
-// grid: (ncol / CB, N2).  LDS: twiddles (N1 / 2) | CB sequences of stride N1 + 1.
+// grid: (ncol / CB, N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
 // Thread work is unrolled over constant bounds (CB N1 <= 16 NT) so that every
 // global load of a round is in flight at once.
 constexpr int QS = 16;  // staging rounds: CB N1 / NT <= 16 (N1 <= 512)
@@ -131,7 +137,7 @@ constexpr int QS = 16;  // staging rounds: CB N1 / NT <= 16 (N1 <= 512)
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, ld = N1 + 1, lg1 = p.log2N1;
+    const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y, half = CB * N1 / 2;
     load_twl(twl, p.tw, N1, p.Nf);
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
 #pragma unroll
     for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
-        if (t < CB * N1) buf[(t & 7) * ld + bitrev(t >> 3, lg1)] = q < 8 ? v[q] : make_double2(0.0, 0.0);
+        if (t < CB * N1) buf[(t & 7) * ld + px(bitrev(t >> 3, lg1))] = q < 8 ? v[q] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     lds_fft(buf, CB, ld, lg1, -1, twl);
@@ -162,7 +168,7 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
         const int t = threadIdx.x + i * NT;
         if (t < CB * N1) {
             const int col = t & 7, k1 = t >> 3;
-            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * ld + k1], w[i]);
+            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * ld + px(k1)], w[i]);
         }
     }
 }
@@ -219,15 +225,14 @@ __device__ __forceinline__ void pair_sigma(const double2 *r_ij, const double2 *L
 }
 
 // Middle pass with one row of the (k1, -k1) pair held in registers: LDS holds
-// a single row of the 16 columns (16 sequences of stride N2 + 1, 66 KB at
-// N2 = 256) and the row-DFT twiddles, so two workgroups share a CU.  Thread t
+// a single row of the 16 columns (16 padded sequences, 74 KB at N2 = 256) and the row-DFT twiddles, so two workgroups share a CU.  Thread t
 // owns position t of row k1 and, for the spin algebra, the partner position
 // (-k1, N2-1-t); both outputs are computed by the same thread from the same
 // two inputs.
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid(double2 *__restrict__ Z,
                                                                                           Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA, ld = N2 + 1;
+    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA, ld = seq_stride(N2);
     double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (16)][ld]
     int pid = blockIdx.x, ia = 0;
     while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 #pragma unroll
         for (int q = 0; q < QM; q++) {
             const int e = threadIdx.x + q * NT;
-            if (e < 16 * N2) buf[(e & 15) * ld + bitrev(e >> 4, lg2)] = pre[q];
+            if (e < 16 * N2) buf[(e & 15) * ld + px(bitrev(e >> 4, lg2))] = pre[q];
         }
     };
     // twiddle e^{+2 pi i row n2 / Nf} between the inverse row DFT and the
@@ -283,7 +288,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             const int e = threadIdx.x + q * NT;
             if (e < 16 * N2) {
                 const int s = e & 15, n2 = e >> 4;
-                Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * ld + n2], w[q]);
+                Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * ld + px(n2)], w[q]);
             }
         }
     };
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         double2 A[16];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) A[s] = buf[s * ld + t];
+            for (int s = 0; s < 16; s++) A[s] = buf[s * ld + px(t)];
         __syncthreads();
         put_row();
         __syncthreads();
@@ -305,13 +310,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         double2 Q[16];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) Q[s] = buf[s * ld + m2];
+            for (int s = 0; s < 16; s++) Q[s] = buf[s * ld + px(m2)];
         __syncthreads();
         if (t < N2) {
             // Two independent groups keep the live set at A, Q and 8 outputs:
             // {A0-7, Q8-15} -> (k1, t) blocks 0-7 and (k1p, m2) blocks 8-15,
             // {A8-15, Q0-7} -> (k1, t) blocks 8-15 and (k1p, m2) blocks 0-7.
-            const int bm = bitrev(m2, lg2);
+            const int bm = px(bitrev(m2, lg2));
             double2 O[8];
             pair_sigma(Q + 8, Q + 12, A + 0, A + 4, c, O + 0, O + 4);
 #pragma unroll
@@ -332,7 +337,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * ld + bitrev(t, lg2)] = A[s];
+            for (int s = 0; s < 16; s++) buf[s * ld + px(bitrev(t, lg2))] = A[s];
         __syncthreads();
     } else {
         // k1 = 0 or N1/2: the partner position is in the same row
@@ -341,14 +346,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         if (t < N2) {
             double2 P[16], Q[16];
 #pragma unroll
-            for (int s = 0; s < 16; s++) { P[s] = buf[s * ld + t]; Q[s] = buf[s * ld + m2]; }
+            for (int s = 0; s < 16; s++) { P[s] = buf[s * ld + px(t)]; Q[s] = buf[s * ld + px(m2)]; }
             pair_sigma(P + 0, P + 4, Q + 8, Q + 12, c, O + 0, O + 4);
             pair_sigma(P + 8, P + 12, Q + 0, Q + 4, c, O + 8, O + 12);
         }
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * ld + bitrev(t, lg2)] = O[s];
+            for (int s = 0; s < 16; s++) buf[s * ld + px(bitrev(t, lg2))] = O[s];
         __syncthreads();
     }
     lds_fft(buf, 16, ld, lg2, +1, twl);
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, ld = N1 + 1, lg1 = p.log2N1;
+    const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
     load_twl(twl, p.tw, N1, p.Nf);
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 #pragma unroll
     for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
-        if (t < CB * N1) buf[(t & 7) * ld + bitrev(t >> 3, lg1)] = v[q];
+        if (t < CB * N1) buf[(t & 7) * ld + px(bitrev(t >> 3, lg1))] = v[q];
     }
     __syncthreads();
     lds_fft(buf, CB, ld, lg1, +1, twl);
@@ -382,7 +387,7 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
     for (int i = 0; i < QS; i++) {
         const int t = threadIdx.x + i * NT;
         const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
-        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * ld + n1];
+        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * ld + px(n1)];
     }
 }
 
@@ -397,8 +402,8 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
                             hipStream_t s) {
     const int ncol = 2 * p.nA * p.nA;
-    const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)CB * (p.N1 + 1)) * sizeof(double2);
-    const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)16 * (p.N2 + 1)) * sizeof(double2);
+    const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)CB * (p.N1 + p.N1 / 8 + 1)) * sizeof(double2);
+    const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)16 * (p.N2 + p.N2 / 8 + 1)) * sizeof(double2);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
