@@ -124,8 +124,9 @@ int qs_profile(qs_handle *h, int32_t enable);
 int qs_profile_read(qs_handle *h, double *ms, int64_t *count, int32_t nstage);
 const char *qs_stage_name(int32_t stage);
 /* launch shape of the Dyson kernel picked for this size at qs_create (0: 256
- * threads, 1: 512 threads per energy) and the timed ms of each on a slice of
- * the grid; returns the number of shapes */
+ * threads, 1: 512 threads per energy, 2: one 64-lane wavefront per energy) and
+ * the timed ms of each on a slice of the grid (0 for a shape that does not
+ * apply to this size); returns the number of shapes */
 int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax);
 
 #ifdef __cplusplus

@@ -15,7 +15,7 @@ SO = os.path.join(LIB, "libqs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_sigma2.hip", "qs_vector.hip"]
+SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_dyson_wave.hip", "qs_sigma2.hip", "qs_vector.hip"]
 
 
 def build(verbose=False):

@@ -529,14 +529,22 @@ size_t qs_dyson_lds_bytes(int n, int nCL) {
 
 int qs_dyson_max_norb() { return 88; }
 
-// variant 0: 256 threads (16 x 16), variant 1: 512 threads (32 x 16)
-int qs_dyson_variants(int norb) { return norb <= 16 ? 1 : 2; }
+bool qs_dyson_shape_ok(int norb, int shape) {
+    if (shape == QS_DYSON_WG256) return true;
+    if (shape == QS_DYSON_WG512) return norb > 16;
+    if (shape == QS_DYSON_WAVE) return norb <= qs_dyson_wave_max_norb();
+    return false;
+}
+
+int qs_dyson_variants(int norb) { (void)norb; return QS_DYSON_NSHAPES; }
 
 hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, hipStream_t s) {
     if (count <= 0) return hipSuccess;
+    if (!qs_dyson_shape_ok(a.norb, variant)) return hipErrorInvalidValue;
+    if (variant == QS_DYSON_WAVE) return qs_launch_dyson_wave(a, count, s);
     if (qs_dyson_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
     const int RB = (a.norb + 15) / 16;
-    if (variant == 0 || a.norb <= 16) {
+    if (variant == QS_DYSON_WG256) {
         switch (RB) {
             case 1: return launch_v<16, 1, 1>(a, count, s);
             case 2: return launch_v<16, 2, 2>(a, count, s);

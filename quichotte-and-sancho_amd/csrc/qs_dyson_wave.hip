@@ -1,0 +1,588 @@
+// qs_dyson_wave.hip -- per-energy Dyson and Keldysh solve with a small LDS footprint.
+//
+// Same contract as k_dyson (qs_dyson.hip; src/scf.f90:305-415,
+// common/linear_algebra.f90:58-88):
+//
+//   G^r = [ w + Hemb - Sigma1 - Sigma2^r(w) ]^-1
+//   G^< = G^r(:,sup) [ i(f_L GamL + f_R GamR) + Sigma2^<(w) ] G^r(:,sup)^dagger
+//
+// A workgroup of NT = TI TJ threads (8 x 8: one wavefront, 16 x 16: four)
+// solves one energy.  Thread (li, lj) = (tid / TJ, tid % TJ) keeps the RA x RB
+// elements (li + TI a, lj + TJ b) of the matrix in registers (norb <= TI RA,
+// norb <= TJ RB).  The workgroup keeps only TI-row stages in LDS (37 KB at
+// norb = 80 against 125 KB for k_dyson), so two energies share a CU and each
+// hides the other's barrier and memory latency; the full matrices of the
+// Keldysh products go through a per-energy scratch slab in HBM.
+//
+// Steps:
+//  1. A = w + Hemb - Sigma1 - Sigma2^r in registers.
+//  2. Gauss-Jordan with zgetf2's pivot rule, rows never moved (as k_dyson):
+//     the registers end with M, G^r[q_r][p_c] = M[r][c].  One barrier per
+//     pivot: after the update of step k each wavefront posts its best row for
+//     column k+1 together with that whole row and column k+1 (double-buffered
+//     LDS); after the barrier every thread picks the pivot and has its row and
+//     column.
+//  3. M goes to the slab twice: column-major (Ms, operand of the Keldysh
+//     products) and as G^r in natural order (Gs, for packing).
+//  4. T = M S'' with S''[c][c'] = SL[p_c][p_c'] (outer products over c: the
+//     column of M from Ms, the row of S'' built in LDS), then T to the slab (Ts).
+//  5. P = T M^dagger (outer products over c, both columns from the slab);
+//     G^<[q_r][q_r'] = P[r][r'].
+//  6. Epilogues by mode from TI-row LDS stages of G^< and from Gs rows, with
+//     the loads of x and Fold batched ahead of the stores.
+#include "qs_device.h"
+#include "qs_internal.h"
+
+namespace {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+constexpr int EB = 8;   // G^r packing elements per thread in flight
+constexpr int EBL = 4;  // G^< packing elements per thread in flight (P is live then)
+
+// blocked Sigma2 layout: retarded value of the active pair (a, b); +4 is the lesser value
+__device__ __forceinline__ int sig_index(int a, int b, int nAs) {
+    return ((((a >> 1) * nAs + (b >> 1)) * 2) << 2) + ((a & 1) << 1) + (b & 1);
+}
+
+// packed G^< positions (common/linear_algebra.f90:142-191): Re part over a < b
+// row-wise, Im part over a <= b row-wise, after the 2 nA^2 G^r values
+__device__ __forceinline__ int su_index(int a, int b, int nA) { return a * (nA - 1) - (a * (a - 1)) / 2 + (b - a - 1); }
+__device__ __forceinline__ int u_index(int a, int b, int nA) { return a * nA - (a * (a - 1)) / 2 + (b - a); }
+
+struct WLayout {
+    double2 *stage, *crow, *pcol, *srow, *grcl, *glcl;
+    double *cv, *red;
+    int *ci, *piv, *qinv, *spc, *actl, *posl;
+};
+
+// LDS: stage [TI][n] | crow [2][NW][TJ RB] | pcol [2][TI RA] | srow [2][TJ RB] |
+//      grcl, glcl [nCL^2] | cv [2 NW] | red [3 NW] | ci [2 NW] | 5 int maps [TI RA]
+__host__ __device__ inline size_t wlayout_bytes(int n, int nCL, int RA, int RB, int TI, int TJ) {
+    const int NW = TI * TJ / 64;
+    return (size_t)16 * (TI * n + 2 * NW * TJ * RB + 2 * TI * RA + 2 * TJ * RB + 2 * nCL * nCL) +
+           (size_t)8 * (2 * NW + 3 * NW) + (size_t)4 * (2 * NW + 5 * TI * RA);
+}
+__device__ __forceinline__ WLayout wlayout(char *base, int n, int nCL, int RA, int RB, int TI, int TJ) {
+    const int NW = TI * TJ / 64;
+    WLayout L;
+    L.stage = (double2 *)base;
+    L.crow = L.stage + TI * n;
+    L.pcol = L.crow + 2 * NW * TJ * RB;
+    L.srow = L.pcol + 2 * TI * RA;
+    L.grcl = L.srow + 2 * TJ * RB;
+    L.glcl = L.grcl + nCL * nCL;
+    L.cv = (double *)(L.glcl + nCL * nCL);
+    L.red = L.cv + 2 * NW;
+    L.ci = (int *)(L.red + 3 * NW);
+    L.piv = L.ci + 2 * NW;
+    L.qinv = L.piv + TI * RA;
+    L.spc = L.qinv + TI * RA;
+    L.actl = L.spc + TI * RA;
+    L.posl = L.actl + TI * RA;
+    return L;
+}
+
+template <int RA, int RB, int TI, int TJ>
+__global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
+    constexpr int NT = TI * TJ, NW = NT / 64;
+    extern __shared__ __align__(16) char lds_raw[];
+    const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL;
+    const int tid = threadIdx.x, li = tid / TJ, lj = tid % TJ, lane = tid & 63, wv = tid >> 6;
+    const int iw = A.iw0 + blockIdx.x;
+    const WLayout L = wlayout(lds_raw, n, nCL, RA, RB, TI, TJ);
+    const double w = A.w_init + (double)(iw + 1) * A.delta;
+    const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
+    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
+    // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
+    double2 *Ms = A.mscr + (size_t)blockIdx.x * 3 * n * n;  // [c][r]: M column-major
+    double2 *Gs = Ms + n * n;                                // [i][j]: G^r row-major
+    double2 *Ts = Gs + n * n;                                // [c][r]: T column-major
+    const int colbase = lj * n + li;                         // (r, c) = (li + TI a, lj + TJ b) at colbase + TJ b n + TI a
+
+    for (int i = tid; i < n; i += NT) L.posl[i] = A.pos[i];
+    for (int i = tid; i < nA; i += NT) L.actl[i] = A.act[i];
+
+    // ---- 1. A = w + Hemb - Sigma1 - Sigma2^r ------------------------------------
+    double2 M[RA][RB];
+    // row-clamped base per a, columns past n read the allocation's zero pad
+#pragma unroll
+    for (int a = 0; a < RA; a++) {
+        const double2 *hrow = A.Hemb + min(li + TI * a, n - 1) * n + lj;
+#pragma unroll
+        for (int b = 0; b < RB; b++) M[a][b] = hrow[TJ * b];
+    }
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++)
+            if (li + TI * a == lj + TJ * b) M[a][b].x += w;
+    if (A.use_sigma && A.sig1) {
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const int i = li + TI * a, j = lj + TJ * b;
+                if ((i >> 1) == (j >> 1) && i < n) M[a][b] = csub(M[a][b], A.sig1[(i >> 1) * 4 + (i & 1) * 2 + (j & 1)]);
+            }
+    }
+    if (A.use_sigma && sigw) {
+        int pc[RB];
+#pragma unroll
+        for (int b = 0; b < RB; b++) pc[b] = A.pos[min(lj + TJ * b, n - 1)];
+#pragma unroll
+        for (int a = 0; a < RA; a++) {
+            const int pa = A.pos[min(li + TI * a, n - 1)];
+            double2 s2v[RB];
+#pragma unroll
+            for (int b = 0; b < RB; b++) s2v[b] = sigw[(pa >= 0 && pc[b] >= 0) ? sig_index(pa, pc[b], A.nAs) : 0];
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+                if (pa >= 0 && pc[b] >= 0) M[a][b] = csub(M[a][b], s2v[b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+            const int i = li + TI * a, j = lj + TJ * b;
+            if (i >= n || j >= n) M[a][b] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+    unsigned used = 0;  // bit a: row li + TI a already pivoted (or padding)
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+        if (li + TI * a >= n) used |= 1u << a;
+
+    // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
+    // step k = -1 only posts the candidates of column 0
+    for (int k = -1; k < n; k++) {
+        if (k >= 0) {
+            const int buf = k & 1, kb = k / TJ, kl = k % TJ;
+            double bv = -1.0;
+            int p = -1, pw = 0;
+#pragma unroll
+            for (int w2 = 0; w2 < NW; w2++) {
+                const int i = L.ci[buf * NW + w2];
+                const double v = L.cv[buf * NW + w2];
+                if (i >= 0 && (p < 0 || v > bv || (v == bv && i < p))) { bv = v; p = i; pw = w2; }
+            }
+            if (p < 0) p = 0;  // unreachable: n - k unused rows remain at step k
+            p = __builtin_amdgcn_readfirstlane(p);
+            pw = __builtin_amdgcn_readfirstlane(pw);
+            if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
+            const double2 *prow = L.crow + (buf * NW + pw) * TJ * RB;
+            const double2 *pcl = L.pcol + buf * TI * RA;
+            const int tp = p % TI, ap = p / TI;
+            const double2 pv = pcl[p];
+            if (tid == 0 && pv.x == 0.0 && pv.y == 0.0) atomicOr(A.status, 1);
+            const double2 rinv = cinv(pv);
+            double2 sp[RB];
+#pragma unroll
+            for (int b = 0; b < RB; b++) sp[b] = cmul(prow[lj + TJ * b], rinv);
+            if (lj == kl) {
+#pragma unroll
+                for (int b = 0; b < RB; b++)
+                    if (b == kb) {
+                        sp[b] = rinv;
+#pragma unroll
+                        for (int a = 0; a < RA; a++) M[a][b] = make_double2(0.0, 0.0);
+                    }
+            }
+#pragma unroll
+            for (int a = 0; a < RA; a++) {
+                const double2 fa = pcl[li + TI * a];
+#pragma unroll
+                for (int b = 0; b < RB; b++) cfms(M[a][b], fa, sp[b]);
+            }
+            if (li == tp) {
+#pragma unroll
+                for (int a = 0; a < RA; a++)
+                    if (a == ap) {
+                        used |= 1u << a;
+#pragma unroll
+                        for (int b = 0; b < RB; b++) M[a][b] = sp[b];
+                    }
+            }
+        }
+        if (k + 1 < n) {
+            const int kk = k + 1, buf = kk & 1, bkk = kk / TJ, tkk = kk % TJ;
+            double best = -1.0;
+            int bi = -1;
+            if (lj == tkk) {
+#pragma unroll
+                for (int b = 0; b < RB; b++)
+                    if (b == bkk)
+#pragma unroll
+                        for (int a = 0; a < RA; a++) {
+                            L.pcol[buf * TI * RA + li + TI * a] = M[a][b];
+                            if (!(used & (1u << a))) {
+                                double v = fabs(M[a][b].x) + fabs(M[a][b].y);
+                                if (v != v) v = -0.5;  // NaN never wins over a number
+                                if (bi < 0 || v > best) { best = v; bi = li + TI * a; }
+                            }
+                        }
+            }
+            // the wavefront's owners of column kk: lanes tkk + TJ j
+#pragma unroll
+            for (int o = TJ; o <= 32; o <<= 1) {
+                const double vo = __shfl_xor(best, o, 64);
+                const int io = __shfl_xor(bi, o, 64);
+                if (io >= 0 && (bi < 0 || vo > best || (vo == best && io < bi))) { best = vo; bi = io; }
+            }
+            const int rw = __builtin_amdgcn_readlane(bi, tkk);
+            if (lane == tkk) {
+                L.cv[buf * NW + wv] = best;
+                L.ci[buf * NW + wv] = bi;
+            }
+            if (rw >= 0 && li == rw % TI) {
+                const int aw = rw / TI;
+                double2 *dst = L.crow + (buf * NW + wv) * TJ * RB;
+#pragma unroll
+                for (int a = 0; a < RA; a++)
+                    if (a == aw)
+#pragma unroll
+                        for (int b = 0; b < RB; b++) dst[lj + TJ * b] = M[a][b];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. scratch copies of M: Ms (column-major), Gs (G^r, natural) ------------
+    for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+            const int r = li + TI * a, c = lj + TJ * b;
+            if (r < n && c < n) {
+                Ms[colbase + TJ * b * n + TI * a] = M[a][b];
+                Gs[L.qinv[r] * n + L.piv[c]] = M[a][b];
+            }
+        }
+    __threadfence_block();
+    __syncthreads();
+
+    // ---- 4. T = M S'' -----------------------------------------------------------
+    // step c: T += M(:, c) S''(c, :), S''(c, c') = SL(spc[c], spc[c']) on the
+    // support: i (f_L GamL + f_R GamR) + Sigma2^<  (src/scf.f90:333-336).
+    // Row c of S'' is built by the workgroup into srow[c & 1]; the raw values
+    // and my rows of M(:, c) are fetched one step ahead.
+    constexpr int QS = (TJ * RB + NT - 1) / NT;
+    double2 T[RA][RB];
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++) T[a][b] = make_double2(0.0, 0.0);
+    {
+        double2 mn[RA], gl[QS], gr[QS], sg[QS];
+        unsigned okm = 0;
+        auto fetch = [&](int c) {
+#pragma unroll
+            for (int a = 0; a < RA; a++) mn[a] = Ms[c * n + min(li + TI * a, n - 1)];
+            const int b1 = L.spc[c];
+            okm = 0;
+#pragma unroll
+            for (int q = 0; q < QS; q++) {
+                const int cc = tid + NT * q;
+                const int b2 = cc < n ? L.spc[cc] : -1;
+                const bool ok = b1 >= 0 && b2 >= 0;
+                if (ok) okm |= 1u << q;
+                const int gi = ok ? b1 * A.nS + b2 : 0;
+                gl[q] = A.gamL_s[gi];
+                gr[q] = A.gamR_s[gi];
+                sg[q] = sigw ? sigw[ok ? sig_index(b1, b2, A.nAs) + 4 : 0] : make_double2(0.0, 0.0);
+            }
+        };
+        fetch(0);
+        for (int c = 0; c < n; c++) {
+            double2 *sr = L.srow + (c & 1) * TJ * RB;
+#pragma unroll
+            for (int q = 0; q < QS; q++) {
+                const int cc = tid + NT * q;
+                if (cc < TJ * RB) {
+                    double2 v = make_double2(0.0, 0.0);
+                    if (okm & (1u << q)) {
+                        const double re = fL * gl[q].x + fR * gr[q].x, im = fL * gl[q].y + fR * gr[q].y;
+                        v = make_double2(-im, re);
+                        if (sigw) v = cadd(v, sg[q]);
+                    }
+                    sr[cc] = v;
+                }
+            }
+            double2 mc[RA];
+#pragma unroll
+            for (int a = 0; a < RA; a++) mc[a] = mn[a];
+            __syncthreads();
+            if (c + 1 < n) fetch(c + 1);
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const double2 sv = sr[lj + TJ * b];
+#pragma unroll
+                for (int a = 0; a < RA; a++) cfma(T[a][b], mc[a], sv);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+            const int r = li + TI * a, c = lj + TJ * b;
+            if (r < n && c < n) Ts[colbase + TJ * b * n + TI * a] = T[a][b];
+        }
+    __threadfence_block();
+    __syncthreads();
+
+    // ---- 5. P = T M^dagger --------------------------------------------------------
+    // step c: P += T(:, c) conj(M(:, c))^T: my rows of T(:, c) from Ts, my
+    // columns (rows lj + TJ b of M) from Ms, fetched one step ahead.
+    double2 P[RA][RB];
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < RB; b++) P[a][b] = make_double2(0.0, 0.0);
+    {
+        double2 tn[RA], mn[RB];
+        auto fetch = [&](int c) {
+#pragma unroll
+            for (int a = 0; a < RA; a++) tn[a] = Ts[c * n + min(li + TI * a, n - 1)];
+#pragma unroll
+            for (int b = 0; b < RB; b++) mn[b] = Ms[c * n + min(lj + TJ * b, n - 1)];
+        };
+        fetch(0);
+        for (int c = 0; c < n; c++) {
+            double2 tc[RA], mc[RB];
+#pragma unroll
+            for (int a = 0; a < RA; a++) tc[a] = tn[a];
+#pragma unroll
+            for (int b = 0; b < RB; b++) mc[b] = mn[b];
+            if (c + 1 < n) fetch(c + 1);
+#pragma unroll
+            for (int a = 0; a < RA; a++)
+#pragma unroll
+                for (int b = 0; b < RB; b++) cfmac(P[a][b], tc[a], mc[b]);
+        }
+    }
+
+    // ---- 6. epilogues ---------------------------------------------------------------
+    const size_t nblk = 3 * (size_t)nA2, off = (size_t)iw * nblk;
+    const bool scf2 = A.mode == QS_M_SCF2, init2 = A.mode == QS_M_INIT2;
+    // this energy's packed slices; offsets below are 32-bit (3 nA^2 < 2^31)
+    const double *xe = A.x + off, *oe = A.Fold + off;
+    double *fe = A.F + off, *ye = A.xout + off;
+    double st_max = 0.0, st_s2 = 0.0, st_d2 = 0.0;
+    const int nsu = (nA2 - nA) / 2;
+    // G^< in TI-row stages: pivoted row r = TI a + s is natural row qinv[r]
+#pragma unroll
+    for (int a = 0; a < RA; a++) {
+        if (TI * a >= n) break;
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+            const int r = li + TI * a, c = lj + TJ * b;
+            if (r < n && c < n) L.stage[li * n + L.qinv[c]] = P[a][b];
+        }
+        __syncthreads();
+        const int rows = min(TI, n - TI * a);
+        if (init2 || scf2) {
+            // element e of the stage: row s = e / nA, active column index bb = e % nA
+            for (int e0 = tid; e0 < rows * nA; e0 += NT * EBL) {
+                double xv[EBL][2], ov[EBL][2];
+                int dst[EBL][2];
+#pragma unroll
+                for (int u = 0; u < EBL; u++) {
+                    const int e = e0 + NT * u, s = e / nA, bb = e - s * nA;
+                    dst[u][0] = dst[u][1] = -1;
+                    if (e < rows * nA) {
+                        const int ar = L.posl[L.qinv[TI * a + s]];
+                        if (ar >= 0) {
+                            if (ar < bb) dst[u][0] = su_index(ar, bb, nA);
+                            if (ar <= bb) dst[u][1] = nsu + u_index(ar, bb, nA);
+                        }
+                    }
+                    if (scf2) {
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            const int d = 2 * nA2 + (dst[u][h] >= 0 ? dst[u][h] : 0);
+                            xv[u][h] = xe[d];
+                            ov[u][h] = oe[d];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < EBL; u++) {
+                    const int e = e0 + NT * u, s = e / nA, bb = e - s * nA;
+                    if (e < rows * nA && dst[u][1] >= 0) {
+                        const double2 g = L.stage[s * n + L.actl[bb]];
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            if (dst[u][h] < 0) continue;
+                            const double val = h == 0 ? g.x : g.y;
+                            const int d = 2 * nA2 + dst[u][h];
+                            if (init2) {
+                                ye[d] = val;
+                            } else {
+                                const double f = val - xv[u][h];
+                                fe[d] = f;
+                                st_max = fmax(st_max, fabs(f));
+                                st_s2 += f * f;
+                                const double e1 = f - ov[u][h];
+                                st_d2 += e1 * e1;
+                            }
+                        }
+                        const int ar = L.posl[L.qinv[TI * a + s]];
+                        if (scf2 && ar == bb) A.diagL[(size_t)iw * nA + ar] = g.y;
+                    }
+                }
+            }
+        } else {
+            for (int s = 0; s < rows; s++) {
+                const int i = L.qinv[TI * a + s];
+                if (A.mode == QS_M_SCF1 && tid == 0 && L.posl[i] >= 0)
+                    A.diagL[(size_t)iw * nA + L.posl[i]] = L.stage[s * n + i].y;
+                // on-site 2x2 blocks: element (2 si + a2, 2 si + b2) is stored at 4 si + a2 + 2 b2
+                if (tid < 2 && (i >> 1) < A.nsites)
+                    A.onsL[(size_t)iw * 4 * A.nsites + 4 * (i >> 1) + (i & 1) + 2 * tid] = L.stage[s * n + 2 * (i >> 1) + tid];
+                if (A.mode == QS_M_FINAL) {
+                    if (A.atomL) {
+                        // atom block of row i: element (lo + a2, lo + b2), stored at b2 * m + a2
+                        int at = 0;
+                        while (at + 1 < A.n_atoms && 2 * A.atom_first[at + 1] <= i) at++;
+                        const int lo = 2 * A.atom_first[at], m = 2 * A.atom_norb[at], a2 = i - lo;
+                        if (a2 >= 0 && a2 < m)
+                            for (int b2 = tid; b2 < m; b2 += NT)
+                                A.atomL[(size_t)iw * A.atom_tot + A.atom_off[at] + b2 * m + a2] = L.stage[s * n + lo + b2];
+                    }
+                    for (int e = tid; e < nCL * nCL; e += NT) {
+                        const int l = e / nCL, a2 = e - l * nCL;
+                        if (A.cl[l] == i) L.glcl[e] = L.stage[s * n + A.cl[a2]];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // G^r from Gs rows (natural order)
+    if (init2 || scf2) {
+        for (int e0 = tid; e0 < nA2; e0 += NT * EB) {
+            double x0[EB], x1[EB], o0[EB], o1[EB];
+            double2 g[EB];
+#pragma unroll
+            for (int u = 0; u < EB; u++) {
+                const int e = min(e0 + NT * u, nA2 - 1), ar = e / nA, bb = e - ar * nA;
+                g[u] = Gs[L.actl[ar] * n + L.actl[bb]];
+                if (scf2) {
+                    x0[u] = xe[e]; x1[u] = xe[e + nA2];
+                    o0[u] = oe[e]; o1[u] = oe[e + nA2];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < EB; u++) {
+                const int e = e0 + NT * u;
+                if (e >= nA2) continue;
+                if (init2) {
+                    ye[e] = g[u].x;
+                    ye[e + nA2] = g[u].y;
+                } else {
+                    const double f0 = g[u].x - x0[u], f1 = g[u].y - x1[u];
+                    fe[e] = f0;
+                    fe[e + nA2] = f1;
+                    st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
+                    st_s2 += f0 * f0 + f1 * f1;
+                    const double e0v = f0 - o0[u], e1v = f1 - o1[u];
+                    st_d2 += e0v * e0v + e1v * e1v;
+                    const int ar = e / nA;
+                    if (ar * nA + ar == e) A.diagR[(size_t)iw * nA + ar] = g[u].y;
+                }
+            }
+        }
+        if (scf2) {
+            st_max = wave_max(st_max);
+            st_s2 = wave_sum(st_s2);
+            st_d2 = wave_sum(st_d2);
+            if (lane == 0) { L.red[3 * wv] = st_max; L.red[3 * wv + 1] = st_s2; L.red[3 * wv + 2] = st_d2; }
+            __syncthreads();
+            if (tid == 0) {
+                double m = L.red[0], s2 = L.red[1], d2 = L.red[2];
+                for (int w2 = 1; w2 < NW; w2++) { m = fmax(m, L.red[3 * w2]); s2 += L.red[3 * w2 + 1]; d2 += L.red[3 * w2 + 2]; }
+                double *pp = A.partials + (size_t)iw * 4;
+                pp[0] = m; pp[1] = s2; pp[2] = d2; pp[3] = 0.0;
+            }
+        }
+    } else if (A.mode == QS_M_SCF1) {
+        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = Gs[L.actl[k2] * n + L.actl[k2]].y;
+    } else {
+        for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = Gs[i * n + i].y;
+        for (int e = tid; e < nCL * nCL; e += NT) {
+            const int k2 = e / nCL, a2 = e - k2 * nCL;
+            L.grcl[e] = Gs[A.cl[k2] * n + A.cl[a2]];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // Tr_s{ i GamL [ (f_L - 1) G^< - f_L G^> ] } / 2pi on the left contact  (src/scf.f90:390-409)
+            double tu = 0.0, td = 0.0;
+            for (int a2 = 0; a2 < nCL; a2++) {
+                double2 s = make_double2(0.0, 0.0);
+                for (int l = 0; l < nCL; l++) {
+                    const double2 glc = L.glcl[l * nCL + a2];
+                    const double2 gr_la = L.grcl[l * nCL + a2], gr_al = L.grcl[a2 * nCL + l];
+                    const double2 gg = cadd(glc, csub(gr_la, cconj(gr_al)));
+                    const double2 X = make_double2((fL - 1.0) * glc.x - fL * gg.x, (fL - 1.0) * glc.y - fL * gg.y);
+                    cfma(s, A.gamL_cl[a2 * nCL + l], X);
+                }
+                if ((A.cl[a2] & 1) == 0) tu += -s.y; else td += -s.y;
+            }
+            A.trUD[(size_t)iw * 2 + 0] = tu / TWO_PI;
+            A.trUD[(size_t)iw * 2 + 1] = td / TWO_PI;
+        }
+    }
+}
+
+template <int RA, int RB, int TI, int TJ>
+hipError_t launch_w(const DysonArgs &a0, int count, hipStream_t s) {
+    const size_t lds = wlayout_bytes(a0.norb, a0.nCL, RA, RB, TI, TJ);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_dyson_w<RA, RB, TI, TJ>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    // the scratch slab holds mscr_energies energies: launch in chunks of that many
+    for (int c0 = 0; c0 < count; c0 += a0.mscr_energies) {
+        DysonArgs a = a0;
+        a.iw0 = a0.iw0 + c0;
+        const int cnt = count - c0 < a0.mscr_energies ? count - c0 : a0.mscr_energies;
+        hipLaunchKernelGGL((k_dyson_w<RA, RB, TI, TJ>), dim3(cnt), dim3(TI * TJ), lds, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+// 8 x 8 threads (one wavefront) up to 16 orbitals, 16 x 16 above
+size_t qs_dyson_wave_lds_bytes(int n, int nCL) {
+    const int T = n <= 16 ? 8 : 16, R = (n + T - 1) / T;
+    return wlayout_bytes(n, nCL, R, R, T, T);
+}
+
+size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)3 * n * n * sizeof(double2); }
+
+int qs_dyson_wave_max_norb() { return 96; }
+
+hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (!a.mscr || a.mscr_energies <= 0 || a.norb > qs_dyson_wave_max_norb()) return hipErrorInvalidValue;
+    if (qs_dyson_wave_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
+    const int n = a.norb;
+    if (n <= 8) return launch_w<1, 1, 8, 8>(a, count, s);
+    if (n <= 16) return launch_w<2, 2, 8, 8>(a, count, s);
+    switch ((n + 15) / 16) {
+        case 2: return launch_w<2, 2, 16, 16>(a, count, s);
+        case 3: return launch_w<3, 3, 16, 16>(a, count, s);
+        case 4: return launch_w<4, 4, 16, 16>(a, count, s);
+        case 5: return launch_w<5, 5, 16, 16>(a, count, s);
+        case 6: return launch_w<6, 6, 16, 16>(a, count, s);
+        default: return hipErrorInvalidValue;
+    }
+}

@@ -33,6 +33,9 @@ using zd = std::complex<double>;
 constexpr double PI = 3.14159265358979323846;
 constexpr int MAXH_DEV = 32;
 
+// energies per launch of the one-wavefront Dyson kernel (bounds its scratch)
+constexpr int kWaveChunk = 8192;
+
 // stages timed with HIP events on the handle's stream (qs_profile)
 enum Stage {
     ST_EQT = 0, ST_PERMUTE, ST_FFT_F1, ST_FFT_MID, ST_FFT_I2, ST_DYSON, ST_REDUCE, ST_HIST, ST_UPDATE, ST_FINAL,
@@ -83,6 +86,8 @@ struct qs_handle {
     double2 *d_Gt = nullptr, *d_Z = nullptr;  // Gt doubles as the Sigma2 output
     double2 *d_sig1 = nullptr, *d_gt0 = nullptr;
     double *d_partials = nullptr, *d_red = nullptr, *d_scratch = nullptr;
+    double2 *d_mscr = nullptr;   // one-wavefront Dyson kernel scratch (mscr_energies energies)
+    int mscr_energies = 0;
     double *d_diagR = nullptr, *d_diagL = nullptr, *d_dosd = nullptr, *d_trUD = nullptr;
     double2 *d_onsL = nullptr, *d_atomL = nullptr;
     int *d_status = nullptr;
@@ -263,7 +268,13 @@ int setup_device(qs_handle *h) {
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamCreate(&h->stream));
     HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 4096 * sizeof(double), hipHostMallocDefault));
-    if ((rc = upload_z(h, &h->d_Hemb, h->Hemb))) return rc;
+    {
+        // zero pad past n^2: the small-footprint Dyson kernel reads whole
+        // 16-column blocks of the last row
+        std::vector<zd> hp(h->Hemb);
+        hp.resize(h->Hemb.size() + 64, zd(0.0, 0.0));
+        if ((rc = upload_z(h, &h->d_Hemb, hp))) return rc;
+    }
     if ((rc = upload_z(h, &h->d_gamL_s, h->GamL_s))) return rc;
     if ((rc = upload_z(h, &h->d_gamR_s, h->GamR_s))) return rc;
     if ((rc = upload_z(h, &h->d_gamL_cl, h->GamL_cl))) return rc;
@@ -285,6 +296,10 @@ int setup_device(qs_handle *h) {
     if ((rc = dalloc(h, &h->d_diagR, (size_t)nw * std::max(nA, 1)))) return rc;
     if ((rc = dalloc(h, &h->d_diagL, (size_t)nw * std::max(nA, 1)))) return rc;
     if ((rc = dalloc(h, &h->d_partials, (size_t)nw * 4))) return rc;
+    if (n <= qs_dyson_wave_max_norb()) {
+        h->mscr_energies = std::min(nw, kWaveChunk);
+        if ((rc = dalloc(h, &h->d_mscr, h->mscr_energies * qs_dyson_wave_scratch_bytes(n) / sizeof(double2)))) return rc;
+    }
     if ((rc = dalloc(h, &h->d_red, 64))) return rc;
     // column-sum scratch: 64 row-chunks x the widest column set
     int widest = std::max({8 * h->nsites, n, 2 * std::max(nA, 1), 2 * std::max(h->atom_tot, 1), 2});
@@ -407,6 +422,8 @@ DysonArgs dyson_args(qs_handle *h, int mode, bool use_sigma, double V) {
     a.diagR = h->d_diagR; a.diagL = h->d_diagL;
     a.onsL = h->d_onsL; a.dosd = h->d_dosd; a.atomL = h->d_atomL; a.trUD = h->d_trUD;
     a.status = h->d_status;
+    a.mscr = h->d_mscr;
+    a.mscr_energies = h->mscr_energies;
     return a;
 }
 
@@ -473,10 +490,9 @@ int tune_dyson(qs_handle *h) {
     h->dyson_variant = 0;
     if (const char *e = getenv("QS_DYSON_VARIANT")) {
         const int v = atoi(e);
-        if (v >= 0 && v < nv) h->dyson_variant = v;
+        if (v >= 0 && v < nv && qs_dyson_shape_ok(h->norb, v)) h->dyson_variant = v;
         return QS_OK;
     }
-    if (nv <= 1) return QS_OK;
     const int count = std::min(h->nw, 2048);
     DysonArgs a = dyson_args(h, QS_M_FINAL, false, 0.0);
     hipEvent_t e0, e1;
@@ -484,6 +500,7 @@ int tune_dyson(qs_handle *h) {
     HIPCHK(h, hipEventCreate(&e1));
     double best = 1e300;
     for (int v = 0; v < nv; v++) {
+        if (!qs_dyson_shape_ok(h->norb, v)) continue;
         HIPCHK(h, qs_launch_dyson_variant(a, count, v, h->stream));  // warm-up (code load)
         HIPCHK(h, hipEventRecord(e0, h->stream));
         for (int r = 0; r < 2; r++) HIPCHK(h, qs_launch_dyson_variant(a, count, v, h->stream));
@@ -1253,6 +1270,7 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
         if (p->nhist > 0) b += 2 * nv * (int64_t)p->nhist * (p->single_history ? 4 : 8);
         b += (nw + Nf) * 2 * nA * nA * 16;
     }
+    if (n <= qs_dyson_wave_max_norb()) b += std::min<int64_t>(nw, kWaveChunk) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
     return b + ((int64_t)64 << 20);
 }
 

@@ -35,6 +35,8 @@ struct DysonArgs {
     double2 *atomL;          // FINAL: [iw][atom_tot]
     double *trUD;            // FINAL: [iw][2] current integrand up / down
     int *status;             // bit 0: singular pivot met
+    double2 *mscr;           // one-wavefront kernel: 3 norb^2 complex of scratch per energy
+    int mscr_energies;       //   for this many energies (launches are chunked to it)
 };
 
 // Dyson + Keldysh at every energy of [iw0, iw0 + count).  Returns the hip error.
@@ -44,6 +46,15 @@ int qs_dyson_variants(int norb);
 hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, hipStream_t s);
 size_t qs_dyson_lds_bytes(int norb, int nCL);
 int qs_dyson_max_norb();
+// launch shapes: 0 = 256-thread workgroup, 1 = 512-thread workgroup (norb > 16),
+// 2 = one wavefront per energy (norb <= qs_dyson_wave_max_norb(), needs mscr)
+enum { QS_DYSON_WG256 = 0, QS_DYSON_WG512 = 1, QS_DYSON_WAVE = 2, QS_DYSON_NSHAPES = 3 };
+bool qs_dyson_shape_ok(int norb, int shape);
+// one wavefront per energy (qs_dyson_wave.hip)
+hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s);
+size_t qs_dyson_wave_lds_bytes(int norb, int nCL);
+size_t qs_dyson_wave_scratch_bytes(int norb);
+int qs_dyson_wave_max_norb();
 
 // Second-order self-energy stage (qs_sigma2.hip)
 struct Sigma2Plan {

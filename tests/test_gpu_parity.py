@@ -135,10 +135,11 @@ def test_runaway_and_errors_are_reported():
     assert ab and not F.any()
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("norb", [18, 48, 80, 88])
 def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
-    """Both launch shapes of the Dyson kernel (256 and 512 threads per energy)."""
+    """Every launch shape of the Dyson kernel: 256- and 512-thread workgroups with
+    the matrix staged in LDS, and the 16 x 16 workgroup with a scratch slab."""
     monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
     prob = random_problem(norb, 48, seed=100 + norb, U=0.9)
     o = Oracle(prob)
@@ -151,3 +152,24 @@ def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     Fo, _ = o.residual2(0.1, x1)
     Fg, _ = g.residual2(0.1, x1)
     assert rel_max(Fg, Fo) < 1e-11
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("order", [1, 2])
+def test_each_dyson_launch_shape_full_solve(order, variant, monkeypatch):
+    """Full solves through one launch shape: the SCF1, SCF2, INIT2 and FINAL
+    epilogues (diagonals, on-site blocks, atom blocks, current integrand)."""
+    monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
+    prob = helix_problem(4, 3, 0.8, order=order, n_omega=300, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
+                         gamma=1.0, mu=2.0, tolf=1e-10, tolf_order1=1e-12, nhist=8, voltages=[0.3])
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    assert g.dyson_choice()[0] == variant
+    ro = o.solve(0.3, True)
+    rg = g.solve_scf(0.3, True)
+    assert ro["converged"] and rg.converged and ro["n_iter"] == rg.n_iter
+    Jo, Jg = np.array(o.current()), np.array(g.current())
+    assert np.all(np.abs(Jg - Jo) <= 1e-10 * np.abs(Jo) + 1e-12), (Jg, Jo)
+    Go, Gg = o.g_less_obs(), g.g_less_obs()
+    assert np.abs(Gg - Go).max() <= 1e-10 * np.abs(Go).max()
+    assert rel_max(g.dosd(), o.dosd()) < 1e-10
