@@ -73,9 +73,11 @@ def test_sigma2_nonpower_of_two_grid_and_noninteracting_site():
 @pytest.mark.parametrize("order,single", [(0, True), (1, True), (2, True), (1, False), (2, False)])
 def test_solve_scf_matches_oracle(order, single):
     # well-conditioned system (leads of 1 eV as in the reference's tests): the fixed point is unique and
-    # insensitive to rounding, so two implementations on the same Broyden path agree to rounding
+    # insensitive to rounding, so two implementations on the same Broyden path agree to rounding.
+    # The order-1 tolerance stays well above that residual's rounding floor (~1e-12), where the step
+    # count of any two builds (or one build on two thread counts) can differ by one.
     prob = helix_problem(4, 2, 0.8, order=order, n_omega=400, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
-                         gamma=1.0, mu=2.0, tolf=1e-10, tolf_order1=1e-12, nhist=8, voltages=[0.0, 0.3, 0.6],
+                         gamma=1.0, mu=2.0, tolf=1e-10, tolf_order1=1e-10, nhist=8, voltages=[0.0, 0.3, 0.6],
                          single_history=single)
     o = Oracle(prob)
     g = gpu_engine(prob)
@@ -161,7 +163,7 @@ def test_each_dyson_launch_shape_full_solve(order, variant, monkeypatch):
     epilogues (diagonals, on-site blocks, atom blocks, current integrand)."""
     monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
     prob = helix_problem(4, 3, 0.8, order=order, n_omega=300, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
-                         gamma=1.0, mu=2.0, tolf=1e-10, tolf_order1=1e-12, nhist=8, voltages=[0.3])
+                         gamma=1.0, mu=2.0, tolf=1e-10, tolf_order1=1e-10, nhist=8, voltages=[0.3])
     o = Oracle(prob)
     g = gpu_engine(prob)
     assert g.dyson_choice()[0] == variant
