@@ -162,8 +162,9 @@ def test_each_dyson_launch_shape_full_solve(order, variant, monkeypatch):
     """Full solves through one launch shape: the SCF1, SCF2, INIT2 and FINAL
     epilogues (diagonals, on-site blocks, atom blocks, current integrand)."""
     monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
+    # 24 spin-orbitals, leads of 1.5 eV: the step count does not depend on the oracle's thread count
     prob = helix_problem(4, 3, 0.8, order=order, n_omega=300, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
-                         gamma=1.0, mu=2.0, tolf=1e-10, tolf_order1=1e-10, nhist=8, voltages=[0.3])
+                         gamma=1.5, mu=2.0, tolf=1e-10, tolf_order1=1e-10, nhist=8, voltages=[0.3])
     o = Oracle(prob)
     g = gpu_engine(prob)
     assert g.dyson_choice()[0] == variant
