@@ -22,14 +22,14 @@
 //     column k+1 together with that whole row and column k+1 (double-buffered
 //     LDS); after the barrier every thread picks the pivot and has its row and
 //     column.
-//  3. G^r is packed from TI-row LDS stages of M; M goes to the slab
-//     column-major (Ms) for the second product.
+//  3. M goes to the slab twice: column-major (Ms, operand of the Keldysh
+//     products) and as G^r in natural order (Gs, for packing).
 //  4. T = M S'' with S''[c][c'] = SL[p_c][p_c'] (outer products over c: the
 //     column of M from Ms, the row of S'' built in LDS), then T to the slab (Ts).
 //  5. P = T M^dagger (outer products over c, both columns from the slab);
 //     G^<[q_r][q_r'] = P[r][r'].
-//  6. G^< epilogues from TI-row LDS stages of P; the loads of x and Fold are
-//     batched ahead of the stores in both epilogues.
+//  6. Epilogues by mode from TI-row LDS stages of G^< and from Gs rows, with
+//     the loads of x and Fold batched ahead of the stores.
 #include "qs_device.h"
 #include "qs_internal.h"
 
@@ -94,8 +94,9 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
     // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
-    double2 *Ms = A.mscr + (size_t)blockIdx.x * 2 * n * n;  // [c][r]: M column-major
-    double2 *Ts = Ms + n * n;                                // [c][r]: T column-major
+    double2 *Ms = A.mscr + (size_t)blockIdx.x * 3 * n * n;  // [c][r]: M column-major
+    double2 *Gs = Ms + n * n;                                // [i][j]: G^r row-major
+    double2 *Ts = Gs + n * n;                                // [c][r]: T column-major
     const int colbase = lj * n + li;                         // (r, c) = (li + TI a, lj + TJ b) at colbase + TJ b n + TI a
 
     for (int i = tid; i < n; i += NT) L.posl[i] = A.pos[i];
@@ -245,91 +246,18 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         __syncthreads();
     }
 
-    // ---- 3. G^r epilogue from TI-row stages of M; M to the slab ----------------------
+    // ---- 3. scratch copies of M: Ms (column-major), Gs (G^r, natural) ------------
     for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
 #pragma unroll
     for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < RB; b++) {
             const int r = li + TI * a, c = lj + TJ * b;
-            if (r < n && c < n) Ms[colbase + TJ * b * n + TI * a] = M[a][b];
-        }
-    const size_t nblk = 3 * (size_t)nA2, off = (size_t)iw * nblk;
-    const bool scf2 = A.mode == QS_M_SCF2, init2 = A.mode == QS_M_INIT2;
-    // this energy's packed slices; offsets below are 32-bit (3 nA^2 < 2^31)
-    const double *xe = A.x + off, *oe = A.Fold + off;
-    double *fe = A.F + off, *ye = A.xout + off;
-    double st_max = 0.0, st_s2 = 0.0, st_d2 = 0.0;
-    // pivoted row r = TI a + s of M is natural row qinv[r] of G^r, column c is piv[c]
-#pragma unroll
-    for (int a = 0; a < RA; a++) {
-        if (TI * a >= n) break;
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < RB; b++) {
-            const int r = li + TI * a, c = lj + TJ * b;
-            if (r < n && c < n) L.stage[li * n + L.piv[c]] = M[a][b];
-        }
-        __syncthreads();
-        const int rows = min(TI, n - TI * a);
-        if (init2 || scf2) {
-            // element e of the stage: row s = e / nA, active column bb = e % nA
-            for (int e0 = tid; e0 < rows * nA; e0 += NT * EB) {
-                double x0[EB], x1[EB], o0[EB], o1[EB];
-                int dst[EB];
-#pragma unroll
-                for (int u = 0; u < EB; u++) {
-                    const int e = e0 + NT * u, s2 = e / nA, bb = e - s2 * nA;
-                    dst[u] = -1;
-                    if (e < rows * nA) {
-                        const int ar = L.posl[L.qinv[TI * a + s2]];
-                        if (ar >= 0) dst[u] = ar * nA + bb;
-                    }
-                    if (scf2) {
-                        const int d = dst[u] >= 0 ? dst[u] : 0;
-                        x0[u] = xe[d]; x1[u] = xe[d + nA2];
-                        o0[u] = oe[d]; o1[u] = oe[d + nA2];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < EB; u++) {
-                    if (dst[u] < 0) continue;
-                    const int e = e0 + NT * u, s2 = e / nA, bb = e - s2 * nA;
-                    const double2 g = L.stage[s2 * n + L.actl[bb]];
-                    const int d = dst[u];
-                    if (init2) {
-                        ye[d] = g.x;
-                        ye[d + nA2] = g.y;
-                    } else {
-                        const double f0 = g.x - x0[u], f1 = g.y - x1[u];
-                        fe[d] = f0;
-                        fe[d + nA2] = f1;
-                        st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
-                        st_s2 += f0 * f0 + f1 * f1;
-                        const double e0v = f0 - o0[u], e1v = f1 - o1[u];
-                        st_d2 += e0v * e0v + e1v * e1v;
-                        const int ar = d / nA;
-                        if (ar == bb) A.diagR[(size_t)iw * nA + ar] = g.y;
-                    }
-                }
+            if (r < n && c < n) {
+                Ms[colbase + TJ * b * n + TI * a] = M[a][b];
+                Gs[L.qinv[r] * n + L.piv[c]] = M[a][b];
             }
-        } else {
-            for (int s2 = tid; s2 < rows; s2 += NT) {
-                const int i = L.qinv[TI * a + s2];
-                if (A.mode == QS_M_SCF1) {
-                    if (L.posl[i] >= 0) A.diagR[(size_t)iw * nA + L.posl[i]] = L.stage[s2 * n + i].y;
-                } else {
-                    A.dosd[(size_t)iw * n + i] = L.stage[s2 * n + i].y;
-                }
-            }
-            if (A.mode == QS_M_FINAL)
-                for (int e = tid; e < nCL * nCL; e += NT) {
-                    const int k2 = e / nCL, a2 = e - k2 * nCL;
-                    for (int s2 = 0; s2 < rows; s2++)
-                        if (L.qinv[TI * a + s2] == A.cl[k2]) L.grcl[e] = L.stage[s2 * n + A.cl[a2]];
-                }
         }
-    }
     __threadfence_block();
     __syncthreads();
 
@@ -337,8 +265,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     // step c: T += M(:, c) S''(c, :), S''(c, c') = SL(spc[c], spc[c']) on the
     // support: i (f_L GamL + f_R GamR) + Sigma2^<  (src/scf.f90:333-336).
     // Row c of S'' is built by the workgroup into srow[c & 1]; the raw values
-    // and my rows of M(:, c) (from Ms: M's registers are free from here on)
-    // are fetched one step ahead.
+    // and my rows of M(:, c) are fetched one step ahead.
     constexpr int QS = (TJ * RB + NT - 1) / NT;
     double2 T[RA][RB];
 #pragma unroll
@@ -435,7 +362,13 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         }
     }
 
-    // ---- 6. G^< epilogue ------------------------------------------------------------
+    // ---- 6. epilogues ---------------------------------------------------------------
+    const size_t nblk = 3 * (size_t)nA2, off = (size_t)iw * nblk;
+    const bool scf2 = A.mode == QS_M_SCF2, init2 = A.mode == QS_M_INIT2;
+    // this energy's packed slices; offsets below are 32-bit (3 nA^2 < 2^31)
+    const double *xe = A.x + off, *oe = A.Fold + off;
+    double *fe = A.F + off, *ye = A.xout + off;
+    double st_max = 0.0, st_s2 = 0.0, st_d2 = 0.0;
     const int nsu = (nA2 - nA) / 2;
     // G^< in TI-row stages: pivoted row r = TI a + s is natural row qinv[r]
 #pragma unroll
@@ -527,7 +460,40 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         __syncthreads();
     }
 
+    // G^r from Gs rows (natural order)
     if (init2 || scf2) {
+        for (int e0 = tid; e0 < nA2; e0 += NT * EB) {
+            double x0[EB], x1[EB], o0[EB], o1[EB];
+            double2 g[EB];
+#pragma unroll
+            for (int u = 0; u < EB; u++) {
+                const int e = min(e0 + NT * u, nA2 - 1), ar = e / nA, bb = e - ar * nA;
+                g[u] = Gs[L.actl[ar] * n + L.actl[bb]];
+                if (scf2) {
+                    x0[u] = xe[e]; x1[u] = xe[e + nA2];
+                    o0[u] = oe[e]; o1[u] = oe[e + nA2];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < EB; u++) {
+                const int e = e0 + NT * u;
+                if (e >= nA2) continue;
+                if (init2) {
+                    ye[e] = g[u].x;
+                    ye[e + nA2] = g[u].y;
+                } else {
+                    const double f0 = g[u].x - x0[u], f1 = g[u].y - x1[u];
+                    fe[e] = f0;
+                    fe[e + nA2] = f1;
+                    st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
+                    st_s2 += f0 * f0 + f1 * f1;
+                    const double e0v = f0 - o0[u], e1v = f1 - o1[u];
+                    st_d2 += e0v * e0v + e1v * e1v;
+                    const int ar = e / nA;
+                    if (ar * nA + ar == e) A.diagR[(size_t)iw * nA + ar] = g[u].y;
+                }
+            }
+        }
         if (scf2) {
             st_max = wave_max(st_max);
             st_s2 = wave_sum(st_s2);
@@ -541,7 +507,14 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
                 pp[0] = m; pp[1] = s2; pp[2] = d2; pp[3] = 0.0;
             }
         }
-    } else if (A.mode == QS_M_FINAL) {
+    } else if (A.mode == QS_M_SCF1) {
+        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = Gs[L.actl[k2] * n + L.actl[k2]].y;
+    } else {
+        for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = Gs[i * n + i].y;
+        for (int e = tid; e < nCL * nCL; e += NT) {
+            const int k2 = e / nCL, a2 = e - k2 * nCL;
+            L.grcl[e] = Gs[A.cl[k2] * n + A.cl[a2]];
+        }
         __syncthreads();
         if (tid == 0) {
             // Tr_s{ i GamL [ (f_L - 1) G^< - f_L G^> ] } / 2pi on the left contact  (src/scf.f90:390-409)
@@ -593,7 +566,7 @@ size_t qs_dyson_wave_lds_bytes(int n, int nCL) {
     return wlayout_bytes(n, nCL, R, R, T, T);
 }
 
-size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)2 * n * n * sizeof(double2); }
+size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)3 * n * n * sizeof(double2); }
 
 int qs_dyson_wave_max_norb() { return 96; }
 
