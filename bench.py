@@ -111,6 +111,23 @@ def stage_counts(eng, nh):
     }
 
 
+def measured_traffic(workload_name, stage):
+    """HBM bytes per launch of a stage's kernel from the committed counter summary
+    (profiles/<workload>_hbm_traffic.md: rocprofv3 FETCH_SIZE and WRITE_SIZE
+    passes scaled by a plain copy kernel of known size), or None."""
+    path = os.path.join(REPO, "profiles", "%s_hbm_traffic.md" % workload_name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == stage:
+            try:
+                return float(cells[5]) * 1e9
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
     """Oracle timed on bounded samples of one SCF iteration of the same workload."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -304,6 +321,7 @@ def main():
         ach = work / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": None}
+    roof["traffic"] = measured_traffic(args.workload, dom)
     roof["kernel"] = dom
     roof["avg_launch_ms"] = avg_s * 1e3
     stages = {s: {"ms": round(v[0], 3), "launches": v[1]} for s, v in prof.items() if v[1] > 0}
@@ -322,7 +340,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "form": "plain FP64 path, one bias point at a time",
                    "n_spin_orbitals": eng.norb, "n_omega": eng.prob.n_omega, "fft_length": eng.Nf,
-                   "dyson_threads_per_energy": 256 * (dv + 1)},
+                   "dyson_launch_shape": ["256-thread workgroup, matrix in LDS", "512-thread workgroup, matrix in LDS",
+                                          "small-LDS workgroup, scratch slab"][dv]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "scf_steps_per_point": float(np.mean(iters)),
         "all_converged": bool(all(conv)),
