@@ -25,8 +25,6 @@
 // Each pass reads and writes HBM once in 128-byte segments; the DFTs run in LDS
 // (radix-2, decimation in time, bit-reversed load).
 
-#include <cstdlib>
-
 #include "qs_device.h"
 #include "qs_internal.h"
 
@@ -131,48 +129,46 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 }
 
 // ---- F1 -----------------------------------------------------------------------
-// grid: (ncol / CBF, N2).  LDS: twiddles (N1 / 2) | CBF padded sequences.
-// CBF = 8 columns (128-byte segments) or 4 (64-byte segments, half the LDS).
-// Thread work is unrolled over constant bounds (CBF N1 <= 16 NT for N1 <= 512)
-// so that every global load of a round is in flight at once.
-template <int CBF>
+// grid: (ncol / CB, N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
+// Thread work is unrolled over constant bounds (CB N1 <= 16 NT) so that every
+// global load of a round is in flight at once.
+constexpr int QS = 16;  // staging rounds: CB N1 / NT <= 16 (N1 <= 512)
+
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
-    constexpr int LCB = CBF == 8 ? 3 : 2, QF = CBF * 512 / NT, QH = QF / 2;
     extern __shared__ __align__(16) char lds_raw[];
     const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
-    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y, half = CBF * N1 / 2;
+    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y, half = CB * N1 / 2;
     load_twl(twl, p.tw, N1, p.Nf);
-    // rows n1 >= N1 / 2 are the zero padding: the inputs are one round of QH
-    double2 v[QH];
+    // rows n1 >= N1 / 2 are the zero padding: the inputs are one round of 8
+    double2 v[8];
 #pragma unroll
-    for (int q = 0; q < QH; q++) {
-        const int t = threadIdx.x + q * NT, n = n2 + p.N2 * (t >> LCB);
+    for (int q = 0; q < 8; q++) {
+        const int t = threadIdx.x + q * NT, n = n2 + p.N2 * (t >> 3);
         v[q] = make_double2(0.0, 0.0);
-        if (t < half && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CBF + (t & (CBF - 1))];
+        if (t < half && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & 7)];
     }
-    // output twiddles e^{-2 pi i k1 n2 / Nf}, k1 = (tid >> LCB) + (NT / CBF) i
-    double2 w[QF];
+    // output twiddles e^{-2 pi i k1 n2 / Nf}, k1 = (tid >> 3) + (NT / 8) i
+    double2 w[QS];
 #pragma unroll
-    for (int i = 0; i < QF; i++) {
-        const int k1 = (threadIdx.x >> LCB) + (NT / CBF) * i;
+    for (int i = 0; i < QS; i++) {
+        const int k1 = (threadIdx.x >> 3) + (NT / 8) * i;
         w[i] = k1 < N1 ? p.tw[k1 * n2] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int q = 0; q < QF; q++) {
+    for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
-        if (t < CBF * N1)
-            buf[(t & (CBF - 1)) * ld + px(bitrev(t >> LCB, lg1))] = q < QH ? v[q] : make_double2(0.0, 0.0);
+        if (t < CB * N1) buf[(t & 7) * ld + px(bitrev(t >> 3, lg1))] = q < 8 ? v[q] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft(buf, CBF, ld, lg1, -1, twl);
+    lds_fft(buf, CB, ld, lg1, -1, twl);
 #pragma unroll
-    for (int i = 0; i < QF; i++) {
+    for (int i = 0; i < QS; i++) {
         const int t = threadIdx.x + i * NT;
-        if (t < CBF * N1) {
-            const int col = t & (CBF - 1), k1 = t >> LCB;
-            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CBF + col] = cmul(buf[col * ld + px(k1)], w[i]);
+        if (t < CB * N1) {
+            const int col = t & 7, k1 = t >> 3;
+            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * ld + px(k1)], w[i]);
         }
     }
 }
@@ -365,35 +361,33 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 }
 
 // ---- I2 -----------------------------------------------------------------------
-// grid: (ncol / CBF, N2).  LDS as in F1.  Only the outputs n < nw are kept.
-template <int CBF>
+// grid: (ncol / CB, N2).  LDS as in F1.  Only the outputs n < nw are kept.
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
-    constexpr int LCB = CBF == 8 ? 3 : 2, QF = CBF * 512 / NT;
     extern __shared__ __align__(16) char lds_raw[];
     const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
     load_twl(twl, p.tw, N1, p.Nf);
-    double2 v[QF];
+    double2 v[QS];
 #pragma unroll
-    for (int q = 0; q < QF; q++) {
+    for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
         v[q] = make_double2(0.0, 0.0);
-        if (t < CBF * N1) v[q] = Z[((size_t)(t >> LCB) * p.N2 + n2) * ncol + cb * CBF + (t & (CBF - 1))];
+        if (t < CB * N1) v[q] = Z[((size_t)(t >> 3) * p.N2 + n2) * ncol + cb * CB + (t & 7)];
     }
 #pragma unroll
-    for (int q = 0; q < QF; q++) {
+    for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
-        if (t < CBF * N1) buf[(t & (CBF - 1)) * ld + px(bitrev(t >> LCB, lg1))] = v[q];
+        if (t < CB * N1) buf[(t & 7) * ld + px(bitrev(t >> 3, lg1))] = v[q];
     }
     __syncthreads();
-    lds_fft(buf, CBF, ld, lg1, +1, twl);
+    lds_fft(buf, CB, ld, lg1, +1, twl);
 #pragma unroll
-    for (int i = 0; i < QF; i++) {
+    for (int i = 0; i < QS; i++) {
         const int t = threadIdx.x + i * NT;
-        const int col = t & (CBF - 1), n1 = t >> LCB, n = n2 + p.N2 * n1;
-        if (t < CBF * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CBF + col] = buf[col * ld + px(n1)];
+        const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
+        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * ld + px(n1)];
     }
 }
 
@@ -408,23 +402,19 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
                             hipStream_t s) {
     const int ncol = 2 * p.nA * p.nA;
+    const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)CB * (p.N1 + p.N1 / 8 + 1)) * sizeof(double2);
     const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)16 * (p.N2 + p.N2 / 8 + 1)) * sizeof(double2);
-    static int cbf = 0;
-    if (cbf == 0) {
-        const char *e = getenv("QS_FFT_CB");
-        cbf = (e && atoi(e) == 8) ? 8 : 4;
-        (void)hipFuncSetAttribute((const void *)k_fft_f1<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_f1<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
     }
-    const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)cbf * (p.N1 + p.N1 / 8 + 1)) * sizeof(double2);
-    const dim3 g1(ncol / cbf, p.N2);
+    const dim3 g1(ncol / CB, p.N2);
     hipError_t e = hipSuccess;
     if (parts & 1) {
-        if (cbf == 8) hipLaunchKernelGGL(k_fft_f1<8>, g1, dim3(NT), lds1, s, Gt, Z, p);
-        else hipLaunchKernelGGL(k_fft_f1<4>, g1, dim3(NT), lds1, s, Gt, Z, p);
+        hipLaunchKernelGGL(k_fft_f1, g1, dim3(NT), lds1, s, Gt, Z, p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (parts & 2) {
@@ -433,8 +423,7 @@ hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, 
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (parts & 4) {
-        if (cbf == 8) hipLaunchKernelGGL(k_fft_i2<8>, g1, dim3(NT), lds1, s, Z, Sig, p);
-        else hipLaunchKernelGGL(k_fft_i2<4>, g1, dim3(NT), lds1, s, Z, Sig, p);
+        hipLaunchKernelGGL(k_fft_i2, g1, dim3(NT), lds1, s, Z, Sig, p);
         e = hipGetLastError();
     }
     return e;
