@@ -97,7 +97,8 @@ def workload(name):
 def stage_counts(eng, nh):
     """Algorithmic work per launch of each stage (flops or HBM bytes)."""
     nw, norb, nA, nv, Nf = eng.prob.n_omega, eng.norb, eng.nA, eng.n_vec, eng.Nf
-    ncol = 2.0 * nA * nA          # complex columns of the blocked Sigma2 layout
+    nAs = nA // 2
+    ncol = 12.0 * (nAs * (nAs + 1) // 2)   # complex columns of the site-pair layout (12 per unordered pair)
     return {
         "dyson_keldysh": ("mfma", nw * (8.0 * norb ** 3 + 16.0 * nA ** 3), "flop"),
         # each pass reads its input once and writes its output once
@@ -106,7 +107,7 @@ def stage_counts(eng, nh):
         "sigma2_fft_i2": ("hbm", 16.0 * ncol * (Nf + nw), "byte"),
         "broyden_history": ("hbm", nv * (32.0 + 4.0 * (nh - 1) + 8.0), "byte"),
         "broyden_update": ("hbm", nv * (16.0 + 4.0 * nh + 8.0), "byte"),
-        "permute_in": ("hbm", nv * 8.0 + 32.0 * nA * nA * nw, "byte"),
+        "permute_in": ("hbm", nv * 8.0 + 16.0 * ncol * nw, "byte"),
         "equal_time_sigma1": ("hbm", 8.0 * 4 * (nA // 2) * nw, "byte"),
     }
 

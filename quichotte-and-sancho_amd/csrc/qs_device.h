@@ -82,3 +82,27 @@ __device__ __host__ __forceinline__ double qs_fermi(double w, double V, double m
     if (e > 700.0) e = 700.0;
     return 1.0 / (exp(e) + 1.0);
 }
+
+// Site-pair layout of G and Sigma2 on the active block (the FFT columns and the
+// Sigma2 read by the Dyson kernels).  Per unordered active site pair (I <= J),
+// pair index sig_pair(I, J) (row-major upper triangle), 12 complex values:
+//   [0, 4)  retarded block (I, J), element 2 s_i + s_j
+//   [4, 8)  lesser   block (I, J), element 2 s_i + s_j
+//   [8, 12) retarded block (J, I), element 2 s_j + s_i (row spin first)
+// The lesser block (J, I) is not stored: G^< and Sigma^< are anti-Hermitian,
+// L(J, I) = -L(I, J)^dagger.  For I = J slots 8-11 repeat slots 0-3.
+__host__ __device__ inline int sig_pair(int I, int J, int nAs) { return I * nAs - (I * (I - 1)) / 2 + (J - I); }
+// retarded element (a, b) of the active block (a, b: active spin-orbital indices)
+__host__ __device__ inline int sig_r_index(int a, int b, int nAs) {
+    const int I = a >> 1, J = b >> 1;
+    return I <= J ? 12 * sig_pair(I, J, nAs) + ((a & 1) << 1) + (b & 1)
+                  : 12 * sig_pair(J, I, nAs) + 8 + ((a & 1) << 1) + (b & 1);
+}
+// lesser element (a, b): stored index and whether it is the -conj of the stored value
+__host__ __device__ inline int sig_l_index(int a, int b, int nAs, bool *flip) {
+    const int I = a >> 1, J = b >> 1;
+    *flip = I > J;
+    return I <= J ? 12 * sig_pair(I, J, nAs) + 4 + ((a & 1) << 1) + (b & 1)
+                  : 12 * sig_pair(J, I, nAs) + 4 + ((b & 1) << 1) + (a & 1);
+}
+__device__ __forceinline__ double2 sig_l_value(double2 v, bool flip) { return flip ? make_double2(-v.x, v.y) : v; }

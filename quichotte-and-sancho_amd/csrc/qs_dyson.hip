@@ -40,10 +40,6 @@ namespace {
 
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
-__device__ __forceinline__ int sig_block_index(int a, int b, int nAs) {
-    return ((((a >> 1) * nAs + (b >> 1)) * 2) << 2) + ((a & 1) << 1) + (b & 1);
-}
-
 struct LdsLayout {
     double2 *big;   // n x (n+1)
     double2 *chunk; // 16 x 16R
@@ -83,7 +79,7 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const int nA = A.nA, nA2 = nA * nA;
-    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
+    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
 
     // ---- 1. A = w + Hemb - Sigma1 - Sigma2^r, in registers -------------------------
     // Every load of the tile is issued before any is used (clamped in-bounds
@@ -117,7 +113,7 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
 #pragma unroll
             for (int b = 0; b < RB; b++) {
                 const bool ok = pr[a] >= 0 && pc[b] >= 0;
-                s2v[a][b] = sigw[ok ? sig_block_index(pr[a], pc[b], A.nAs) : 0];
+                s2v[a][b] = sigw[ok ? sig_r_index(pr[a], pc[b], A.nAs) : 0];
             }
     }
 #pragma unroll
@@ -321,7 +317,9 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
             const int gi = ok ? b1 * A.nS + b2 : 0;
             fgl[q] = A.gamL_s[gi];
             fgr[q] = A.gamR_s[gi];
-            fsg[q] = sigw ? sigw[ok ? sig_block_index(b1, b2, A.nAs) + 4 : 0] : make_double2(0.0, 0.0);
+            bool flip = false;
+            const int si = ok ? sig_l_index(b1, b2, A.nAs, &flip) : 0;
+            fsg[q] = sigw ? sig_l_value(sigw[si], flip) : make_double2(0.0, 0.0);
             if (ok) fok |= 1u << q;
         }
     };

@@ -39,11 +39,6 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 constexpr int EB = 8;   // G^r packing elements per thread in flight
 constexpr int EBL = 4;  // G^< packing elements per thread in flight (P is live then)
 
-// blocked Sigma2 layout: retarded value of the active pair (a, b); +4 is the lesser value
-__device__ __forceinline__ int sig_index(int a, int b, int nAs) {
-    return ((((a >> 1) * nAs + (b >> 1)) * 2) << 2) + ((a & 1) << 1) + (b & 1);
-}
-
 // packed G^< positions (common/linear_algebra.f90:142-191): Re part over a < b
 // row-wise, Im part over a <= b row-wise, after the 2 nA^2 G^r values
 __device__ __forceinline__ int su_index(int a, int b, int nA) { return a * (nA - 1) - (a * (a - 1)) / 2 + (b - a - 1); }
@@ -92,7 +87,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     const WLayout L = wlayout(lds_raw, n, nCL, RA, RB, TI, TJ);
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
-    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * 2 * nA2 : nullptr;
+    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
     double2 *Ms = A.mscr + (size_t)blockIdx.x * 3 * n * n;  // [c][r]: M column-major
     double2 *Gs = Ms + n * n;                                // [i][j]: G^r row-major
@@ -134,7 +129,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
             const int pa = A.pos[min(li + TI * a, n - 1)];
             double2 s2v[RB];
 #pragma unroll
-            for (int b = 0; b < RB; b++) s2v[b] = sigw[(pa >= 0 && pc[b] >= 0) ? sig_index(pa, pc[b], A.nAs) : 0];
+            for (int b = 0; b < RB; b++) s2v[b] = sigw[(pa >= 0 && pc[b] >= 0) ? sig_r_index(pa, pc[b], A.nAs) : 0];
 #pragma unroll
             for (int b = 0; b < RB; b++)
                 if (pa >= 0 && pc[b] >= 0) M[a][b] = csub(M[a][b], s2v[b]);
@@ -289,7 +284,9 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
                 const int gi = ok ? b1 * A.nS + b2 : 0;
                 gl[q] = A.gamL_s[gi];
                 gr[q] = A.gamR_s[gi];
-                sg[q] = sigw ? sigw[ok ? sig_index(b1, b2, A.nAs) + 4 : 0] : make_double2(0.0, 0.0);
+                bool flip = false;
+                const int si = ok ? sig_l_index(b1, b2, A.nAs, &flip) : 0;
+                sg[q] = sigw ? sig_l_value(sigw[si], flip) : make_double2(0.0, 0.0);
             }
         };
         fetch(0);

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "qs_hip.h"
+#include "qs_device.h"
 #include "qs_internal.h"
 
 namespace {
@@ -84,6 +85,7 @@ struct qs_handle {
     double *d_x = nullptr, *d_F = nullptr, *d_Fold = nullptr, *d_xold = nullptr;
     void *d_hist_dF = nullptr, *d_hist_U = nullptr;
     double2 *d_Gt = nullptr, *d_Z = nullptr;  // Gt doubles as the Sigma2 output
+    int ncolS = 0;                            // site-pair layout columns: 12 per unordered active site pair
     double2 *d_sig1 = nullptr, *d_gt0 = nullptr;
     double *d_partials = nullptr, *d_red = nullptr, *d_scratch = nullptr;
     double2 *d_mscr = nullptr;   // one-wavefront Dyson kernel scratch (mscr_energies energies)
@@ -320,31 +322,36 @@ int setup_device(qs_handle *h) {
         for (int i = 0; i < nA; i++)
             for (int j = i; j < nA; j++) tri[e++] = (i << 16) | j;
         if ((rc = upload(h, &h->d_tri, tri))) return rc;
-        std::vector<int> blk(2 * (size_t)2 * nA2);
-        for (int ia = 0; ia < h->nAs; ia++)
-            for (int ja = 0; ja < h->nAs; ja++)
-                for (int kind = 0; kind < 2; kind++)
-                    for (int sa = 0; sa < 2; sa++)
-                        for (int sb = 0; sb < 2; sb++) {
-                            const int c = ((ia * h->nAs + ja) * 2 + kind) * 4 + sa * 2 + sb;
-                            const int i = 2 * ia + sa, j = 2 * ja + sb;
-                            int re, im;
-                            if (kind == 0) {
-                                re = i * nA + j + 1;
-                                im = nA2 + i * nA + j;
-                            } else if (i < j) {
-                                re = 2 * nA2 + su_idx(i, j, nA) + 1;
-                                im = 2 * nA2 + nsu + u_idx(i, j, nA);
-                            } else if (i == j) {
-                                re = 0;
-                                im = 2 * nA2 + nsu + u_idx(i, i, nA);
-                            } else {
-                                re = -(2 * nA2 + su_idx(j, i, nA) + 1);
-                                im = 2 * nA2 + nsu + u_idx(j, i, nA);
-                            }
-                            blk[2 * (size_t)c] = re;
-                            blk[2 * (size_t)c + 1] = im;
-                        }
+        // source of every column of the site-pair layout in the packed vector x:
+        // re = 1 + index (negated for -Re), 0 for a zero real part; im = index
+        const int nAs = h->nAs;
+        h->ncolS = 12 * (nAs * (nAs + 1) / 2);
+        std::vector<int> blk(2 * (size_t)h->ncolS);
+        auto source = [&](int i, int j, bool lesser, int &re, int &im) {
+            if (!lesser) {
+                re = i * nA + j + 1;
+                im = nA2 + i * nA + j;
+            } else if (i < j) {
+                re = 2 * nA2 + su_idx(i, j, nA) + 1;
+                im = 2 * nA2 + nsu + u_idx(i, j, nA);
+            } else if (i == j) {
+                re = 0;
+                im = 2 * nA2 + nsu + u_idx(i, i, nA);
+            } else {
+                re = -(2 * nA2 + su_idx(j, i, nA) + 1);
+                im = 2 * nA2 + nsu + u_idx(j, i, nA);
+            }
+        };
+        for (int I = 0; I < nAs; I++)
+            for (int J = I; J < nAs; J++)
+                for (int slot = 0; slot < 12; slot++) {
+                    const int c = 12 * sig_pair(I, J, nAs) + slot, s1 = (slot >> 1) & 1, s2 = slot & 1;
+                    int re, im;
+                    if (slot < 8) source(2 * I + s1, 2 * J + s2, slot >= 4, re, im);
+                    else source(2 * J + s1, 2 * I + s2, false, re, im);
+                    blk[2 * (size_t)c] = re;
+                    blk[2 * (size_t)c + 1] = im;
+                }
         if ((rc = upload(h, &h->d_blk_src, blk))) return rc;
         std::vector<int> et(4 * (size_t)h->nAs), asite_v(h->nAs);
         std::vector<double> uact(h->nAs);
@@ -394,8 +401,8 @@ int setup_device(qs_handle *h) {
                 h->d_hist_U = b;
             }
         }
-        if ((rc = dalloc(h, &h->d_Gt, (size_t)nw * 2 * nA2))) return rc;
-        if ((rc = dalloc(h, &h->d_Z, (size_t)h->Nf * 2 * nA2))) return rc;
+        if ((rc = dalloc(h, &h->d_Gt, (size_t)nw * h->ncolS))) return rc;
+        if ((rc = dalloc(h, &h->d_Z, (size_t)h->Nf * h->ncolS))) return rc;
         if ((rc = dalloc(h, &h->d_gt0, (size_t)4 * h->nAs))) return rc;
     }
     HIPCHK(h, hipDeviceSynchronize());
@@ -413,6 +420,7 @@ DysonArgs dyson_args(qs_handle *h, int mode, bool use_sigma, double V) {
     a.Hemb = h->d_Hemb;
     a.sig1 = h->d_sig1;
     a.sig = (h->order == 2 && use_sigma) ? h->d_Gt : nullptr;
+    a.sig_ld = h->ncolS;
     a.gamL_s = h->d_gamL_s; a.gamR_s = h->d_gamR_s; a.gamL_cl = h->d_gamL_cl;
     a.act = h->d_act; a.cl = h->d_cl; a.pos = h->d_pos; a.suppos = h->d_suppos;
     a.tri_map = h->d_tri;
@@ -526,11 +534,12 @@ int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, S
                                           h->nAs, h->nw, pp, h->d_scratch, h->stream));
     prof_end(h, ST_EQT, t0);
     t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, nA, h->nw, h->stream));
+    HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, nA, h->ncolS, h->nw, h->stream));
     prof_end(h, ST_PERMUTE, t0);
     Sigma2Plan sp{};
     sp.nA = nA; sp.nAs = h->nAs; sp.nw = h->nw; sp.Nf = h->Nf; sp.N1 = h->N1; sp.N2 = h->N2;
     sp.log2N1 = h->log2N1; sp.log2N2 = h->log2N2; sp.delta = h->p.delta; sp.tw = h->d_tw; sp.Upair = h->d_Upair;
+    sp.ncol = h->ncolS;
     for (int part = 0; part < 3; part++) {
         t0 = prof_begin(h);
         HIPCHK(h, qs_launch_sigma2(sp, h->d_Gt, h->d_Z, h->d_Gt, 1 << part, h->stream));
@@ -1079,17 +1088,20 @@ int qs_get_sigma(const qs_handle *hc, double *SigR2, double *SigL2, double *Sig1
     }
     if ((SigR2 || SigL2) && h->order == 2) {
         if (!h->sigma_valid) return fail(h, QS_ERR_STATE, "no second-order self-energy computed yet");
-        std::vector<zd> blk((size_t)h->nw * 2 * nA2);
+        std::vector<zd> blk((size_t)h->nw * h->ncolS);
         HIPCHK(h, hipMemcpyAsync(blk.data(), h->d_Gt, blk.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         zd *r = (zd *)SigR2, *l = (zd *)SigL2;
         for (int iw = 0; iw < h->nw; iw++)
             for (int j = 0; j < nA; j++)
                 for (int i = 0; i < nA; i++) {
-                    const int c = (((i >> 1) * h->nAs + (j >> 1)) * 2) * 4 + (i & 1) * 2 + (j & 1);
-                    const size_t d = (size_t)iw * nA2 + (size_t)j * nA + i;
-                    if (r) r[d] = blk[(size_t)iw * 2 * nA2 + c];
-                    if (l) l[d] = blk[(size_t)iw * 2 * nA2 + c + 4];
+                    const size_t d = (size_t)iw * nA2 + (size_t)j * nA + i, b = (size_t)iw * h->ncolS;
+                    if (r) r[d] = blk[b + sig_r_index(i, j, h->nAs)];
+                    if (l) {
+                        bool flip;
+                        const zd v = blk[b + sig_l_index(i, j, h->nAs, &flip)];
+                        l[d] = flip ? -std::conj(v) : v;
+                    }
                 }
     }
     return QS_OK;
@@ -1268,7 +1280,8 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
         const int64_t nv = tmp.n_vec;
         b += 4 * nv * 8;
         if (p->nhist > 0) b += 2 * nv * (int64_t)p->nhist * (p->single_history ? 4 : 8);
-        b += (nw + Nf) * 2 * nA * nA * 16;
+        const int64_t nAs = nA / 2;
+        b += (nw + Nf) * 12 * (nAs * (nAs + 1) / 2) * 16;
     }
     if (n <= qs_dyson_wave_max_norb()) b += std::min<int64_t>(nw, kWaveChunk) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
     return b + ((int64_t)64 << 20);

@@ -18,7 +18,8 @@ struct DysonArgs {
     int mode, use_sigma, order;
     const double2 *Hemb;     // (norb, norb): -H + (i/2)(GammaL + GammaR)
     const double2 *sig1;     // (nsites, 2, 2) on-site first-order self-energy
-    const double2 *sig;      // [nw][2 nA^2] second-order self-energy, blocked layout
+    const double2 *sig;      // [nw][sig_ld] second-order self-energy, site-pair layout (qs_device.h)
+    int sig_ld;              // complex values per energy in sig: 12 per unordered active site pair
     const double2 *gamL_s, *gamR_s;  // (nS, nS)
     const double2 *gamL_cl;  // (nCL, nCL)
     const int *act, *cl, *pos, *suppos;  // suppos: orbital -> index in sup or -1
@@ -59,12 +60,14 @@ int qs_dyson_wave_max_norb();
 // Second-order self-energy stage (qs_sigma2.hip)
 struct Sigma2Plan {
     int nA, nAs, nw, Nf, N1, N2, log2N1, log2N2;
+    int ncol;                // complex columns per energy: 12 per unordered active site pair
     double delta;
     const double2 *tw;       // (Nf) e^{-2 pi i k / Nf}
     const double *Upair;     // (nAs, nAs) U_i * U_j * hbar^2 (delta/2pi)^2 / Nf
 };
-// x (packed, reference layout) -> Gt [nw][2 nA^2] blocked layout
-hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int nw, hipStream_t s);
+// x (packed, reference layout) -> Gt [nw][ncol] site-pair layout
+hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
+                                hipStream_t s);
 // parts: bit 0 forward pass over n1, bit 1 middle pass (row DFTs + 2x2 algebra),
 // bit 2 inverse pass over k1; the three in order make one Sigma2 evaluation.
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,

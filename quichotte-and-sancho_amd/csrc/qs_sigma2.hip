@@ -101,8 +101,7 @@ __device__ __forceinline__ void load_twl(double2 *twl, const double2 *tw, int L,
 // doubles) stays inside one XCD's L2, so each line of x is fetched once.
 constexpr int PU = 8;  // gathers in flight per thread
 __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x, double2 *__restrict__ Gt,
-                                                   const int2 *__restrict__ src, int nA, int nw) {
-    const int ncol = 2 * nA * nA;
+                                                   const int2 *__restrict__ src, int nA, int ncol, int nw) {
     const size_t nblk = 3 * (size_t)nA * nA;
     for (int iw = blockIdx.x; iw < nw; iw += gridDim.x) {
         const double *xb = x + (size_t)iw * nblk;
@@ -139,7 +138,7 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
     extern __shared__ __align__(16) char lds_raw[];
     const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
-    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y, half = CB * N1 / 2;
+    const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y, half = CB * N1 / 2;
     load_twl(twl, p.tw, N1, p.Nf);
     // rows n1 >= N1 / 2 are the zero padding: the inputs are one round of 8
     double2 v[8];
@@ -225,138 +224,148 @@ __device__ __forceinline__ void pair_sigma(const double2 *r_ij, const double2 *L
 }
 
 // Middle pass with one row of the (k1, -k1) pair held in registers: LDS holds
-// a single row of the 16 columns (16 padded sequences, 74 KB at N2 = 256) and the row-DFT twiddles, so two workgroups share a CU.  Thread t
+// a single row of the pair's 12 columns (12 padded sequences, 55 KB at
+// N2 = 256) and the row-DFT twiddles, so two workgroups share a CU.  Thread t
 // owns position t of row k1 and, for the spin algebra, the partner position
 // (-k1, N2-1-t); both outputs are computed by the same thread from the same
 // two inputs.
+//
+// The lesser block (J, I) is not among the columns: G^< is anti-Hermitian at
+// every energy, G^<_JI(w) = -G^<_IJ(w)^dagger, so after the forward transform
+// L_JI(l) = -L_IJ(-l)^dagger, which the thread has at the partner position.
+// Likewise only Sigma^<_IJ is stored.
+constexpr int PS = 12;  // columns per site pair
+
+__device__ __forceinline__ void neg_dag2(const double2 *M, double2 *A) {  // A = -M^dagger (2x2 row-major)
+    A[0] = make_double2(-M[0].x, M[0].y);
+    A[1] = make_double2(-M[2].x, M[2].y);
+    A[2] = make_double2(-M[1].x, M[1].y);
+    A[3] = make_double2(-M[3].x, M[3].y);
+}
+
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid(double2 *__restrict__ Z,
                                                                                           Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = 2 * p.nA * p.nA, ld = seq_stride(N2);
-    double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (16)][ld]
+    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = p.ncol, ld = seq_stride(N2);
+    double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (12)][ld]
     int pid = blockIdx.x, ia = 0;
     while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
     const int ja = ia + pid;
     const int k1 = blockIdx.y, k1p = (N1 - k1) & (N1 - 1);
-    const int cb_ij = (ia * nAs + ja) * CB, cb_ji = (ja * nAs + ia) * CB;
+    const int cb = PS * blockIdx.x;  // the pair's columns (blockIdx.x = sig_pair(ia, ja))
     const double c = p.Upair[ia * nAs + ja];
     if (c == 0.0) {
         const int nrow = (k1 == k1p) ? 1 : 2;
-        for (int t = threadIdx.x; t < 16 * nrow * N2; t += NT) {
-            const int s = t & 15, rest = t >> 4, n2 = rest % N2, rr = rest / N2;
+        for (int t = threadIdx.x; t < PS * nrow * N2; t += NT) {
+            const int s = t % PS, rest = t / PS, n2 = rest % N2, rr = rest / N2;
             const int row = rr ? k1p : k1;
-            Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = make_double2(0.0, 0.0);
+            Z[((size_t)row * N2 + n2) * ncol + cb + s] = make_double2(0.0, 0.0);
         }
         return;
     }
     const int t = threadIdx.x;
     load_twl(twl, p.tw, N2, p.Nf);
-    // A row is 16 N2 <= 16 NT values: one round of loads into registers, then
+    // A row is PS N2 <= PS NT values: one round of loads into registers, then
     // into LDS at bit-reversed positions.  Row k1p is fetched while row k1 is
     // transformed.
-    constexpr int QM = 16;
-    double2 pre[QM];
+    double2 pre[PS];
     auto fetch_row = [&](int row) {
 #pragma unroll
-        for (int q = 0; q < QM; q++) {
+        for (int q = 0; q < PS; q++) {
             const int e = threadIdx.x + q * NT;
             pre[q] = make_double2(0.0, 0.0);
-            if (e < 16 * N2) {
-                const int s = e & 15, n2 = e >> 4;
-                pre[q] = Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)];
-            }
+            if (e < PS * N2) pre[q] = Z[((size_t)row * N2 + e / PS) * ncol + cb + e % PS];
         }
     };
     auto put_row = [&]() {
 #pragma unroll
-        for (int q = 0; q < QM; q++) {
+        for (int q = 0; q < PS; q++) {
             const int e = threadIdx.x + q * NT;
-            if (e < 16 * N2) buf[(e & 15) * ld + px(bitrev(e >> 4, lg2))] = pre[q];
+            if (e < PS * N2) buf[(e % PS) * ld + px(bitrev(e / PS, lg2))] = pre[q];
         }
     };
     // twiddle e^{+2 pi i row n2 / Nf} between the inverse row DFT and the
-    // inverse pass over k1; position n2 = (tid >> 4) + (NT / 16) q
+    // inverse pass over k1
     auto store_row = [&](int row) {
-        double2 w[QM];
+        double2 w[PS];
 #pragma unroll
-        for (int q = 0; q < QM; q++) {
-            const int n2 = (threadIdx.x >> 4) + (NT / 16) * q;
+        for (int q = 0; q < PS; q++) {
+            const int n2 = (threadIdx.x + q * NT) / PS;
             w[q] = n2 < N2 ? cconj(p.tw[row * n2]) : make_double2(0.0, 0.0);
         }
 #pragma unroll
-        for (int q = 0; q < QM; q++) {
+        for (int q = 0; q < PS; q++) {
             const int e = threadIdx.x + q * NT;
-            if (e < 16 * N2) {
-                const int s = e & 15, n2 = e >> 4;
-                Z[((size_t)row * N2 + n2) * ncol + (s < 8 ? cb_ij : cb_ji) + (s & 7)] = cmul(buf[s * ld + px(n2)], w[q]);
+            if (e < PS * N2) {
+                const int s = e % PS, n2 = e / PS;
+                Z[((size_t)row * N2 + n2) * ncol + cb + s] = cmul(buf[s * ld + px(n2)], w[q]);
             }
         }
+    };
+    // outputs at position x from its values X (at +l) and the partner's Y (at -l):
+    // [Sigma^r_IJ, Sigma^<_IJ, Sigma^r_JI], src/self_energies.f90:175-198
+    auto pair_out = [&](const double2 *X, const double2 *Y, double2 *O) {
+        double2 LjiY[4], LjiX[4], scratch[4];
+        neg_dag2(X + 4, LjiY);  // L_JI at -l
+        neg_dag2(Y + 4, LjiX);  // L_JI at +l
+        pair_sigma(X + 0, X + 4, Y + 8, LjiY, c, O + 0, O + 4);
+        pair_sigma(X + 8, LjiX, Y + 0, Y + 4, c, O + 8, scratch);
     };
     fetch_row(k1);
     put_row();
     __syncthreads();
     if (k1 != k1p) fetch_row(k1p);
-    lds_fft(buf, 16, ld, lg2, -1, twl);
+    lds_fft(buf, PS, ld, lg2, -1, twl);
     if (k1 != k1p) {
-        double2 A[16];
+        double2 A[PS];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) A[s] = buf[s * ld + px(t)];
+            for (int s = 0; s < PS; s++) A[s] = buf[s * ld + px(t)];
         __syncthreads();
         put_row();
         __syncthreads();
-        lds_fft(buf, 16, ld, lg2, -1, twl);
+        lds_fft(buf, PS, ld, lg2, -1, twl);
         const int m2 = N2 - 1 - t;  // partner of (k1, t) is (k1p, N2 - 1 - t)
-        double2 Q[16];
+        double2 Q[PS];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) Q[s] = buf[s * ld + px(m2)];
+            for (int s = 0; s < PS; s++) Q[s] = buf[s * ld + px(m2)];
         __syncthreads();
         if (t < N2) {
-            // Two independent groups keep the live set at A, Q and 8 outputs:
-            // {A0-7, Q8-15} -> (k1, t) blocks 0-7 and (k1p, m2) blocks 8-15,
-            // {A8-15, Q0-7} -> (k1, t) blocks 8-15 and (k1p, m2) blocks 0-7.
+            double2 O[PS];
+            pair_out(Q, A, O);
             const int bm = px(bitrev(m2, lg2));
-            double2 O[8];
-            pair_sigma(Q + 8, Q + 12, A + 0, A + 4, c, O + 0, O + 4);
 #pragma unroll
-            for (int s = 0; s < 8; s++) buf[(8 + s) * ld + bm] = O[s];
-            pair_sigma(A + 0, A + 4, Q + 8, Q + 12, c, O + 0, O + 4);
+            for (int s = 0; s < PS; s++) buf[s * ld + bm] = O[s];
+            pair_out(A, Q, O);
 #pragma unroll
-            for (int s = 0; s < 8; s++) A[s] = O[s];
-            pair_sigma(Q + 0, Q + 4, A + 8, A + 12, c, O + 0, O + 4);
-#pragma unroll
-            for (int s = 0; s < 8; s++) buf[s * ld + bm] = O[s];
-            pair_sigma(A + 8, A + 12, Q + 0, Q + 4, c, O + 0, O + 4);
-#pragma unroll
-            for (int s = 0; s < 8; s++) A[8 + s] = O[s];
+            for (int s = 0; s < PS; s++) A[s] = O[s];
         }
         __syncthreads();
-        lds_fft(buf, 16, ld, lg2, +1, twl);
+        lds_fft(buf, PS, ld, lg2, +1, twl);
         store_row(k1p);
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * ld + px(bitrev(t, lg2))] = A[s];
+            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, lg2))] = A[s];
         __syncthreads();
     } else {
         // k1 = 0 or N1/2: the partner position is in the same row
         const int m2 = (k1 == 0) ? ((N2 - t) & (N2 - 1)) : (N2 - 1 - t);
-        double2 O[16];
+        double2 O[PS];
         if (t < N2) {
-            double2 P[16], Q[16];
+            double2 P[PS], Q[PS];
 #pragma unroll
-            for (int s = 0; s < 16; s++) { P[s] = buf[s * ld + px(t)]; Q[s] = buf[s * ld + px(m2)]; }
-            pair_sigma(P + 0, P + 4, Q + 8, Q + 12, c, O + 0, O + 4);
-            pair_sigma(P + 8, P + 12, Q + 0, Q + 4, c, O + 8, O + 12);
+            for (int s = 0; s < PS; s++) { P[s] = buf[s * ld + px(t)]; Q[s] = buf[s * ld + px(m2)]; }
+            pair_out(P, Q, O);
         }
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < 16; s++) buf[s * ld + px(bitrev(t, lg2))] = O[s];
+            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, lg2))] = O[s];
         __syncthreads();
     }
-    lds_fft(buf, 16, ld, lg2, +1, twl);
+    lds_fft(buf, PS, ld, lg2, +1, twl);
     store_row(k1);
 }
 
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
     extern __shared__ __align__(16) char lds_raw[];
     const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
-    const int ncol = 2 * p.nA * p.nA, cb = blockIdx.x, n2 = blockIdx.y;
+    const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y;
     load_twl(twl, p.tw, N1, p.Nf);
     double2 v[QS];
 #pragma unroll
@@ -393,17 +402,18 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 
 }  // namespace
 
-hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int nw, hipStream_t s) {
+hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
+                                hipStream_t s) {
     const int blocks = nw < 65536 ? nw : 65536;
-    hipLaunchKernelGGL(k_permute_in, dim3(blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, nw);
+    hipLaunchKernelGGL(k_permute_in, dim3(blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, ncol, nw);
     return hipGetLastError();
 }
 
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
                             hipStream_t s) {
-    const int ncol = 2 * p.nA * p.nA;
+    const int ncol = p.ncol;
     const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)CB * (p.N1 + p.N1 / 8 + 1)) * sizeof(double2);
-    const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)16 * (p.N2 + p.N2 / 8 + 1)) * sizeof(double2);
+    const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)PS * (p.N2 + p.N2 / 8 + 1)) * sizeof(double2);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
