@@ -98,7 +98,7 @@ def stage_counts(eng, nh):
     """Algorithmic work per launch of each stage (flops or HBM bytes)."""
     nw, norb, nA, nv, Nf = eng.prob.n_omega, eng.norb, eng.nA, eng.n_vec, eng.Nf
     nAs = nA // 2
-    ncol = 12.0 * (nAs * (nAs + 1) // 2)   # complex columns of the site-pair layout (12 per unordered pair)
+    ncol = float((12 * (nAs * (nAs + 1) // 2) + 7) // 8 * 8)   # site-pair layout columns (12 per pair, padded to 8)
     return {
         "dyson_keldysh": ("mfma", nw * (8.0 * norb ** 3 + 16.0 * nA ** 3), "flop"),
         # each pass reads its input once and writes its output once

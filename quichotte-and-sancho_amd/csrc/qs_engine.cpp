@@ -323,10 +323,12 @@ int setup_device(qs_handle *h) {
             for (int j = i; j < nA; j++) tri[e++] = (i << 16) | j;
         if ((rc = upload(h, &h->d_tri, tri))) return rc;
         // source of every column of the site-pair layout in the packed vector x:
-        // re = 1 + index (negated for -Re), 0 for a zero real part; im = index
+        // re = 1 + index (negated for -Re), 0 for a zero real part; im = index, -1 for zero
         const int nAs = h->nAs;
-        h->ncolS = 12 * (nAs * (nAs + 1) / 2);
-        std::vector<int> blk(2 * (size_t)h->ncolS);
+        // 12 columns per unordered pair, padded to the FFT column block of 8 (zero columns)
+        h->ncolS = (12 * (nAs * (nAs + 1) / 2) + 7) / 8 * 8;
+        std::vector<int> blk(2 * (size_t)h->ncolS, 0);
+        for (size_t c = 12 * (size_t)(nAs * (nAs + 1) / 2); c < (size_t)h->ncolS; c++) blk[2 * c + 1] = -1;
         auto source = [&](int i, int j, bool lesser, int &re, int &im) {
             if (!lesser) {
                 re = i * nA + j + 1;
@@ -1280,8 +1282,8 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
         const int64_t nv = tmp.n_vec;
         b += 4 * nv * 8;
         if (p->nhist > 0) b += 2 * nv * (int64_t)p->nhist * (p->single_history ? 4 : 8);
-        const int64_t nAs = nA / 2;
-        b += (nw + Nf) * 12 * (nAs * (nAs + 1) / 2) * 16;
+        const int64_t nAs = nA / 2, ncolS = (12 * (nAs * (nAs + 1) / 2) + 7) / 8 * 8;
+        b += (nw + Nf) * ncolS * 16;
     }
     if (n <= qs_dyson_wave_max_norb()) b += std::min<int64_t>(nw, kWaveChunk) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
     return b + ((int64_t)64 << 20);

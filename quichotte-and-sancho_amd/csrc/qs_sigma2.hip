@@ -114,9 +114,10 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
                 const int2 sc = src[c];
                 const int ir = sc.x > 0 ? sc.x - 1 : (sc.x < 0 ? -sc.x - 1 : 0);
                 re[u] = xb[ir];
-                im[u] = xb[sc.y];
+                im[u] = xb[sc.y >= 0 ? sc.y : 0];
                 if (sc.x < 0) re[u] = -re[u];
                 if (sc.x == 0) re[u] = 0.0;
+                if (sc.y < 0) im[u] = 0.0;  // padding column
             }
 #pragma unroll
             for (int u = 0; u < PU; u++) {
