@@ -98,11 +98,17 @@ __host__ __device__ inline int sig_r_index(int a, int b, int nAs) {
     return I <= J ? 12 * sig_pair(I, J, nAs) + ((a & 1) << 1) + (b & 1)
                   : 12 * sig_pair(J, I, nAs) + 8 + ((a & 1) << 1) + (b & 1);
 }
-// lesser element (a, b): stored index and whether it is the -conj of the stored value
-__host__ __device__ inline int sig_l_index(int a, int b, int nAs, bool *flip) {
+// lesser element (a, b): the stored index, and whether the element is -conj of it
+struct SigL {
+    int idx;
+    bool flip;
+};
+__host__ __device__ inline SigL sig_l_at(int a, int b, int nAs) {
     const int I = a >> 1, J = b >> 1;
-    *flip = I > J;
-    return I <= J ? 12 * sig_pair(I, J, nAs) + 4 + ((a & 1) << 1) + (b & 1)
-                  : 12 * sig_pair(J, I, nAs) + 4 + ((b & 1) << 1) + (a & 1);
+    SigL r;
+    r.flip = I > J;
+    r.idx = I <= J ? 12 * sig_pair(I, J, nAs) + 4 + ((a & 1) << 1) + (b & 1)
+                   : 12 * sig_pair(J, I, nAs) + 4 + ((b & 1) << 1) + (a & 1);
+    return r;
 }
 __device__ __forceinline__ double2 sig_l_value(double2 v, bool flip) { return flip ? make_double2(-v.x, v.y) : v; }

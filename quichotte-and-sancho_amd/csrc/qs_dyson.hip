@@ -317,9 +317,9 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
             const int gi = ok ? b1 * A.nS + b2 : 0;
             fgl[q] = A.gamL_s[gi];
             fgr[q] = A.gamR_s[gi];
-            bool flip = false;
-            const int si = ok ? sig_l_index(b1, b2, A.nAs, &flip) : 0;
-            fsg[q] = sigw ? sig_l_value(sigw[si], flip) : make_double2(0.0, 0.0);
+            const SigL sl = sig_l_at(b1, b2, A.nAs);
+            const int si = ok ? sl.idx : 0;
+            fsg[q] = sigw ? sig_l_value(sigw[si], ok && sl.flip) : make_double2(0.0, 0.0);
             if (ok) fok |= 1u << q;
         }
     };

@@ -9,7 +9,7 @@
 // A workgroup of NT = TI TJ threads (8 x 8: one wavefront, 16 x 16: four)
 // solves one energy.  Thread (li, lj) = (tid / TJ, tid % TJ) keeps the RA x RB
 // elements (li + TI a, lj + TJ b) of the matrix in registers (norb <= TI RA,
-// norb <= TJ RB).  The workgroup keeps only TI-row stages in LDS (37 KB at
+// norb <= TJ RB).  The workgroup keeps only TI-row stages in LDS (about 60 KB at
 // norb = 80 against 125 KB for k_dyson), so two energies share a CU and each
 // hides the other's barrier and memory latency; the full matrices of the
 // Keldysh products go through a per-energy scratch slab in HBM.
@@ -24,9 +24,11 @@
 //     column.
 //  3. M goes to the slab twice: column-major (Ms, operand of the Keldysh
 //     products) and as G^r in natural order (Gs, for packing).
-//  4. T = M S'' with S''[c][c'] = SL[p_c][p_c'] (outer products over c: the
-//     column of M from Ms, the row of S'' built in LDS), then T to the slab (Ts).
-//  5. P = T M^dagger (outer products over c, both columns from the slab);
+//  4. T = M S'' with S''[c][c'] = SL[p_c][p_c'] as outer products over c, TI
+//     values of c at a time: one round of loads brings TI columns of M (from
+//     Ms) and builds TI rows of S'' into two LDS chunks, then TI rank-1
+//     updates read them from LDS.  T goes to the slab (Ts).
+//  5. P = T M^dagger the same way (TI columns of T and of M per round);
 //     G^<[q_r][q_r'] = P[r][r'].
 //  6. Epilogues by mode from TI-row LDS stages of G^< and from Gs rows, with
 //     the loads of x and Fold batched ahead of the stores.
@@ -45,28 +47,34 @@ __device__ __forceinline__ int su_index(int a, int b, int nA) { return a * (nA -
 __device__ __forceinline__ int u_index(int a, int b, int nA) { return a * nA - (a * (a - 1)) / 2 + (b - a); }
 
 struct WLayout {
-    double2 *stage, *crow, *pcol, *srow, *grcl, *glcl;
+    double2 *stage, *crow, *pcol, *srow, *grcl, *glcl, *chunk;
     double *cv, *red;
     int *ci, *piv, *qinv, *spc, *actl, *posl;
 };
 
-// LDS: stage [TI][n] | crow [2][NW][TJ RB] | pcol [2][TI RA] | srow [2][TJ RB] |
-//      grcl, glcl [nCL^2] | cv [2 NW] | red [3 NW] | ci [2 NW] | 5 int maps [TI RA]
+// LDS: stage [SC] | crow [2][NW][TJ RB] | pcol [2][TI RA] | srow [2][TJ RB] |
+//      grcl, glcl [nCL^2] | chunk [SC] | cv [2 NW] | red [3 NW] | ci [2 NW] | 5 int maps [TI RA]
+// SC = TI TJ RA elements: one store per thread and load batch q < RA, which
+// covers TI rows of n (RA = ceil(n / TI), TJ >= TI)
+__host__ __device__ inline int wlayout_sc(int n, int RA, int TI, int TJ) {
+    return TI * TJ * RA > TI * n ? TI * TJ * RA : TI * n;
+}
 __host__ __device__ inline size_t wlayout_bytes(int n, int nCL, int RA, int RB, int TI, int TJ) {
-    const int NW = TI * TJ / 64;
-    return (size_t)16 * (TI * n + 2 * NW * TJ * RB + 2 * TI * RA + 2 * TJ * RB + 2 * nCL * nCL) +
+    const int NW = TI * TJ / 64, SC = wlayout_sc(n, RA, TI, TJ);
+    return (size_t)16 * (2 * SC + 2 * NW * TJ * RB + 2 * TI * RA + 2 * TJ * RB + 2 * nCL * nCL) +
            (size_t)8 * (2 * NW + 3 * NW) + (size_t)4 * (2 * NW + 5 * TI * RA);
 }
 __device__ __forceinline__ WLayout wlayout(char *base, int n, int nCL, int RA, int RB, int TI, int TJ) {
-    const int NW = TI * TJ / 64;
+    const int NW = TI * TJ / 64, SC = wlayout_sc(n, RA, TI, TJ);
     WLayout L;
     L.stage = (double2 *)base;
-    L.crow = L.stage + TI * n;
+    L.crow = L.stage + SC;
     L.pcol = L.crow + 2 * NW * TJ * RB;
     L.srow = L.pcol + 2 * TI * RA;
     L.grcl = L.srow + 2 * TJ * RB;
     L.glcl = L.grcl + nCL * nCL;
-    L.cv = (double *)(L.glcl + nCL * nCL);
+    L.chunk = L.glcl + nCL * nCL;
+    L.cv = (double *)(L.chunk + SC);
     L.red = L.cv + 2 * NW;
     L.ci = (int *)(L.red + 3 * NW);
     L.piv = L.ci + 2 * NW;
@@ -257,66 +265,61 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     __syncthreads();
 
     // ---- 4. T = M S'' -----------------------------------------------------------
-    // step c: T += M(:, c) S''(c, :), S''(c, c') = SL(spc[c], spc[c']) on the
+    // T = sum_c M(:, c) S''(c, :), S''(c, c') = SL(spc[c], spc[c']) on the
     // support: i (f_L GamL + f_R GamR) + Sigma2^<  (src/scf.f90:333-336).
-    // Row c of S'' is built by the workgroup into srow[c & 1]; the raw values
-    // and my rows of M(:, c) are fetched one step ahead.
-    constexpr int QS = (TJ * RB + NT - 1) / NT;
+    // TI steps at a time: columns c of M (from Ms) go to stage[kk][*] and rows
+    // c of S'' to chunk[kk][*], all loads of a chunk in one round, then TI
+    // rank-1 updates from LDS.
     double2 T[RA][RB];
 #pragma unroll
     for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < RB; b++) T[a][b] = make_double2(0.0, 0.0);
-    {
-        double2 mn[RA], gl[QS], gr[QS], sg[QS];
+    for (int k0 = 0; k0 < n; k0 += TI) {
+        const int kc = min(TI, n - k0);
+        double2 mv[RA], gl[RA], gr[RA], sg[RA];
         unsigned okm = 0;
-        auto fetch = [&](int c) {
 #pragma unroll
-            for (int a = 0; a < RA; a++) mn[a] = Ms[c * n + min(li + TI * a, n - 1)];
-            const int b1 = L.spc[c];
-            okm = 0;
+        for (int q = 0; q < RA; q++) {  // TI n <= NT RA elements per chunk
+            const int e = tid + NT * q, kk = e / n, r = e - kk * n;
+            const bool in = e < kc * n;
+            mv[q] = Ms[k0 * n + (in ? e : 0)];
+            const int b1 = in ? L.spc[k0 + kk] : -1, b2 = in ? L.spc[r] : -1;
+            const bool ok = b1 >= 0 && b2 >= 0;
+            if (ok) okm |= 1u << q;
+            const int gi = ok ? b1 * A.nS + b2 : 0;
+            gl[q] = A.gamL_s[gi];
+            gr[q] = A.gamR_s[gi];
+            const SigL sl = sig_l_at(b1, b2, A.nAs);
+            const int si = ok ? sl.idx : 0;
+            sg[q] = sigw ? sig_l_value(sigw[si], ok && sl.flip) : make_double2(0.0, 0.0);
+        }
+        // unconditional stores (e < SC); entries past kc n are never read
 #pragma unroll
-            for (int q = 0; q < QS; q++) {
-                const int cc = tid + NT * q;
-                const int b2 = cc < n ? L.spc[cc] : -1;
-                const bool ok = b1 >= 0 && b2 >= 0;
-                if (ok) okm |= 1u << q;
-                const int gi = ok ? b1 * A.nS + b2 : 0;
-                gl[q] = A.gamL_s[gi];
-                gr[q] = A.gamR_s[gi];
-                bool flip = false;
-                const int si = ok ? sig_l_index(b1, b2, A.nAs, &flip) : 0;
-                sg[q] = sigw ? sig_l_value(sigw[si], flip) : make_double2(0.0, 0.0);
+        for (int q = 0; q < RA; q++) {
+            const int e = tid + NT * q;
+            double2 v = make_double2(0.0, 0.0);
+            if (okm & (1u << q)) {
+                const double re = fL * gl[q].x + fR * gr[q].x, im = fL * gl[q].y + fR * gr[q].y;
+                v = make_double2(-im, re);
+                if (sigw) v = cadd(v, sg[q]);
             }
-        };
-        fetch(0);
-        for (int c = 0; c < n; c++) {
-            double2 *sr = L.srow + (c & 1) * TJ * RB;
+            L.stage[e].x = mv[q].x; L.stage[e].y = mv[q].y;
+            L.chunk[e].x = v.x; L.chunk[e].y = v.y;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; kk++) {
+            double2 ma[RA];
 #pragma unroll
-            for (int q = 0; q < QS; q++) {
-                const int cc = tid + NT * q;
-                if (cc < TJ * RB) {
-                    double2 v = make_double2(0.0, 0.0);
-                    if (okm & (1u << q)) {
-                        const double re = fL * gl[q].x + fR * gr[q].x, im = fL * gl[q].y + fR * gr[q].y;
-                        v = make_double2(-im, re);
-                        if (sigw) v = cadd(v, sg[q]);
-                    }
-                    sr[cc] = v;
-                }
-            }
-            double2 mc[RA];
-#pragma unroll
-            for (int a = 0; a < RA; a++) mc[a] = mn[a];
-            __syncthreads();
-            if (c + 1 < n) fetch(c + 1);
+            for (int a = 0; a < RA; a++) ma[a] = L.stage[kk * n + min(li + TI * a, n - 1)];
 #pragma unroll
             for (int b = 0; b < RB; b++) {
-                const double2 sv = sr[lj + TJ * b];
+                const double2 sv = L.chunk[kk * n + min(lj + TJ * b, n - 1)];
 #pragma unroll
-                for (int a = 0; a < RA; a++) cfma(T[a][b], mc[a], sv);
+                for (int a = 0; a < RA; a++) cfma(T[a][b], ma[a], sv);
             }
         }
+        __syncthreads();
     }
 #pragma unroll
     for (int a = 0; a < RA; a++)
@@ -329,34 +332,40 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     __syncthreads();
 
     // ---- 5. P = T M^dagger --------------------------------------------------------
-    // step c: P += T(:, c) conj(M(:, c))^T: my rows of T(:, c) from Ts, my
-    // columns (rows lj + TJ b of M) from Ms, fetched one step ahead.
+    // P = sum_c T(:, c) conj(M(:, c))^T, TI steps at a time: columns c of T
+    // (from Ts) to stage[kk][*] and of M (from Ms) to chunk[kk][*].
     double2 P[RA][RB];
 #pragma unroll
     for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < RB; b++) P[a][b] = make_double2(0.0, 0.0);
-    {
-        double2 tn[RA], mn[RB];
-        auto fetch = [&](int c) {
+    for (int k0 = 0; k0 < n; k0 += TI) {
+        const int kc = min(TI, n - k0);
+        double2 tv[RA], mv[RA];
 #pragma unroll
-            for (int a = 0; a < RA; a++) tn[a] = Ts[c * n + min(li + TI * a, n - 1)];
-#pragma unroll
-            for (int b = 0; b < RB; b++) mn[b] = Ms[c * n + min(lj + TJ * b, n - 1)];
-        };
-        fetch(0);
-        for (int c = 0; c < n; c++) {
-            double2 tc[RA], mc[RB];
-#pragma unroll
-            for (int a = 0; a < RA; a++) tc[a] = tn[a];
-#pragma unroll
-            for (int b = 0; b < RB; b++) mc[b] = mn[b];
-            if (c + 1 < n) fetch(c + 1);
-#pragma unroll
-            for (int a = 0; a < RA; a++)
-#pragma unroll
-                for (int b = 0; b < RB; b++) cfmac(P[a][b], tc[a], mc[b]);
+        for (int q = 0; q < RA; q++) {
+            const int e = tid + NT * q, ec = e < kc * n ? e : 0;
+            tv[q] = Ts[k0 * n + ec];
+            mv[q] = Ms[k0 * n + ec];
         }
+#pragma unroll
+        for (int q = 0; q < RA; q++) {
+            L.stage[tid + NT * q].x = tv[q].x; L.stage[tid + NT * q].y = tv[q].y;
+            L.chunk[tid + NT * q].x = mv[q].x; L.chunk[tid + NT * q].y = mv[q].y;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; kk++) {
+            double2 ta[RA];
+#pragma unroll
+            for (int a = 0; a < RA; a++) ta[a] = L.stage[kk * n + min(li + TI * a, n - 1)];
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const double2 mb = L.chunk[kk * n + min(lj + TJ * b, n - 1)];
+#pragma unroll
+                for (int a = 0; a < RA; a++) cfmac(P[a][b], ta[a], mb);
+            }
+        }
+        __syncthreads();
     }
 
     // ---- 6. epilogues ---------------------------------------------------------------

@@ -1100,9 +1100,9 @@ int qs_get_sigma(const qs_handle *hc, double *SigR2, double *SigL2, double *Sig1
                     const size_t d = (size_t)iw * nA2 + (size_t)j * nA + i, b = (size_t)iw * h->ncolS;
                     if (r) r[d] = blk[b + sig_r_index(i, j, h->nAs)];
                     if (l) {
-                        bool flip;
-                        const zd v = blk[b + sig_l_index(i, j, h->nAs, &flip)];
-                        l[d] = flip ? -std::conj(v) : v;
+                        const SigL sl = sig_l_at(i, j, h->nAs);
+                        const zd v = blk[b + sl.idx];
+                        l[d] = sl.flip ? -std::conj(v) : v;
                     }
                 }
     }

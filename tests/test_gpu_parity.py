@@ -22,6 +22,18 @@ def rel_max(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
 
 
+def worst_packed(a, b, nA):
+    """Where two packed vectors (3 nA^2 doubles per energy) differ most: energy,
+    part (Re G^r, Im G^r, G^<) and element, with the two values and how many
+    energies differ at all."""
+    a, b = np.asarray(a).reshape(-1, 3, nA * nA), np.asarray(b).reshape(-1, 3, nA * nA)
+    d = np.abs(a - b)
+    iw, part, e = np.unravel_index(int(d.argmax()), d.shape)
+    bad = int((d.reshape(d.shape[0], -1).max(axis=1) > 1e-11 * np.abs(b).max()).sum())
+    return (f"energy {iw} of {a.shape[0]}, part {('ReGr', 'ImGr', 'Gless')[part]}, element ({e // nA}, {e % nA}): "
+            f"{a[iw, part, e]!r} vs {b[iw, part, e]!r}; {bad} energies differ")
+
+
 @pytest.mark.parametrize("norb,nw", [(2, 64), (8, 200), (16, 300), (18, 128), (34, 96), (48, 64), (80, 40), (88, 24)])
 def test_init2_matches_oracle(norb, nw):
     prob = random_problem(norb, nw, seed=norb)
@@ -148,12 +160,13 @@ def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     g = gpu_engine(prob)
     assert g.dyson_choice()[0] == variant
     x = o.init2(0.1)
-    assert rel_max(g.init2(0.1), x) < 1e-11
+    xg = g.init2(0.1)
+    assert rel_max(xg, x) < 1e-11, worst_packed(xg, x, o.nA)
     F0, _ = o.residual2(0.1, x)
     x1 = x + 0.5 * F0
     Fo, _ = o.residual2(0.1, x1)
     Fg, _ = g.residual2(0.1, x1)
-    assert rel_max(Fg, Fo) < 1e-11
+    assert rel_max(Fg, Fo) < 1e-11, worst_packed(Fg, Fo, o.nA)
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2])
