@@ -528,7 +528,10 @@ size_t qs_dyson_lds_bytes(int n, int nCL) {
 int qs_dyson_max_norb() { return 88; }
 
 bool qs_dyson_shape_ok(int norb, int shape) {
-    if (shape == QS_DYSON_WG256) return true;
+    // k_dyson<16, 5, 5> (WG256 at 65..80 orbitals) is switched off: it returned
+    // wrong init2 values at 80 orbitals and a GPU memory fault followed, with no
+    // cause found in its source; the other shapes cover these sizes
+    if (shape == QS_DYSON_WG256) return (norb + 15) / 16 != 5;
     if (shape == QS_DYSON_WG512) return norb > 16;
     if (shape == QS_DYSON_WAVE) return norb <= qs_dyson_wave_max_norb();
     return false;
@@ -548,7 +551,6 @@ hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, h
             case 2: return launch_v<16, 2, 2>(a, count, s);
             case 3: return launch_v<16, 3, 3>(a, count, s);
             case 4: return launch_v<16, 4, 4>(a, count, s);
-            case 5: return launch_v<16, 5, 5>(a, count, s);
             case 6: return launch_v<16, 6, 6>(a, count, s);
             default: return hipErrorInvalidValue;
         }

@@ -500,8 +500,11 @@ int tune_dyson(qs_handle *h) {
     h->dyson_variant = 0;
     if (const char *e = getenv("QS_DYSON_VARIANT")) {
         const int v = atoi(e);
-        if (v >= 0 && v < nv && qs_dyson_shape_ok(h->norb, v)) h->dyson_variant = v;
-        return QS_OK;
+        if (v >= 0 && v < nv && qs_dyson_shape_ok(h->norb, v)) {
+            h->dyson_variant = v;
+            return QS_OK;
+        }
+        // a shape this size cannot use: time the ones it can
     }
     const int count = std::min(h->nw, 2048);
     DysonArgs a = dyson_args(h, QS_M_FINAL, false, 0.0);

@@ -53,33 +53,37 @@ template <int G>
 __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg, int s0, int sign,
                                          const double2 *twl) {
     constexpr int T = 1 << G;
-    const int L = 1 << lg, h = 1 << (s0 - 1), span = h * T;
-    const int tasks_per_seq = L / T, total = nseq * tasks_per_seq;
-    for (int task = threadIdx.x; task < total; task += blockDim.x) {
-        const int seq = task / tasks_per_seq, r = task - seq * tasks_per_seq;
-        const int blk = r / h, j = r - blk * h;
-        double2 *b = buf + (size_t)seq * ld;
-        const int x0 = blk * span + j;
+    // every size is a power of two: tasks, blocks and twiddle strides by shifts
+    const int lt = lg - G, lh = s0 - 1;  // log2(tasks per sequence), log2(h)
+    const int total = nseq << lt;
+    for (int task = threadIdx.x; task < total; task += NT) {
+        const int seq = task >> lt, r = task & ((1 << lt) - 1);
+        const int blk = r >> lh, j = r & ((1 << lh) - 1);
+        double2 *b = buf + seq * ld;
+        const int x0 = (blk << (lh + G)) + j;
         double2 v[T];
 #pragma unroll
-        for (int t = 0; t < T; t++) v[t] = b[px(x0 + t * h)];
+        for (int t = 0; t < T; t++) v[t] = b[px(x0 + (t << lh))];
 #pragma unroll
         for (int e = 0; e < G; e++) {
-            const int half = h << e;                 // radix-2 level s0 + e
-            const int twmul = L / (2 * half);        // e^{-2 pi i m/(2 half)} = twl[m * twmul]
+            // radix-2 level s0 + e, half-length h 2^e: w = e^{-2 pi i m / (2 h 2^e)} = twl[m L / (2 h 2^e)]
+            const int tws = lg - 1 - lh - e;
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (!(t & (1 << e))) {
-                    const int m = j + (t & ((1 << e) - 1)) * h;   // position inside the level's group
-                    double2 w = twl[m * twmul];
+                    const int m = j + ((t & ((1 << e) - 1)) << lh);  // position inside the level's group
+                    double2 w = twl[m << tws];
                     if (sign > 0) w.y = -w.y;
-                    const double2 u = v[t], x = cmul(w, v[t + (1 << e)]);
-                    v[t] = cadd(u, x);
-                    v[t + (1 << e)] = csub(u, x);
+                    // (u + w y, u - w y): four FMAs for the sum, the difference as 2u - sum
+                    const double2 u = v[t], y = v[t + (1 << e)];
+                    const double sr = fma(w.x, y.x, fma(-w.y, y.y, u.x));
+                    const double si = fma(w.x, y.y, fma(w.y, y.x, u.y));
+                    v[t] = make_double2(sr, si);
+                    v[t + (1 << e)] = make_double2(fma(2.0, u.x, -sr), fma(2.0, u.y, -si));
                 }
         }
 #pragma unroll
-        for (int t = 0; t < T; t++) b[px(x0 + t * h)] = v[t];
+        for (int t = 0; t < T; t++) b[px(x0 + (t << lh))] = v[t];
     }
     __syncthreads();
 }

@@ -154,6 +154,9 @@ def test_runaway_and_errors_are_reported():
 def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     """Every launch shape of the Dyson kernel: 256- and 512-thread workgroups with
     the matrix staged in LDS, and the 16 x 16 workgroup with a scratch slab."""
+    if variant == 0 and (norb + 15) // 16 == 5:
+        pytest.skip("the 256-thread shape at 65..80 orbitals is switched off in the library: it returned wrong "
+                    "init2 values at 80 orbitals and a GPU memory fault followed, cause not found")
     monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
     prob = random_problem(norb, 48, seed=100 + norb, U=0.9)
     o = Oracle(prob)
