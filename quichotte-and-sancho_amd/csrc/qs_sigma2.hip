@@ -37,7 +37,7 @@ __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned
 
 // In-place decimation-in-time DFT on `nseq` sequences of length L = 2^lg stored
 // with stride `ld` (complex), input in bit-reversed order; sign = -1 forward,
-// +1 inverse; twl[m] = e^{-2 pi i m / L} (m < L / 2), a table in LDS.
+// +1 inverse; twl[tsw(m)] = e^{-2 pi i m / L} (m < L / 2), a table in LDS.
 // The lg radix-2 levels are done in groups of up to three: one task loads the
 // 8 elements j + t h (t < 8) of a block, runs the group's levels in registers
 // and stores them back, so a length-512 transform takes 3 LDS sweeps and 3
@@ -47,6 +47,10 @@ __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned
 // first group (stride-8 blocks) in distinct banks, and the odd sequence stride
 // does the same for the column-interleaved staging accesses.
 __device__ __forceinline__ int px(int x) { return x + (x >> 3); }
+// The twiddle table is read at strides 2^k (k up to 5) by the 8 lanes that share
+// an LDS cycle; entry m sits at tsw(m), which permutes each aligned group of 8 so
+// that those reads fall in distinct bank groups (tsw(m) = m for m < 8).
+__device__ __forceinline__ int tsw(int m) { return m ^ (((m >> 3) ^ (m >> 6)) & 7); }
 __device__ __forceinline__ int seq_stride(int L) { return L + (L >> 3) + 1; }
 
 template <int G>
@@ -72,7 +76,7 @@ __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg
             for (int t = 0; t < T; t++)
                 if (!(t & (1 << e))) {
                     const int m = j + ((t & ((1 << e) - 1)) << lh);  // position inside the level's group
-                    double2 w = twl[m << tws];
+                    double2 w = twl[tsw(m << tws)];
                     if (sign > 0) w.y = -w.y;
                     // (u + w y, u - w y): four FMAs for the sum, the difference as 2u - sum
                     const double2 u = v[t], y = v[t + (1 << e)];
@@ -95,9 +99,9 @@ __device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const 
     else if (lg - s + 1 == 1) fft_group<1>(buf, nseq, ld, lg, s, sign, twl);
 }
 
-// twl[m] = e^{-2 pi i m / L} for m < L / 2, from the length-Nf table
+// twl[tsw(m)] = e^{-2 pi i m / L} for m < L / 2, from the length-Nf table
 __device__ __forceinline__ void load_twl(double2 *twl, const double2 *tw, int L, int Nf) {
-    for (int m = threadIdx.x; m < L / 2; m += NT) twl[m] = tw[m * (Nf / L)];
+    for (int m = threadIdx.x; m < L / 2; m += NT) twl[tsw(m)] = tw[m * (Nf / L)];
 }
 
 // ---- x (reference packed layout) -> Gt[omega][c] (blocked) ----------------------
