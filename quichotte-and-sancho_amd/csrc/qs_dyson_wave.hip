@@ -48,12 +48,13 @@ __device__ __forceinline__ int u_index(int a, int b, int nA) { return a * nA - (
 
 struct WLayout {
     double2 *stage, *crow, *pcol, *srow, *grcl, *glcl, *chunk;
-    double *cv, *red;
-    int *ci, *piv, *qinv, *spc, *actl, *posl;
+    double2 *cand;  // [2][NW] posted pivot candidates: (value, row as a double)
+    double *red;
+    int *piv, *qinv, *spc, *actl, *posl;
 };
 
 // LDS: stage [SC] | crow [2][NW][TJ RB] | pcol [2][TI RA] | srow [2][TJ RB] |
-//      grcl, glcl [nCL^2] | chunk [SC] | cv [2 NW] | red [3 NW] | ci [2 NW] | 5 int maps [TI RA]
+//      grcl, glcl [nCL^2] | chunk [SC] | cand [2 NW] | red [3 NW] | 5 int maps [TI RA]
 // SC = TI TJ RA elements: one store per thread and load batch q < RA, which
 // covers TI rows of n (RA = ceil(n / TI), TJ >= TI)
 __host__ __device__ inline int wlayout_sc(int n, int RA, int TI, int TJ) {
@@ -61,8 +62,8 @@ __host__ __device__ inline int wlayout_sc(int n, int RA, int TI, int TJ) {
 }
 __host__ __device__ inline size_t wlayout_bytes(int n, int nCL, int RA, int RB, int TI, int TJ) {
     const int NW = TI * TJ / 64, SC = wlayout_sc(n, RA, TI, TJ);
-    return (size_t)16 * (2 * SC + 2 * NW * TJ * RB + 2 * TI * RA + 2 * TJ * RB + 2 * nCL * nCL) +
-           (size_t)8 * (2 * NW + 3 * NW) + (size_t)4 * (2 * NW + 5 * TI * RA);
+    return (size_t)16 * (2 * SC + 2 * NW * TJ * RB + 2 * TI * RA + 2 * TJ * RB + 2 * nCL * nCL + 2 * NW) +
+           (size_t)8 * 3 * NW + (size_t)4 * 5 * TI * RA;
 }
 __device__ __forceinline__ WLayout wlayout(char *base, int n, int nCL, int RA, int RB, int TI, int TJ) {
     const int NW = TI * TJ / 64, SC = wlayout_sc(n, RA, TI, TJ);
@@ -74,10 +75,9 @@ __device__ __forceinline__ WLayout wlayout(char *base, int n, int nCL, int RA, i
     L.grcl = L.srow + 2 * TJ * RB;
     L.glcl = L.grcl + nCL * nCL;
     L.chunk = L.glcl + nCL * nCL;
-    L.cv = (double *)(L.chunk + SC);
-    L.red = L.cv + 2 * NW;
-    L.ci = (int *)(L.red + 3 * NW);
-    L.piv = L.ci + 2 * NW;
+    L.cand = L.chunk + SC;
+    L.red = (double *)(L.cand + 2 * NW);
+    L.piv = (int *)(L.red + 3 * NW);
     L.qinv = L.piv + TI * RA;
     L.spc = L.qinv + TI * RA;
     L.actl = L.spc + TI * RA;
@@ -160,20 +160,32 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     for (int k = -1; k < n; k++) {
         if (k >= 0) {
             const int buf = k & 1, kb = k / TJ, kl = k % TJ;
+            const double2 *pcl = L.pcol + buf * TI * RA;
+            // every LDS operand that does not depend on the pivot is read first, in
+            // one round: column k for my rows and the NW posted candidates
+            double2 fa[RA];
+#pragma unroll
+            for (int a = 0; a < RA; a++) fa[a] = pcl[li + TI * a];
+            double2 cnd[NW];
+#pragma unroll
+            for (int w2 = 0; w2 < NW; w2++) cnd[w2] = L.cand[buf * NW + w2];
+            // largest value, then smallest row; selects, no branches
             double bv = -1.0;
             int p = -1, pw = 0;
 #pragma unroll
             for (int w2 = 0; w2 < NW; w2++) {
-                const int i = L.ci[buf * NW + w2];
-                const double v = L.cv[buf * NW + w2];
-                if (i >= 0 && (p < 0 || v > bv || (v == bv && i < p))) { bv = v; p = i; pw = w2; }
+                const int i = (int)cnd[w2].y;
+                const double v = cnd[w2].x;
+                const bool take = i >= 0 && (p < 0 || v > bv || (v == bv && i < p));
+                bv = take ? v : bv;
+                p = take ? i : p;
+                pw = take ? w2 : pw;
             }
-            if (p < 0) p = 0;  // unreachable: n - k unused rows remain at step k
+            p = max(p, 0);  // p < 0 is unreachable: n - k unused rows remain at step k
             p = __builtin_amdgcn_readfirstlane(p);
             pw = __builtin_amdgcn_readfirstlane(pw);
             if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
             const double2 *prow = L.crow + (buf * NW + pw) * TJ * RB;
-            const double2 *pcl = L.pcol + buf * TI * RA;
             const int tp = p % TI, ap = p / TI;
             const double2 pv = pcl[p];
             if (tid == 0 && pv.x == 0.0 && pv.y == 0.0) atomicOr(A.status, 1);
@@ -191,11 +203,9 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
                     }
             }
 #pragma unroll
-            for (int a = 0; a < RA; a++) {
-                const double2 fa = pcl[li + TI * a];
+            for (int a = 0; a < RA; a++)
 #pragma unroll
-                for (int b = 0; b < RB; b++) cfms(M[a][b], fa, sp[b]);
-            }
+                for (int b = 0; b < RB; b++) cfms(M[a][b], fa[a], sp[b]);
             if (li == tp) {
 #pragma unroll
                 for (int a = 0; a < RA; a++)
@@ -233,8 +243,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
             }
             const int rw = __builtin_amdgcn_readlane(bi, tkk);
             if (lane == tkk) {
-                L.cv[buf * NW + wv] = best;
-                L.ci[buf * NW + wv] = bi;
+                L.cand[buf * NW + wv] = make_double2(best, (double)bi);
             }
             if (rw >= 0 && li == rw % TI) {
                 const int aw = rw / TI;
