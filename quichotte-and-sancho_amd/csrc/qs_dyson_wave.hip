@@ -41,6 +41,32 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 constexpr int EB = 8;   // G^r packing elements per thread in flight
 constexpr int EBL = 4;  // G^< packing elements per thread in flight (P is live then)
 
+// pivot candidates (value, row; row < 0: none): keep the larger value, then the
+// smaller row.  Commutative, so both lanes of a pair agree on the result.
+__device__ __forceinline__ void cand_merge(double &bv, int &bi, double v2, int i2) {
+    const bool take = i2 >= 0 && (bi < 0 || v2 > bv || (v2 == bv && i2 < bi));
+    bv = take ? v2 : bv;
+    bi = take ? i2 : bi;
+}
+// merge with lane l ^ O (O = 16 or 32) through the gfx950 lane-swap instructions:
+// permlane{16,32}_swap(x, x) returns x at lane l and at lane l ^ O, in an order
+// that depends on the lane, which the commutative merge does not care about
+template <int O>
+__device__ __forceinline__ void lane_merge(double &bv, int &bi) {
+    const unsigned lo = (unsigned)__double2loint(bv), hi = (unsigned)__double2hiint(bv), ix = (unsigned)bi;
+    const auto l2 = O == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                            : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h2 = O == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                            : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const auto i2 = O == 16 ? __builtin_amdgcn_permlane16_swap(ix, ix, false, false)
+                            : __builtin_amdgcn_permlane32_swap(ix, ix, false, false);
+    double v = __hiloint2double((int)h2[0], (int)l2[0]);
+    int i = (int)i2[0];
+    cand_merge(v, i, __hiloint2double((int)h2[1], (int)l2[1]), (int)i2[1]);
+    bv = v;
+    bi = i;
+}
+
 // packed G^< positions (common/linear_algebra.f90:142-191): Re part over a < b
 // row-wise, Im part over a <= b row-wise, after the 2 nA^2 G^r values
 __device__ __forceinline__ int su_index(int a, int b, int nA) { return a * (nA - 1) - (a * (a - 1)) / 2 + (b - a - 1); }
@@ -235,12 +261,13 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
                         }
             }
             // the wavefront's owners of column kk: lanes tkk + TJ j
-#pragma unroll
-            for (int o = TJ; o <= 32; o <<= 1) {
-                const double vo = __shfl_xor(best, o, 64);
-                const int io = __shfl_xor(bi, o, 64);
-                if (io >= 0 && (bi < 0 || vo > best || (vo == best && io < bi))) { best = vo; bi = io; }
+            if (TJ == 8) {
+                const double vo = __shfl_xor(best, 8, 64);
+                const int io = __shfl_xor(bi, 8, 64);
+                cand_merge(best, bi, vo, io);
             }
+            lane_merge<16>(best, bi);
+            lane_merge<32>(best, bi);
             const int rw = __builtin_amdgcn_readlane(bi, tkk);
             if (lane == tkk) {
                 L.cand[buf * NW + wv] = make_double2(best, (double)bi);
