@@ -257,11 +257,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     extern __shared__ __align__(16) char lds_raw[];
     const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = p.ncol, ld = seq_stride(N2);
     double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (12)][ld]
-    int pid = blockIdx.x, ia = 0;
+    // Work item w = (k1, pair), pair fastest.  Workgroup L of the dispatch runs on
+    // XCD L % 8, so the items are dealt out in 8 contiguous ranges: neighbouring
+    // pairs, whose 192-byte column groups share 128-byte lines, then meet in one
+    // L2.  The tail that does not fill 8 equal ranges keeps w = L.
+    const int npairs = gridDim.x, total = npairs * gridDim.y, L = blockIdx.y * npairs + blockIdx.x;
+    const int per = total / 8;
+    const int w = L < 8 * per ? (L % 8) * per + L / 8 : L;
+    const int pair = w % npairs;
+    int pid = pair, ia = 0;
     while (pid >= nAs - ia) { pid -= nAs - ia; ia++; }
     const int ja = ia + pid;
-    const int k1 = blockIdx.y, k1p = (N1 - k1) & (N1 - 1);
-    const int cb = PS * blockIdx.x;  // the pair's columns (blockIdx.x = sig_pair(ia, ja))
+    const int k1 = w / npairs, k1p = (N1 - k1) & (N1 - 1);
+    const int cb = PS * pair;  // the pair's columns (pair = sig_pair(ia, ja))
     const double c = p.Upair[ia * nAs + ja];
     if (c == 0.0) {
         const int nrow = (k1 == k1p) ? 1 : 2;
