@@ -35,49 +35,50 @@ constexpr int CB = 8;  // columns per site-pair block
 
 __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned)v) >> (32 - bits); }
 
-// In-place decimation-in-time DFT on `nseq` sequences of length L = 2^lg stored
-// with stride `ld` (complex), input in bit-reversed order; sign = -1 forward,
+// In-place decimation-in-time DFT on NSEQ sequences of length L = 2^LG stored
+// with stride ld(L) (complex), input in bit-reversed order; SIGN = -1 forward,
 // +1 inverse; twl[tsw(m)] = e^{-2 pi i m / L} (m < L / 2), a table in LDS.
-// The lg radix-2 levels are done in groups of up to three: one task loads the
+// The LG radix-2 levels are done in groups of up to three: one task loads the
 // 8 elements j + t h (t < 8) of a block, runs the group's levels in registers
 // and stores them back, so a length-512 transform takes 3 LDS sweeps and 3
-// barriers.  The arithmetic is exactly that of the radix-2 levels.
+// barriers.  The arithmetic is exactly that of the radix-2 levels.  Every size
+// is a compile-time constant, so the element addresses of a task are one base
+// plus immediate offsets.
 // Layout: element x of sequence q sits at q * ld + px(x), px(x) = x + x / 8,
 // ld = px(L) + 1.  The pad after every 8 elements puts the 8 lanes of the
 // first group (stride-8 blocks) in distinct banks, and the odd sequence stride
 // does the same for the column-interleaved staging accesses.
-__device__ __forceinline__ int px(int x) { return x + (x >> 3); }
+__host__ __device__ constexpr int px(int x) { return x + (x >> 3); }
 // The twiddle table is read at strides 2^k (k up to 5) by the 8 lanes that share
 // an LDS cycle; entry m sits at tsw(m), which permutes each aligned group of 8 so
 // that those reads fall in distinct bank groups (tsw(m) = m for m < 8).
 __device__ __forceinline__ int tsw(int m) { return m ^ (((m >> 3) ^ (m >> 6)) & 7); }
-__device__ __forceinline__ int seq_stride(int L) { return L + (L >> 3) + 1; }
+__host__ __device__ constexpr int seq_stride(int L) { return L + (L >> 3) + 1; }
 
-template <int G>
-__device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg, int s0, int sign,
-                                         const double2 *twl) {
-    constexpr int T = 1 << G;
-    // every size is a power of two: tasks, blocks and twiddle strides by shifts
-    const int lt = lg - G, lh = s0 - 1;  // log2(tasks per sequence), log2(h)
-    const int total = nseq << lt;
-    for (int task = threadIdx.x; task < total; task += NT) {
-        const int seq = task >> lt, r = task & ((1 << lt) - 1);
-        const int blk = r >> lh, j = r & ((1 << lh) - 1);
-        double2 *b = buf + seq * ld;
-        const int x0 = (blk << (lh + G)) + j;
+template <int G, int LG, int S0, int NSEQ, int SIGN>
+__device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
+    constexpr int T = 1 << G, LT = LG - G, LH = S0 - 1, H = 1 << LH;  // tasks per sequence 2^LT, element stride H
+    constexpr int TOTAL = NSEQ << LT, LD = seq_stride(1 << LG);
+    // px(x0 + t H) = px(x0) + t (H + H / 8): x0 is a multiple of 8 when H = 1
+    constexpr int STEP = H + (H >> 3);
+#pragma unroll 1
+    for (int task = threadIdx.x; task < TOTAL; task += NT) {
+        const int seq = task >> LT, r = task & ((1 << LT) - 1);
+        const int blk = r >> LH, j = r & (H - 1);
+        double2 *b = buf + seq * LD + px((blk << (LH + G)) + j);
         double2 v[T];
 #pragma unroll
-        for (int t = 0; t < T; t++) v[t] = b[px(x0 + (t << lh))];
+        for (int t = 0; t < T; t++) v[t] = b[t * STEP];
 #pragma unroll
         for (int e = 0; e < G; e++) {
-            // radix-2 level s0 + e, half-length h 2^e: w = e^{-2 pi i m / (2 h 2^e)} = twl[m L / (2 h 2^e)]
-            const int tws = lg - 1 - lh - e;
+            // radix-2 level S0 + e, half-length H 2^e: w = e^{-2 pi i m / (2 H 2^e)} = twl[m L / (2 H 2^e)]
+            const int tws = LG - 1 - LH - e;
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (!(t & (1 << e))) {
-                    const int m = j + ((t & ((1 << e) - 1)) << lh);  // position inside the level's group
+                    const int m = j + ((t & ((1 << e) - 1)) << LH);  // position inside the level's group
                     double2 w = twl[tsw(m << tws)];
-                    if (sign > 0) w.y = -w.y;
+                    if (SIGN > 0) w.y = -w.y;
                     // (u + w y, u - w y): four FMAs for the sum, the difference as 2u - sum
                     const double2 u = v[t], y = v[t + (1 << e)];
                     const double sr = fma(w.x, y.x, fma(-w.y, y.y, u.x));
@@ -87,20 +88,27 @@ __device__ __forceinline__ void fft_group(double2 *buf, int nseq, int ld, int lg
                 }
         }
 #pragma unroll
-        for (int t = 0; t < T; t++) b[px(x0 + (t << lh))] = v[t];
+        for (int t = 0; t < T; t++) b[t * STEP] = v[t];
     }
     __syncthreads();
 }
 
-__device__ void lds_fft(double2 *buf, int nseq, int ld, int lg, int sign, const double2 *twl) {
-    int s = 1;
-    while (lg - s + 1 >= 3) { fft_group<3>(buf, nseq, ld, lg, s, sign, twl); s += 3; }
-    if (lg - s + 1 == 2) fft_group<2>(buf, nseq, ld, lg, s, sign, twl);
-    else if (lg - s + 1 == 1) fft_group<1>(buf, nseq, ld, lg, s, sign, twl);
+template <int LG, int NSEQ, int SIGN, int S0 = 1>
+__device__ __forceinline__ void lds_fft(double2 *buf, const double2 *twl) {
+    constexpr int REM = LG - S0 + 1;
+    if constexpr (REM >= 3) {
+        fft_group<3, LG, S0, NSEQ, SIGN>(buf, twl);
+        lds_fft<LG, NSEQ, SIGN, S0 + 3>(buf, twl);
+    } else if constexpr (REM == 2) {
+        fft_group<2, LG, S0, NSEQ, SIGN>(buf, twl);
+    } else if constexpr (REM == 1) {
+        fft_group<1, LG, S0, NSEQ, SIGN>(buf, twl);
+    }
 }
 
 // twl[tsw(m)] = e^{-2 pi i m / L} for m < L / 2, from the length-Nf table
-__device__ __forceinline__ void load_twl(double2 *twl, const double2 *tw, int L, int Nf) {
+template <int L>
+__device__ __forceinline__ void load_twl(double2 *twl, const double2 *tw, int Nf) {
     for (int m = threadIdx.x; m < L / 2; m += NT) twl[tsw(m)] = tw[m * (Nf / L)];
 }
 
@@ -138,24 +146,25 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 
 // ---- F1 -----------------------------------------------------------------------
 // grid: (ncol / CB, N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
-// Thread work is unrolled over constant bounds (CB N1 <= 16 NT) so that every
-// global load of a round is in flight at once.
-constexpr int QS = 16;  // staging rounds: CB N1 / NT <= 16 (N1 <= 512)
-
+// Thread work is unrolled over constant bounds (QS = CB N1 / NT elements per
+// thread, QI of them nonzero inputs) so that every global load of a round is in
+// flight at once.
+template <int LG1>
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
+    constexpr int N1 = 1 << LG1, LD = seq_stride(N1), HALF = CB * N1 / 2;
+    constexpr int QS = (CB * N1 + NT - 1) / NT, QI = (HALF + NT - 1) / NT;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
-    const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y, half = CB * N1 / 2;
-    load_twl(twl, p.tw, N1, p.Nf);
-    // rows n1 >= N1 / 2 are the zero padding: the inputs are one round of 8
-    double2 v[8];
+    const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y;
+    load_twl<N1>(twl, p.tw, p.Nf);
+    // rows n1 >= N1 / 2 are the zero padding: the inputs are QI rounds
+    double2 v[QI];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < QI; q++) {
         const int t = threadIdx.x + q * NT, n = n2 + p.N2 * (t >> 3);
         v[q] = make_double2(0.0, 0.0);
-        if (t < half && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & 7)];
+        if (t < HALF && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & 7)];
     }
     // output twiddles e^{-2 pi i k1 n2 / Nf}, k1 = (tid >> 3) + (NT / 8) i
     double2 w[QS];
@@ -167,16 +176,16 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
 #pragma unroll
     for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
-        if (t < CB * N1) buf[(t & 7) * ld + px(bitrev(t >> 3, lg1))] = q < 8 ? v[q] : make_double2(0.0, 0.0);
+        if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = q < QI ? v[q] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft(buf, CB, ld, lg1, -1, twl);
+    lds_fft<LG1, CB, -1>(buf, twl);
 #pragma unroll
     for (int i = 0; i < QS; i++) {
         const int t = threadIdx.x + i * NT;
         if (t < CB * N1) {
             const int col = t & 7, k1 = t >> 3;
-            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * ld + px(k1)], w[i]);
+            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * LD + px(k1)], w[i]);
         }
     }
 }
@@ -252,10 +261,12 @@ __device__ __forceinline__ void neg_dag2(const double2 *M, double2 *A) {  // A =
     A[3] = make_double2(-M[3].x, M[3].y);
 }
 
+template <int LG2>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid(double2 *__restrict__ Z,
                                                                                           Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, N2 = p.N2, lg2 = p.log2N2, nAs = p.nAs, ncol = p.ncol, ld = seq_stride(N2);
+    constexpr int N2 = 1 << LG2, ld = seq_stride(N2);  // N2 <= NT
+    const int N1 = p.N1, nAs = p.nAs, ncol = p.ncol;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (12)][ld]
     // Work item w = (k1, pair), pair fastest.  Workgroup L of the dispatch runs on
     // XCD L % 8, so the items are dealt out in 8 contiguous ranges: neighbouring
@@ -281,7 +292,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         return;
     }
     const int t = threadIdx.x;
-    load_twl(twl, p.tw, N2, p.Nf);
+    load_twl<N2>(twl, p.tw, p.Nf);
     // A row is PS N2 <= PS NT values: one round of loads into registers, then
     // into LDS at bit-reversed positions.  Row k1p is fetched while row k1 is
     // transformed.
@@ -298,7 +309,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 #pragma unroll
         for (int q = 0; q < PS; q++) {
             const int e = threadIdx.x + q * NT;
-            if (e < PS * N2) buf[(e % PS) * ld + px(bitrev(e / PS, lg2))] = pre[q];
+            if (e < PS * N2) buf[(e % PS) * ld + px(bitrev(e / PS, LG2))] = pre[q];
         }
     };
     // twiddle e^{+2 pi i row n2 / Nf} between the inverse row DFT and the
@@ -332,7 +343,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     put_row();
     __syncthreads();
     if (k1 != k1p) fetch_row(k1p);
-    lds_fft(buf, PS, ld, lg2, -1, twl);
+    lds_fft<LG2, PS, -1>(buf, twl);
     if (k1 != k1p) {
         double2 A[PS];
         if (t < N2)
@@ -341,7 +352,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         __syncthreads();
         put_row();
         __syncthreads();
-        lds_fft(buf, PS, ld, lg2, -1, twl);
+        lds_fft<LG2, PS, -1>(buf, twl);
         const int m2 = N2 - 1 - t;  // partner of (k1, t) is (k1p, N2 - 1 - t)
         double2 Q[PS];
         if (t < N2)
@@ -351,7 +362,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         if (t < N2) {
             double2 O[PS];
             pair_out(Q, A, O);
-            const int bm = px(bitrev(m2, lg2));
+            const int bm = px(bitrev(m2, LG2));
 #pragma unroll
             for (int s = 0; s < PS; s++) buf[s * ld + bm] = O[s];
             pair_out(A, Q, O);
@@ -359,12 +370,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             for (int s = 0; s < PS; s++) A[s] = O[s];
         }
         __syncthreads();
-        lds_fft(buf, PS, ld, lg2, +1, twl);
+        lds_fft<LG2, PS, +1>(buf, twl);
         store_row(k1p);
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, lg2))] = A[s];
+            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, LG2))] = A[s];
         __syncthreads();
     } else {
         // k1 = 0 or N1/2: the partner position is in the same row
@@ -379,22 +390,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, lg2))] = O[s];
+            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, LG2))] = O[s];
         __syncthreads();
     }
-    lds_fft(buf, PS, ld, lg2, +1, twl);
+    lds_fft<LG2, PS, +1>(buf, twl);
     store_row(k1);
 }
 
 // ---- I2 -----------------------------------------------------------------------
 // grid: (ncol / CB, N2).  LDS as in F1.  Only the outputs n < nw are kept.
+template <int LG1>
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    const int N1 = p.N1, ld = seq_stride(N1), lg1 = p.log2N1;
+    constexpr int N1 = 1 << LG1, LD = seq_stride(N1), QS = (CB * N1 + NT - 1) / NT;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y;
-    load_twl(twl, p.tw, N1, p.Nf);
+    load_twl<N1>(twl, p.tw, p.Nf);
     double2 v[QS];
 #pragma unroll
     for (int q = 0; q < QS; q++) {
@@ -405,15 +417,15 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 #pragma unroll
     for (int q = 0; q < QS; q++) {
         const int t = threadIdx.x + q * NT;
-        if (t < CB * N1) buf[(t & 7) * ld + px(bitrev(t >> 3, lg1))] = v[q];
+        if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = v[q];
     }
     __syncthreads();
-    lds_fft(buf, CB, ld, lg1, +1, twl);
+    lds_fft<LG1, CB, +1>(buf, twl);
 #pragma unroll
     for (int i = 0; i < QS; i++) {
         const int t = threadIdx.x + i * NT;
         const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
-        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * ld + px(n1)];
+        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * LD + px(n1)];
     }
 }
 
@@ -426,32 +438,83 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
     return hipGetLastError();
 }
 
-hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
-                            hipStream_t s) {
-    const int ncol = p.ncol;
-    const size_t lds1 = ((size_t)p.N1 / 2 + (size_t)CB * (p.N1 + p.N1 / 8 + 1)) * sizeof(double2);
-    const size_t ldsm = ((size_t)p.N2 / 2 + (size_t)PS * (p.N2 + p.N2 / 8 + 1)) * sizeof(double2);
+namespace {
+
+template <int LG1>
+hipError_t launch_f1(const Sigma2Plan &p, const double2 *Gt, double2 *Z, hipStream_t s) {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_fft_f1, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_i2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)k_fft_mid, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_fft_f1<LG1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    const dim3 g1(ncol / CB, p.N2);
+    const size_t lds = ((size_t)(1 << LG1) / 2 + (size_t)CB * seq_stride(1 << LG1)) * sizeof(double2);
+    hipLaunchKernelGGL(k_fft_f1<LG1>, dim3(p.ncol / CB, p.N2), dim3(NT), lds, s, Gt, Z, p);
+    return hipGetLastError();
+}
+template <int LG1>
+hipError_t launch_i2(const Sigma2Plan &p, const double2 *Z, double2 *Sig, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft_i2<LG1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    const size_t lds = ((size_t)(1 << LG1) / 2 + (size_t)CB * seq_stride(1 << LG1)) * sizeof(double2);
+    hipLaunchKernelGGL(k_fft_i2<LG1>, dim3(p.ncol / CB, p.N2), dim3(NT), lds, s, Z, Sig, p);
+    return hipGetLastError();
+}
+template <int LG2>
+hipError_t launch_mid(const Sigma2Plan &p, double2 *Z, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_fft_mid<LG2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    const size_t lds = ((size_t)(1 << LG2) / 2 + (size_t)PS * seq_stride(1 << LG2)) * sizeof(double2);
+    const int npairs = p.nAs * (p.nAs + 1) / 2;
+    hipLaunchKernelGGL(k_fft_mid<LG2>, dim3(npairs, p.N1 / 2 + 1), dim3(NT), lds, s, Z, p);
+    return hipGetLastError();
+}
+
+
+}  // namespace
+
+hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
+                            hipStream_t s) {
+    if (p.log2N1 < 1 || p.log2N1 > 10 || p.log2N2 < 1 || p.log2N2 > 8 || p.ncol % CB) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
+#define QS_LG1_CASES(F, ...)                                                                            \
+    switch (p.log2N1) {                                                                                 \
+        case 1: e = F<1>(__VA_ARGS__); break;                                                           \
+        case 2: e = F<2>(__VA_ARGS__); break;                                                           \
+        case 3: e = F<3>(__VA_ARGS__); break;                                                           \
+        case 4: e = F<4>(__VA_ARGS__); break;                                                           \
+        case 5: e = F<5>(__VA_ARGS__); break;                                                           \
+        case 6: e = F<6>(__VA_ARGS__); break;                                                           \
+        case 7: e = F<7>(__VA_ARGS__); break;                                                           \
+        case 8: e = F<8>(__VA_ARGS__); break;                                                           \
+        case 9: e = F<9>(__VA_ARGS__); break;                                                           \
+        default: e = F<10>(__VA_ARGS__); break;                                                         \
+    }
     if (parts & 1) {
-        hipLaunchKernelGGL(k_fft_f1, g1, dim3(NT), lds1, s, Gt, Z, p);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        QS_LG1_CASES(launch_f1, p, Gt, Z, s)
+        if (e != hipSuccess) return e;
     }
     if (parts & 2) {
-        const int npairs = p.nAs * (p.nAs + 1) / 2;
-        hipLaunchKernelGGL(k_fft_mid, dim3(npairs, p.N1 / 2 + 1), dim3(NT), ldsm, s, Z, p);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        switch (p.log2N2) {
+            case 1: e = launch_mid<1>(p, Z, s); break;
+            case 2: e = launch_mid<2>(p, Z, s); break;
+            case 3: e = launch_mid<3>(p, Z, s); break;
+            case 4: e = launch_mid<4>(p, Z, s); break;
+            case 5: e = launch_mid<5>(p, Z, s); break;
+            case 6: e = launch_mid<6>(p, Z, s); break;
+            case 7: e = launch_mid<7>(p, Z, s); break;
+            default: e = launch_mid<8>(p, Z, s); break;
+        }
+        if (e != hipSuccess) return e;
     }
     if (parts & 4) {
-        hipLaunchKernelGGL(k_fft_i2, g1, dim3(NT), lds1, s, Z, Sig, p);
-        e = hipGetLastError();
+        QS_LG1_CASES(launch_i2, p, Z, Sig, s)
     }
+#undef QS_LG1_CASES
     return e;
 }
