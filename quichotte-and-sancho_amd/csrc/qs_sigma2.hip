@@ -96,14 +96,9 @@ __device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
 template <int LG, int NSEQ, int SIGN, int S0 = 1>
 __device__ __forceinline__ void lds_fft(double2 *buf, const double2 *twl) {
     constexpr int REM = LG - S0 + 1;
-    if constexpr (REM >= 3) {
-        fft_group<3, LG, S0, NSEQ, SIGN>(buf, twl);
-        lds_fft<LG, NSEQ, SIGN, S0 + 3>(buf, twl);
-    } else if constexpr (REM == 2) {
-        fft_group<2, LG, S0, NSEQ, SIGN>(buf, twl);
-    } else if constexpr (REM == 1) {
-        fft_group<1, LG, S0, NSEQ, SIGN>(buf, twl);
-    }
+    constexpr int G = REM >= 3 ? 3 : REM;
+    if constexpr (REM >= 1) fft_group<G, LG, S0, NSEQ, SIGN>(buf, twl);
+    if constexpr (REM > 3) lds_fft<LG, NSEQ, SIGN, S0 + 3>(buf, twl);
 }
 
 // twl[tsw(m)] = e^{-2 pi i m / L} for m < L / 2, from the length-Nf table

@@ -67,6 +67,22 @@ def test_residual2_and_sigma_match_oracle(norb, nw):
     assert np.abs(s1g - s1o).max() <= 1e-12 * max(np.abs(s1o).max(), 1.0)
 
 
+@pytest.mark.parametrize("nw", [65537, 70000])
+def test_sigma2_longest_split(nw):
+    """More than 2^16 energies: Nf = 2^18 = 1024 x 256, the longest pass over n1."""
+    prob = random_problem(2, nw, seed=21, U=1.1)
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    x = o.init2(0.3)
+    Fo, _ = o.residual2(0.3, x)
+    Fg, _ = g.residual2(0.3, x)
+    assert rel_max(Fg, Fo) < 1e-11, worst_packed(Fg, Fo, o.nA)
+    sro, slo = o.sigma2()
+    srg, slg, _ = g.sigma()
+    assert rel_max(srg, sro) < 1e-11
+    assert rel_max(slg, slo) < 1e-11
+
+
 def test_sigma2_nonpower_of_two_grid_and_noninteracting_site():
     """7-smooth oracle length vs power-of-two device length, U = 0 on a contact site."""
     prob = random_problem(8, 361, seed=5)
