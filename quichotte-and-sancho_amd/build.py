@@ -15,7 +15,9 @@ SO = os.path.join(LIB, "libqs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_dyson_wave.hip", "qs_sigma2.hip", "qs_vector.hip"]
+SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_dyson_wave.hip", "qs_dyson_big.hip", "qs_sigma2.hip", "qs_vector.hip"]
+# the large-Norb Dyson path calls rocSOLVER (batched zgetrf / zgetri) and rocBLAS (batched zgemm)
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def build(verbose=False):
@@ -23,7 +25,7 @@ def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = SO + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-shared", "-std=c++17", *DEVICE_FLAGS,
-           "-I" + os.path.join(REPO, "include"), "-I" + CSRC, *srcs, "-o", tmp]
+           "-I" + os.path.join(REPO, "include"), "-I" + CSRC, *srcs, *LIBS, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -531,9 +531,10 @@ bool qs_dyson_shape_ok(int norb, int shape) {
     // k_dyson<16, 5, 5> (WG256 at 65..80 orbitals) is switched off: it returned
     // wrong init2 values at 80 orbitals and a GPU memory fault followed, with no
     // cause found in its source; the other shapes cover these sizes
-    if (shape == QS_DYSON_WG256) return (norb + 15) / 16 != 5;
-    if (shape == QS_DYSON_WG512) return norb > 16;
+    if (shape == QS_DYSON_WG256) return norb <= qs_dyson_max_norb() && (norb + 15) / 16 != 5;
+    if (shape == QS_DYSON_WG512) return norb > 16 && norb <= qs_dyson_max_norb();
     if (shape == QS_DYSON_WAVE) return norb <= qs_dyson_wave_max_norb();
+    if (shape == QS_DYSON_BIG) return norb <= qs_dyson_big_max_norb();
     return false;
 }
 
@@ -543,6 +544,7 @@ hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, h
     if (count <= 0) return hipSuccess;
     if (!qs_dyson_shape_ok(a.norb, variant)) return hipErrorInvalidValue;
     if (variant == QS_DYSON_WAVE) return qs_launch_dyson_wave(a, count, s);
+    if (variant == QS_DYSON_BIG) return qs_launch_dyson_big(a, count, s);
     if (qs_dyson_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
     const int RB = (a.norb + 15) / 16;
     if (variant == QS_DYSON_WG256) {

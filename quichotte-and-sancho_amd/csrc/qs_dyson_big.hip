@@ -1,0 +1,269 @@
+// qs_dyson_big.hip -- per-energy Dyson and Keldysh solve for systems whose matrix
+// does not fit one workgroup's registers (Norb above the small-LDS kernel's 96).
+//
+// Same contract as k_dyson / k_dyson_w (src/scf.f90:305-415):
+//   G^r = [ w + Hemb - Sigma1 - Sigma2^r(w) ]^-1
+//   G^< = G^r(:,sup) [ i(f_L GamL + f_R GamR) + Sigma2^<(w) ] G^r(:,sup)^dagger
+// with every epilogue mode.  The energies go in chunks through HBM:
+//   k_big_build   A (column-major) and S'' = SL on the support, zero elsewhere
+//   zgetrf + zgetri  rocSOLVER, batched: the LAPACK pair the reference calls
+//                 (common/linear_algebra.f90:58-88), partial pivoting on |Re| + |Im|
+//   zgemm x 2     rocBLAS, batched: T = G^r S'', G^< = T G^r^dagger
+//   k_big_pack    the packing epilogues of the mode (as the other two kernels)
+// All matrices of a chunk are column-major n x n with leading dimension n.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "qs_device.h"
+#include "qs_internal.h"
+
+struct QsBigWork {
+    rocblas_handle rb = nullptr;
+    double2 *A = nullptr, *S = nullptr, *T = nullptr;  // [chunk][n][n] each
+    rocblas_int *ipiv = nullptr, *info = nullptr;      // [chunk][n], [chunk]
+    int n = 0, chunk = 0;
+};
+
+namespace {
+
+constexpr int NT = 256;
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+
+// A(i, j) = w + Hemb - Sigma1 - Sigma2^r and S(i, j) = i (f_L GamL + f_R GamR) + Sigma2^<
+// on the support (zero elsewhere), for energy iwc + blockIdx.x
+__global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, double2 *__restrict__ Ab,
+                                                  double2 *__restrict__ Sb) {
+    const int n = A.norb, iw = iwc + blockIdx.x;
+    const double w = A.w_init + (double)(iw + 1) * A.delta;
+    const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
+    const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
+    const bool use_s1 = A.use_sigma && A.sig1;
+    double2 *a = Ab + (size_t)blockIdx.x * n * n, *s = Sb + (size_t)blockIdx.x * n * n;
+    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+        const int j = idx / n, i = idx - j * n;  // column-major (i, j)
+        double2 v = A.Hemb[(size_t)i * n + j];
+        if (i == j) v.x += w;
+        if (use_s1 && (i >> 1) == (j >> 1)) v = csub(v, A.sig1[(i >> 1) * 4 + (i & 1) * 2 + (j & 1)]);
+        const int pi = A.pos[i], pj = A.pos[j];
+        if (sigw && pi >= 0 && pj >= 0) v = csub(v, sigw[sig_r_index(pi, pj, A.nAs)]);
+        a[idx] = v;
+        const int si = A.suppos[i], sj = A.suppos[j];
+        double2 sv = make_double2(0.0, 0.0);
+        if (si >= 0 && sj >= 0) {
+            const double2 gl = A.gamL_s[si * A.nS + sj], gr = A.gamR_s[si * A.nS + sj];
+            const double re = fL * gl.x + fR * gr.x, im = fL * gl.y + fR * gr.y;
+            sv = make_double2(-im, re);  // (src/scf.f90:333-336)
+            if (sigw) {
+                const SigL sl = sig_l_at(si, sj, A.nAs);
+                sv = cadd(sv, sig_l_value(sigw[sl.idx], sl.flip));
+            }
+        }
+        s[idx] = sv;
+    }
+}
+
+__device__ __forceinline__ double block_max_d(double v, double *red) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int k = 1; k < NT / 64; k++) m = fmax(m, red[k]);
+    return m;
+}
+__device__ __forceinline__ double block_sum_d(double v, double *red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double t = red[0];
+    for (int k = 1; k < NT / 64; k++) t += red[k];
+    return t;
+}
+
+// the epilogue of the mode for energy iwc + blockIdx.x from G^r and G^< (column-major)
+__global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, const double2 *__restrict__ Gr,
+                                                 const double2 *__restrict__ Gl, const rocblas_int *__restrict__ info) {
+    __shared__ double red[NT / 64];
+    const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL, iw = iwc + blockIdx.x, tid = threadIdx.x;
+    const double2 *gr = Gr + (size_t)blockIdx.x * n * n, *gl = Gl + (size_t)blockIdx.x * n * n;
+    auto GR = [&](int i, int j) { return gr[(size_t)j * n + i]; };
+    auto GL = [&](int i, int j) { return gl[(size_t)j * n + i]; };
+    if (tid == 0 && info[blockIdx.x] != 0) atomicOr(A.status, 1);
+    const double w = A.w_init + (double)(iw + 1) * A.delta;
+    const double fL = qs_fermi(w, A.V, A.mu, A.beta);
+    if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
+        const bool scf2 = A.mode == QS_M_SCF2;
+        const size_t off = (size_t)iw * 3 * nA2;
+        const int nsu = (nA2 - nA) / 2;
+        double st_max = 0.0, st_s2 = 0.0, st_d2 = 0.0;
+        // packed layout (common/linear_algebra.f90:142-191): Re G^r, Im G^r row-major,
+        // then Re G^<(i < j), Im G^<(i <= j) row-wise (tri_map)
+        for (int e = tid; e < nA2; e += NT) {
+            const int ar = e / nA, bb = e - ar * nA;
+            const double2 g = GR(A.act[ar], A.act[bb]);
+            if (!scf2) {
+                A.xout[off + e] = g.x;
+                A.xout[off + nA2 + e] = g.y;
+            } else {
+                const double f0 = g.x - A.x[off + e], f1 = g.y - A.x[off + nA2 + e];
+                A.F[off + e] = f0;
+                A.F[off + nA2 + e] = f1;
+                st_max = fmax(st_max, fmax(fabs(f0), fabs(f1)));
+                st_s2 += f0 * f0 + f1 * f1;
+                const double e0 = f0 - A.Fold[off + e], e1 = f1 - A.Fold[off + nA2 + e];
+                st_d2 += e0 * e0 + e1 * e1;
+                if (ar == bb) A.diagR[(size_t)iw * nA + ar] = g.y;
+            }
+        }
+        for (int e = tid; e < nA2; e += NT) {
+            const int code = A.tri_map[e], i = code >> 16, j = code & 0xffff;
+            const double2 g = GL(A.act[i], A.act[j]);
+            const double val = e < nsu ? g.x : g.y;
+            const size_t d = off + 2 * (size_t)nA2 + e;
+            if (!scf2) {
+                A.xout[d] = val;
+            } else {
+                const double f = val - A.x[d];
+                A.F[d] = f;
+                st_max = fmax(st_max, fabs(f));
+                st_s2 += f * f;
+                const double e1 = f - A.Fold[d];
+                st_d2 += e1 * e1;
+                if (e >= nsu && i == j) A.diagL[(size_t)iw * nA + i] = g.y;
+            }
+        }
+        if (scf2) {
+            const double m = block_max_d(st_max, red);
+            const double s2 = block_sum_d(st_s2, red);
+            const double d2 = block_sum_d(st_d2, red);
+            if (tid == 0) {
+                double *pp = A.partials + (size_t)iw * 4;
+                pp[0] = m; pp[1] = s2; pp[2] = d2; pp[3] = 0.0;
+            }
+        }
+        return;
+    }
+    // SCF1 / FINAL: on-site blocks, element (2 si + a2, 2 si + b2) at 4 si + a2 + 2 b2
+    for (int e = tid; e < 4 * A.nsites; e += NT) {
+        const int si = e >> 2, a2 = e & 1, b2 = (e >> 1) & 1;
+        A.onsL[(size_t)iw * 4 * A.nsites + e] = GL(2 * si + a2, 2 * si + b2);
+    }
+    if (A.mode == QS_M_SCF1) {
+        for (int k = tid; k < nA; k += NT) {
+            const int i = A.act[k];
+            A.diagR[(size_t)iw * nA + k] = GR(i, i).y;
+            A.diagL[(size_t)iw * nA + k] = GL(i, i).y;
+        }
+        return;
+    }
+    // FINAL
+    for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = GR(i, i).y;
+    if (A.atomL) {
+        for (int e = tid; e < A.atom_tot; e += NT) {
+            int at = 0;
+            while (at + 1 < A.n_atoms && A.atom_off[at + 1] <= e) at++;
+            const int m = 2 * A.atom_norb[at], loc = e - A.atom_off[at];
+            const int b2 = loc / m, a2 = loc - b2 * m, lo = 2 * A.atom_first[at];
+            A.atomL[(size_t)iw * A.atom_tot + e] = GL(lo + a2, lo + b2);
+        }
+    }
+    if (tid == 0) {
+        // Tr_s{ i GamL [ (f_L - 1) G^< - f_L G^> ] } / 2pi on the left contact  (src/scf.f90:390-409)
+        double tu = 0.0, td = 0.0;
+        for (int a2 = 0; a2 < nCL; a2++) {
+            double2 sacc = make_double2(0.0, 0.0);
+            for (int l = 0; l < nCL; l++) {
+                const double2 glc = GL(A.cl[l], A.cl[a2]);
+                const double2 gr_la = GR(A.cl[l], A.cl[a2]), gr_al = GR(A.cl[a2], A.cl[l]);
+                const double2 gg = cadd(glc, csub(gr_la, cconj(gr_al)));
+                const double2 X = make_double2((fL - 1.0) * glc.x - fL * gg.x, (fL - 1.0) * glc.y - fL * gg.y);
+                cfma(sacc, A.gamL_cl[a2 * nCL + l], X);
+            }
+            if ((A.cl[a2] & 1) == 0) tu += -sacc.y; else td += -sacc.y;
+        }
+        A.trUD[(size_t)iw * 2 + 0] = tu / TWO_PI;
+        A.trUD[(size_t)iw * 2 + 1] = td / TWO_PI;
+    }
+}
+
+hipError_t rb_err(rocblas_status st) { return st == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure; }
+
+}  // namespace
+
+int qs_dyson_big_max_norb() { return 1024; }
+
+size_t qs_dyson_big_bytes(int n, int nw, size_t budget) {
+    const size_t per = (size_t)3 * n * n * sizeof(double2) + (size_t)(n + 1) * sizeof(rocblas_int);
+    size_t chunk = budget / per;
+    if (chunk < 1) chunk = 1;
+    if (chunk > (size_t)nw) chunk = nw;
+    return chunk * per;
+}
+
+QsBigWork *qs_big_create(int n, int nw, size_t budget, size_t *bytes) {
+    QsBigWork *bw = new QsBigWork();
+    bw->n = n;
+    const size_t per = (size_t)3 * n * n * sizeof(double2) + (size_t)(n + 1) * sizeof(rocblas_int);
+    size_t chunk = budget / per;
+    if (chunk < 1) chunk = 1;
+    if (chunk > (size_t)nw) chunk = nw;
+    bw->chunk = (int)chunk;
+    const size_t mat = chunk * n * n * sizeof(double2);
+    bool ok = rocblas_create_handle(&bw->rb) == rocblas_status_success;
+    ok = ok && hipMalloc((void **)&bw->A, mat) == hipSuccess;
+    ok = ok && hipMalloc((void **)&bw->S, mat) == hipSuccess;
+    ok = ok && hipMalloc((void **)&bw->T, mat) == hipSuccess;
+    ok = ok && hipMalloc((void **)&bw->ipiv, chunk * n * sizeof(rocblas_int)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&bw->info, chunk * sizeof(rocblas_int)) == hipSuccess;
+    if (!ok) {
+        qs_big_destroy(bw);
+        return nullptr;
+    }
+    if (bytes) *bytes = 3 * mat + chunk * (n + 1) * sizeof(rocblas_int);
+    return bw;
+}
+
+void qs_big_destroy(QsBigWork *bw) {
+    if (!bw) return;
+    (void)hipFree(bw->A);
+    (void)hipFree(bw->S);
+    (void)hipFree(bw->T);
+    (void)hipFree(bw->ipiv);
+    (void)hipFree(bw->info);
+    if (bw->rb) (void)rocblas_destroy_handle(bw->rb);
+    delete bw;
+}
+
+hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s) {
+    QsBigWork *bw = (QsBigWork *)a.big;
+    if (count <= 0) return hipSuccess;
+    if (!bw || bw->n != a.norb || a.norb > qs_dyson_big_max_norb()) return hipErrorInvalidValue;
+    const int n = a.norb;
+    const rocblas_stride sn = (rocblas_stride)n * n;
+    hipError_t e = rb_err(rocblas_set_stream(bw->rb, s));
+    if (e != hipSuccess) return e;
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+    auto *A = (rocblas_double_complex *)bw->A, *S = (rocblas_double_complex *)bw->S, *T = (rocblas_double_complex *)bw->T;
+    for (int c0 = 0; c0 < count; c0 += bw->chunk) {
+        const int cnt = count - c0 < bw->chunk ? count - c0 : bw->chunk, iwc = a.iw0 + c0;
+        hipLaunchKernelGGL(k_big_build, dim3(cnt), dim3(NT), 0, s, a, iwc, bw->A, bw->S);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = rb_err(rocsolver_zgetrf_strided_batched(bw->rb, n, n, A, n, sn, bw->ipiv, n, bw->info, cnt))) != hipSuccess)
+            return e;
+        if ((e = rb_err(rocsolver_zgetri_strided_batched(bw->rb, n, A, n, sn, bw->ipiv, n, bw->info, cnt))) != hipSuccess)
+            return e;
+        // T = G^r S'';  G^< = T G^r^dagger (into S)
+        if ((e = rb_err(rocblas_zgemm_strided_batched(bw->rb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, A,
+                                                      n, sn, S, n, sn, &zero, T, n, sn, cnt))) != hipSuccess)
+            return e;
+        if ((e = rb_err(rocblas_zgemm_strided_batched(bw->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose,
+                                                      n, n, n, &one, T, n, sn, A, n, sn, &zero, S, n, sn, cnt))) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_big_pack, dim3(cnt), dim3(NT), 0, s, a, iwc, bw->A, bw->S, bw->info);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}

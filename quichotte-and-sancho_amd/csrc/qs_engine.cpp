@@ -36,6 +36,7 @@ constexpr int MAXH_DEV = 32;
 
 // energies per launch of the one-wavefront Dyson kernel (bounds its scratch)
 constexpr int kWaveChunk = 8192;
+constexpr size_t kBigBudget = (size_t)2 << 30;  // chunk buffers of the large-Norb Dyson path
 
 // stages timed with HIP events on the handle's stream (qs_profile)
 enum Stage {
@@ -90,6 +91,7 @@ struct qs_handle {
     double *d_partials = nullptr, *d_red = nullptr, *d_scratch = nullptr;
     double2 *d_mscr = nullptr;   // one-wavefront Dyson kernel scratch (mscr_energies energies)
     int mscr_energies = 0;
+    QsBigWork *big = nullptr;    // large-Norb Dyson path: library handle and chunk buffers
     double *d_diagR = nullptr, *d_diagL = nullptr, *d_dosd = nullptr, *d_trUD = nullptr;
     double2 *d_onsL = nullptr, *d_atomL = nullptr;
     int *d_status = nullptr;
@@ -302,6 +304,14 @@ int setup_device(qs_handle *h) {
         h->mscr_energies = std::min(nw, kWaveChunk);
         if ((rc = dalloc(h, &h->d_mscr, h->mscr_energies * qs_dyson_wave_scratch_bytes(n) / sizeof(double2)))) return rc;
     }
+    // the large-Norb Dyson path, needed above the small-LDS kernel's size or when forced
+    const char *fv = getenv("QS_DYSON_VARIANT");
+    if (n > qs_dyson_wave_max_norb() || (fv && atoi(fv) == QS_DYSON_BIG)) {
+        size_t bytes = 0;
+        h->big = qs_big_create(n, nw, kBigBudget, &bytes);
+        if (!h->big) return fail(h, QS_ERR_HIP, "large-Norb Dyson workspace (rocBLAS handle, chunk buffers)");
+        h->dev_bytes += (int64_t)bytes;
+    }
     if ((rc = dalloc(h, &h->d_red, 64))) return rc;
     // column-sum scratch: 64 row-chunks x the widest column set
     int widest = std::max({8 * h->nsites, n, 2 * std::max(nA, 1), 2 * std::max(h->atom_tot, 1), 2});
@@ -434,6 +444,7 @@ DysonArgs dyson_args(qs_handle *h, int mode, bool use_sigma, double V) {
     a.status = h->d_status;
     a.mscr = h->d_mscr;
     a.mscr_energies = h->mscr_energies;
+    a.big = h->big;
     return a;
 }
 
@@ -514,6 +525,8 @@ int tune_dyson(qs_handle *h) {
     double best = 1e300;
     for (int v = 0; v < nv; v++) {
         if (!qs_dyson_shape_ok(h->norb, v)) continue;
+        // the large-Norb path only where no register-resident shape applies
+        if (v == QS_DYSON_BIG && h->norb <= qs_dyson_wave_max_norb()) continue;
         HIPCHK(h, qs_launch_dyson_variant(a, count, v, h->stream));  // warm-up (code load)
         HIPCHK(h, hipEventRecord(e0, h->stream));
         for (int r = 0; r < 2; r++) HIPCHK(h, qs_launch_dyson_variant(a, count, v, h->stream));
@@ -937,9 +950,9 @@ int check_problem(const qs_problem *p, std::string &msg) {
     if (p->order < 0 || p->order > 2) { msg = "order must be 0, 1 or 2"; return QS_ERR_ARG; }
     if (p->nhist < 0 || p->nhist > MAXH_DEV) { msg = "nhist must be in [0, 32]"; return QS_ERR_ARG; }
     if (p->maxscf < 1) { msg = "maxscf must be >= 1"; return QS_ERR_ARG; }
-    if (p->norb > qs_dyson_max_norb()) {
+    if (p->norb > qs_dyson_big_max_norb()) {
         char b[128];
-        snprintf(b, sizeof b, "norb = %d exceeds the LDS-resident Dyson kernel limit of %d", p->norb, qs_dyson_max_norb());
+        snprintf(b, sizeof b, "norb = %d exceeds the Dyson limit of %d", p->norb, qs_dyson_big_max_norb());
         msg = b;
         return QS_ERR_UNSUPPORTED;
     }
@@ -1012,6 +1025,7 @@ void qs_destroy(qs_handle *h) {
     if (h->p.device >= 0) (void)hipSetDevice(h->p.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
+    qs_big_destroy(h->big);
     for (auto &r : h->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
     if (h->h_pin) (void)hipHostFree(h->h_pin);
@@ -1289,6 +1303,7 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
         b += (nw + Nf) * ncolS * 16;
     }
     if (n <= qs_dyson_wave_max_norb()) b += std::min<int64_t>(nw, kWaveChunk) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
+    else b += (int64_t)qs_dyson_big_bytes((int)n, (int)nw, kBigBudget);
     return b + ((int64_t)64 << 20);
 }
 

@@ -38,6 +38,7 @@ struct DysonArgs {
     int *status;             // bit 0: singular pivot met
     double2 *mscr;           // one-wavefront kernel: 3 norb^2 complex of scratch per energy
     int mscr_energies;       //   for this many energies (launches are chunked to it)
+    void *big;               // large-Norb path: its QsBigWork (library handle and chunk buffers)
 };
 
 // Dyson + Keldysh at every energy of [iw0, iw0 + count).  Returns the hip error.
@@ -49,7 +50,15 @@ size_t qs_dyson_lds_bytes(int norb, int nCL);
 int qs_dyson_max_norb();
 // launch shapes: 0 = 256-thread workgroup, 1 = 512-thread workgroup (norb > 16),
 // 2 = one wavefront per energy (norb <= qs_dyson_wave_max_norb(), needs mscr)
-enum { QS_DYSON_WG256 = 0, QS_DYSON_WG512 = 1, QS_DYSON_WAVE = 2, QS_DYSON_NSHAPES = 3 };
+enum { QS_DYSON_WG256 = 0, QS_DYSON_WG512 = 1, QS_DYSON_WAVE = 2, QS_DYSON_BIG = 3, QS_DYSON_NSHAPES = 4 };
+// large-Norb path (qs_dyson_big.hip): energies in chunks through HBM, batched
+// library factorisation and products, hand-written build and packing kernels
+struct QsBigWork;
+int qs_dyson_big_max_norb();
+size_t qs_dyson_big_bytes(int norb, int nw, size_t budget);
+QsBigWork *qs_big_create(int norb, int nw, size_t budget, size_t *bytes);
+void qs_big_destroy(QsBigWork *bw);
+hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s);
 bool qs_dyson_shape_ok(int norb, int shape);
 // one wavefront per energy (qs_dyson_wave.hip)
 hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s);

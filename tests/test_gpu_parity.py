@@ -45,7 +45,7 @@ def test_init2_matches_oracle(norb, nw):
     assert rel_max(xg, xo) < 1e-11
 
 
-@pytest.mark.parametrize("norb,nw", [(4, 33), (8, 200), (16, 257), (80, 48)])
+@pytest.mark.parametrize("norb,nw", [(4, 33), (8, 200), (16, 257), (80, 48), (100, 24), (128, 12)])
 def test_residual2_and_sigma_match_oracle(norb, nw):
     prob = random_problem(norb, nw, seed=10 + norb, U=1.3)
     o = Oracle(prob)
@@ -165,11 +165,12 @@ def test_runaway_and_errors_are_reported():
     assert ab and not F.any()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("norb", [18, 48, 80, 88])
 def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     """Every launch shape of the Dyson kernel: 256- and 512-thread workgroups with
-    the matrix staged in LDS, and the 16 x 16 workgroup with a scratch slab."""
+    the matrix staged in LDS, the 16 x 16 workgroup with a scratch slab, and the
+    large-Norb path (batched library factorisation) forced at these sizes."""
     if variant == 0 and (norb + 15) // 16 == 5:
         pytest.skip("the 256-thread shape at 65..80 orbitals is switched off in the library: it returned wrong "
                     "init2 values at 80 orbitals and a GPU memory fault followed, cause not found")
@@ -188,7 +189,7 @@ def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     assert rel_max(Fg, Fo) < 1e-11, worst_packed(Fg, Fo, o.nA)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("order", [1, 2])
 def test_each_dyson_launch_shape_full_solve(order, variant, monkeypatch):
     """Full solves through one launch shape: the SCF1, SCF2, INIT2 and FINAL
