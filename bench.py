@@ -72,6 +72,10 @@ def workload(name):
         w = dict(N_ions=8, N_turns=5, U=2.0, log2nw=16, wmax=12.0, volts=[round(-1.0 + 0.1 * k, 10) for k in range(21)])
     elif name == "helix16":
         w = dict(N_ions=4, N_turns=2, U=1.0, log2nw=14, wmax=10.0, volts=[round(0.1 * k, 10) for k in range(11)])
+    elif name == "helix240":
+        # the 240-orbital size of the dftb_hs configuration, at 2^12 energies so that
+        # x, F, Fold, xold and the nhist 16 history fit one GPU
+        w = dict(N_ions=12, N_turns=10, U=1.5, log2nw=12, wmax=12.0, volts=[round(-1.0 + 0.1 * k, 10) for k in range(21)])
     else:
         raise SystemExit("unknown workload " + name)
     H = helix_H(w["N_ions"], w["N_turns"])
@@ -253,7 +257,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="helix80", choices=["helix80", "helix16"])
+    ap.add_argument("--workload", default="helix80", choices=["helix80", "helix16", "helix240"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--serve", default="", help="comma-separated Poisson arrival rates (requests/s)")
@@ -342,7 +346,8 @@ def main():
         "config": {"workload": desc, "form": "plain FP64 path, one bias point at a time",
                    "n_spin_orbitals": eng.norb, "n_omega": eng.prob.n_omega, "fft_length": eng.Nf,
                    "dyson_launch_shape": ["256-thread workgroup, matrix in LDS", "512-thread workgroup, matrix in LDS",
-                                          "small-LDS workgroup, scratch slab"][dv]},
+                                          "small-LDS workgroup, scratch slab",
+                                          "large-Norb chunks: batched zgetrf/zgetri + zgemm"][dv]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "scf_steps_per_point": float(np.mean(iters)),
         "all_converged": bool(all(conv)),
