@@ -31,7 +31,8 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int CB = 8;  // columns per site-pair block
+constexpr int CB = 8;   // columns per block of the passes over n1 / k1
+constexpr int BPW = 4;  // column blocks per workgroup in those passes
 
 __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned)v) >> (32 - bits); }
 
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 }
 
 // ---- F1 -----------------------------------------------------------------------
-// grid: (ncol / CB, N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
+// grid: (ncol / (CB BPW), N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
 // Thread work is unrolled over constant bounds (QS = CB N1 / NT elements per
 // thread, QI of them nonzero inputs) so that every global load of a round is in
 // flight at once.
@@ -151,37 +152,46 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
     constexpr int N1 = 1 << LG1, LD = seq_stride(N1), HALF = CB * N1 / 2;
     constexpr int QS = (CB * N1 + NT - 1) / NT, QI = (HALF + NT - 1) / NT;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
-    const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y;
+    const int ncol = p.ncol, n2 = blockIdx.y, nblk = ncol / CB;
+    const int cb0 = blockIdx.x * BPW, cb1 = min(cb0 + BPW, nblk);
     load_twl<N1>(twl, p.tw, p.Nf);
-    // rows n1 >= N1 / 2 are the zero padding: the inputs are QI rounds
-    double2 v[QI];
+    // rows n1 >= N1 / 2 are the zero padding: the inputs of a block are QI rounds
+    auto fetch = [&](int cb, double2 *v) {
 #pragma unroll
-    for (int q = 0; q < QI; q++) {
-        const int t = threadIdx.x + q * NT, n = n2 + p.N2 * (t >> 3);
-        v[q] = make_double2(0.0, 0.0);
-        if (t < HALF && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & 7)];
-    }
-    // output twiddles e^{-2 pi i k1 n2 / Nf}, k1 = (tid >> 3) + (NT / 8) i
-    double2 w[QS];
-#pragma unroll
-    for (int i = 0; i < QS; i++) {
-        const int k1 = (threadIdx.x >> 3) + (NT / 8) * i;
-        w[i] = k1 < N1 ? p.tw[k1 * n2] : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-    for (int q = 0; q < QS; q++) {
-        const int t = threadIdx.x + q * NT;
-        if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = q < QI ? v[q] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    lds_fft<LG1, CB, -1>(buf, twl);
-#pragma unroll
-    for (int i = 0; i < QS; i++) {
-        const int t = threadIdx.x + i * NT;
-        if (t < CB * N1) {
-            const int col = t & 7, k1 = t >> 3;
-            Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * LD + px(k1)], w[i]);
+        for (int q = 0; q < QI; q++) {
+            const int t = threadIdx.x + q * NT, n = n2 + p.N2 * (t >> 3);
+            v[q] = make_double2(0.0, 0.0);
+            if (t < HALF && n < p.nw) v[q] = Gt[(size_t)n * ncol + cb * CB + (t & 7)];
         }
+    };
+    double2 v[QI];
+    fetch(cb0, v);
+    // BPW column blocks of row n2; the next block's inputs load during this one's transform
+    for (int cb = cb0; cb < cb1; cb++) {
+#pragma unroll
+        for (int q = 0; q < QS; q++) {
+            const int t = threadIdx.x + q * NT;
+            if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = q < QI ? v[q] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (cb + 1 < cb1) fetch(cb + 1, v);
+        lds_fft<LG1, CB, -1>(buf, twl);
+        // output twiddles e^{-2 pi i k1 n2 / Nf}, k1 = (tid >> 3) + (NT / 8) i (cached: the same for every block)
+        double2 w[QS];
+#pragma unroll
+        for (int i = 0; i < QS; i++) {
+            const int k1 = (threadIdx.x >> 3) + (NT / 8) * i;
+            w[i] = k1 < N1 ? p.tw[k1 * n2] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < QS; i++) {
+            const int t = threadIdx.x + i * NT;
+            if (t < CB * N1) {
+                const int col = t & 7, k1 = t >> 3;
+                Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * LD + px(k1)], w[i]);
+            }
+        }
+        __syncthreads();  // the next block overwrites buf
     }
 }
 
@@ -393,34 +403,43 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 }
 
 // ---- I2 -----------------------------------------------------------------------
-// grid: (ncol / CB, N2).  LDS as in F1.  Only the outputs n < nw are kept.
+// grid: (ncol / (CB BPW), N2).  LDS as in F1.  Only the outputs n < nw are kept.
 template <int LG1>
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
     constexpr int N1 = 1 << LG1, LD = seq_stride(N1), QS = (CB * N1 + NT - 1) / NT;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
-    const int ncol = p.ncol, cb = blockIdx.x, n2 = blockIdx.y;
+    const int ncol = p.ncol, n2 = blockIdx.y, nblk = ncol / CB;
+    const int cb0 = blockIdx.x * BPW, cb1 = min(cb0 + BPW, nblk);
     load_twl<N1>(twl, p.tw, p.Nf);
+    auto fetch = [&](int cb, double2 *v) {
+#pragma unroll
+        for (int q = 0; q < QS; q++) {
+            const int t = threadIdx.x + q * NT;
+            v[q] = make_double2(0.0, 0.0);
+            if (t < CB * N1) v[q] = Z[((size_t)(t >> 3) * p.N2 + n2) * ncol + cb * CB + (t & 7)];
+        }
+    };
     double2 v[QS];
+    fetch(cb0, v);
+    // BPW column blocks of row n2; the next block's inputs load during this one's transform
+    for (int cb = cb0; cb < cb1; cb++) {
 #pragma unroll
-    for (int q = 0; q < QS; q++) {
-        const int t = threadIdx.x + q * NT;
-        v[q] = make_double2(0.0, 0.0);
-        if (t < CB * N1) v[q] = Z[((size_t)(t >> 3) * p.N2 + n2) * ncol + cb * CB + (t & 7)];
-    }
+        for (int q = 0; q < QS; q++) {
+            const int t = threadIdx.x + q * NT;
+            if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = v[q];
+        }
+        __syncthreads();
+        if (cb + 1 < cb1) fetch(cb + 1, v);
+        lds_fft<LG1, CB, +1>(buf, twl);
 #pragma unroll
-    for (int q = 0; q < QS; q++) {
-        const int t = threadIdx.x + q * NT;
-        if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = v[q];
-    }
-    __syncthreads();
-    lds_fft<LG1, CB, +1>(buf, twl);
-#pragma unroll
-    for (int i = 0; i < QS; i++) {
-        const int t = threadIdx.x + i * NT;
-        const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
-        if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * LD + px(n1)];
+        for (int i = 0; i < QS; i++) {
+            const int t = threadIdx.x + i * NT;
+            const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
+            if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * LD + px(n1)];
+        }
+        __syncthreads();  // the next block overwrites buf
     }
 }
 
@@ -443,7 +462,7 @@ hipError_t launch_f1(const Sigma2Plan &p, const double2 *Gt, double2 *Z, hipStre
         attr = true;
     }
     const size_t lds = ((size_t)(1 << LG1) / 2 + (size_t)CB * seq_stride(1 << LG1)) * sizeof(double2);
-    hipLaunchKernelGGL(k_fft_f1<LG1>, dim3(p.ncol / CB, p.N2), dim3(NT), lds, s, Gt, Z, p);
+    hipLaunchKernelGGL(k_fft_f1<LG1>, dim3((p.ncol / CB + BPW - 1) / BPW, p.N2), dim3(NT), lds, s, Gt, Z, p);
     return hipGetLastError();
 }
 template <int LG1>
@@ -454,7 +473,7 @@ hipError_t launch_i2(const Sigma2Plan &p, const double2 *Z, double2 *Sig, hipStr
         attr = true;
     }
     const size_t lds = ((size_t)(1 << LG1) / 2 + (size_t)CB * seq_stride(1 << LG1)) * sizeof(double2);
-    hipLaunchKernelGGL(k_fft_i2<LG1>, dim3(p.ncol / CB, p.N2), dim3(NT), lds, s, Z, Sig, p);
+    hipLaunchKernelGGL(k_fft_i2<LG1>, dim3((p.ncol / CB + BPW - 1) / BPW, p.N2), dim3(NT), lds, s, Z, Sig, p);
     return hipGetLastError();
 }
 template <int LG2>
