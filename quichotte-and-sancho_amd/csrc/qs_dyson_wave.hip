@@ -38,8 +38,8 @@
 namespace {
 
 constexpr double TWO_PI = 6.283185307179586476925286766559;
-constexpr int EB = 8;   // G^r packing elements per thread in flight
-constexpr int EBL = 4;  // G^< packing elements per thread in flight (P is live then)
+constexpr int EB = 13;  // G^r packing elements per thread in flight
+constexpr int EBL = 5;  // G^< packing elements per thread in flight (P is live then)
 
 // pivot candidates (value, row; row < 0: none): keep the larger value, then the
 // smaller row.  Commutative, so both lanes of a pair agree on the result.
