@@ -196,12 +196,17 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
 }
 
 // ---- M ------------------------------------------------------------------------
+// a0 b0 + a1 b1 (complex) as one multiply and three FMAs per real component
+__device__ __forceinline__ double2 cdot2(double2 a0, double2 b0, double2 a1, double2 b1) {
+    return make_double2(fma(a0.x, b0.x, fma(-a0.y, b0.y, fma(a1.x, b1.x, -a1.y * b1.y))),
+                        fma(a0.x, b0.y, fma(a0.y, b0.x, fma(a1.x, b1.y, a1.y * b1.x))));
+}
 __device__ __forceinline__ void mm2(const double2 *A, const double2 *B, double2 *C) {
     // 2x2 row-major: [0]=11 [1]=12 [2]=21 [3]=22
-    C[0] = cadd(cmul(A[0], B[0]), cmul(A[1], B[2]));
-    C[1] = cadd(cmul(A[0], B[1]), cmul(A[1], B[3]));
-    C[2] = cadd(cmul(A[2], B[0]), cmul(A[3], B[2]));
-    C[3] = cadd(cmul(A[2], B[1]), cmul(A[3], B[3]));
+    C[0] = cdot2(A[0], B[0], A[1], B[2]);
+    C[1] = cdot2(A[0], B[1], A[1], B[3]);
+    C[2] = cdot2(A[2], B[0], A[3], B[2]);
+    C[3] = cdot2(A[2], B[1], A[3], B[3]);
 }
 __device__ __forceinline__ void adj2(const double2 *M, double2 *A) {
     A[0] = M[3];
