@@ -111,30 +111,43 @@ __device__ __forceinline__ void load_twl(double2 *twl, const double2 *tw, int Nf
 // ---- x (reference packed layout) -> Gt[omega][c] (blocked) ----------------------
 // One workgroup per energy: the gather of an energy's packed slice (3 nA^2
 // doubles) stays inside one XCD's L2, so each line of x is fetched once.
-constexpr int PU = 8;  // gathers in flight per thread
+constexpr int PU = 8;   // columns per thread per batch
+constexpr int EPW = 4;  // energies per workgroup: one read of the column map serves all of them
 __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x, double2 *__restrict__ Gt,
                                                    const int2 *__restrict__ src, int nA, int ncol, int nw) {
     const size_t nblk = 3 * (size_t)nA * nA;
-    for (int iw = blockIdx.x; iw < nw; iw += gridDim.x) {
-        const double *xb = x + (size_t)iw * nblk;
-        double2 *g = Gt + (size_t)iw * ncol;
+    for (int iw0 = blockIdx.x * EPW; iw0 < nw; iw0 += gridDim.x * EPW) {
         for (int c0 = threadIdx.x; c0 < ncol; c0 += PU * NT) {
-            double re[PU], im[PU];
+            // column c: real part from +/- x[|sc.x| - 1] (0: zero), imaginary part from x[sc.y] (< 0: padding)
+            int ir[PU], ii[PU], kr[PU], ki[PU];  // kr: 1 copy, -1 negate, 0 zero; ki: 1 copy, 0 zero
 #pragma unroll
             for (int u = 0; u < PU; u++) {
-                const int c = min(c0 + u * NT, ncol - 1);
-                const int2 sc = src[c];
-                const int ir = sc.x > 0 ? sc.x - 1 : (sc.x < 0 ? -sc.x - 1 : 0);
-                re[u] = xb[ir];
-                im[u] = xb[sc.y >= 0 ? sc.y : 0];
-                if (sc.x < 0) re[u] = -re[u];
-                if (sc.x == 0) re[u] = 0.0;
-                if (sc.y < 0) im[u] = 0.0;  // padding column
+                const int2 sc = src[min(c0 + u * NT, ncol - 1)];
+                ir[u] = sc.x > 0 ? sc.x - 1 : (sc.x < 0 ? -sc.x - 1 : 0);
+                kr[u] = sc.x > 0 ? 1 : (sc.x < 0 ? -1 : 0);
+                ii[u] = sc.y >= 0 ? sc.y : 0;
+                ki[u] = sc.y >= 0;
+            }
+            double re[EPW][PU], im[EPW][PU];
+#pragma unroll
+            for (int e = 0; e < EPW; e++) {
+                const double *xb = x + (size_t)min(iw0 + e, nw - 1) * nblk;
+#pragma unroll
+                for (int u = 0; u < PU; u++) {
+                    re[e][u] = xb[ir[u]];
+                    im[e][u] = xb[ii[u]];
+                }
             }
 #pragma unroll
-            for (int u = 0; u < PU; u++) {
-                const int c = c0 + u * NT;
-                if (c < ncol) g[c] = make_double2(re[u], im[u]);
+            for (int e = 0; e < EPW; e++) {
+                if (iw0 + e >= nw) break;
+                double2 *g = Gt + (size_t)(iw0 + e) * ncol;
+#pragma unroll
+                for (int u = 0; u < PU; u++) {
+                    const int c = c0 + u * NT;
+                    const double r = kr[u] == 0 ? 0.0 : (kr[u] < 0 ? -re[e][u] : re[e][u]);
+                    if (c < ncol) g[c] = make_double2(r, ki[u] ? im[e][u] : 0.0);
+                }
             }
         }
     }
@@ -452,7 +465,7 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
                                 hipStream_t s) {
-    const int blocks = nw < 65536 ? nw : 65536;
+    const int groups = (nw + EPW - 1) / EPW, blocks = groups < 65536 ? groups : 65536;
     hipLaunchKernelGGL(k_permute_in, dim3(blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, ncol, nw);
     return hipGetLastError();
 }
