@@ -83,8 +83,12 @@ def test_sigma2_longest_split(nw):
     assert rel_max(slg, slo) < 1e-11
 
 
-def test_sigma2_nonpower_of_two_grid_and_noninteracting_site():
-    """7-smooth oracle length vs power-of-two device length, U = 0 on a contact site."""
+@pytest.mark.parametrize("shape", ["tuned", "large-Norb"])
+def test_sigma2_nonpower_of_two_grid_and_noninteracting_site(shape, monkeypatch):
+    """7-smooth oracle length vs power-of-two device length, U = 0 on a contact site
+    (an active set smaller than the orbitals)."""
+    if shape == "large-Norb":
+        monkeypatch.setenv("QS_DYSON_VARIANT", "3")
     prob = random_problem(8, 361, seed=5)
     prob.U_site[2] = 0.0
     o = Oracle(prob)
@@ -128,8 +132,13 @@ FIXTURES = ["order0_noninteracting", "order1_hartree_fock", "order2_second_order
             "order1_custom_template"]
 
 
+@pytest.mark.parametrize("shape", ["tuned", "large-Norb"])
 @pytest.mark.parametrize("case", FIXTURES)
-def test_gpu_reproduces_reference_fixture(case):
+def test_gpu_reproduces_reference_fixture(case, shape, monkeypatch):
+    """The reference's own outputs, through the tuned Dyson shape and through the
+    large-Norb path forced at these sizes (every order and epilogue mode)."""
+    if shape == "large-Norb":
+        monkeypatch.setenv("QS_DYSON_VARIANT", "3")
     cd = os.path.join(GOLDEN, "cases", case)
     prob = Problem.from_case(cd)
     g = gpu_engine(prob)
