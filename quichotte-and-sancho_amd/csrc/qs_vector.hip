@@ -30,9 +30,18 @@ __global__ __launch_bounds__(NT) void k_et_partial(const double *__restrict__ x,
     if (q >= nq) return;
     const int off = srcidx[q];
     const int per = (nw + nch - 1) / nch, w0 = chunk * per, w1 = min(nw, w0 + per);
-    double s = 0.0;
-    for (int iw = w0; iw < w1; iw++) s += x[(size_t)iw * nblk + off];
-    part[(size_t)chunk * nq + q] = s;
+    // four interleaved sums keep four loads in flight; combined in a fixed order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int iw = w0;
+    for (; iw + 3 < w1; iw += 4) {
+        const double *xp = x + (size_t)iw * nblk + off;
+        s0 += xp[0];
+        s1 += xp[nblk];
+        s2 += xp[2 * nblk];
+        s3 += xp[3 * nblk];
+    }
+    for (; iw < w1; iw++) s0 += x[(size_t)iw * nblk + off];
+    part[(size_t)chunk * nq + q] = (s0 + s1) + (s2 + s3);
 }
 
 __global__ void k_et_final(const double *__restrict__ part, int nq, int nch, const int *__restrict__ act_site,
@@ -42,9 +51,17 @@ __global__ void k_et_final(const double *__restrict__ part, int nq, int nch, con
     if (ia >= nq / 4) return;
     double S[4];
     for (int q = 0; q < 4; q++) {
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += part[(size_t)c * nq + 4 * ia + q];
-        S[q] = s;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int c = 0;
+        for (; c + 3 < nch; c += 4) {
+            const double *pp = part + (size_t)c * nq + 4 * ia + q;
+            s0 += pp[0];
+            s1 += pp[nq];
+            s2 += pp[2 * (size_t)nq];
+            s3 += pp[3 * (size_t)nq];
+        }
+        for (; c < nch; c++) s0 += part[(size_t)c * nq + 4 * ia + q];
+        S[q] = (s0 + s1) + (s2 + s3);
     }
     // G<(t=0) block: g00 = i pp S0, g11 = i pp S1, g01 = pp (S2 + i S3), g10 = -conj(g01)
     double2 g[4];
