@@ -22,24 +22,27 @@
 //     column k+1 together with that whole row and column k+1 (double-buffered
 //     LDS); after the barrier every thread picks the pivot and has its row and
 //     column.
-//  3. M goes to the slab twice: column-major (Ms, operand of the Keldysh
-//     products) and as G^r in natural order (Gs, for packing).
+//  3. M goes to the slab column-major (Ms): the operand of the Keldysh products,
+//     and G^r for the packing through the pivot maps, G^r(i, j) = M(piv[i], qinv[j]).
 //  4. T = M S'' with S''[c][c'] = SL[p_c][p_c'] as outer products over c, TI
 //     values of c at a time: one round of loads brings TI columns of M (from
 //     Ms) and builds TI rows of S'' into two LDS chunks, then TI rank-1
 //     updates read them from LDS.  T goes to the slab (Ts).
 //  5. P = T M^dagger the same way (TI columns of T and of M per round);
 //     G^<[q_r][q_r'] = P[r][r'].
-//  6. Epilogues by mode from TI-row LDS stages of G^< and from Gs rows, with
-//     the loads of x and Fold batched ahead of the stores.
+//  6. Epilogues by mode from TI-row LDS stages of G^< and from Ms, with the
+//     loads of x and Fold batched ahead of the stores.
 #include "qs_device.h"
 #include "qs_internal.h"
 
 namespace {
 
 constexpr double TWO_PI = 6.283185307179586476925286766559;
-constexpr int EB = 13;  // G^r packing elements per thread in flight
-constexpr int EBL = 5;  // G^< packing elements per thread in flight (P is live then)
+// packing elements per thread in flight: G^r, and G^< (P is live then).  The
+// larger batches only where occupancy is two waves per SIMD anyway (25 elements
+// per thread); smaller tiles keep their registers for occupancy.
+template <int RA, int RB> constexpr int eb_gr() { return RA * RB >= 25 ? 13 : 8; }
+template <int RA, int RB> constexpr int eb_gl() { return RA * RB >= 25 ? 5 : 4; }
 
 // pivot candidates (value, row; row < 0: none): keep the larger value, then the
 // smaller row.  Commutative, so both lanes of a pair agree on the result.
@@ -113,7 +116,7 @@ __device__ __forceinline__ WLayout wlayout(char *base, int n, int nCL, int RA, i
 
 template <int RA, int RB, int TI, int TJ>
 __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
-    constexpr int NT = TI * TJ, NW = NT / 64;
+    constexpr int NT = TI * TJ, NW = NT / 64, EB = eb_gr<RA, RB>(), EBL = eb_gl<RA, RB>();
     extern __shared__ __align__(16) char lds_raw[];
     const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL;
     const int tid = threadIdx.x, li = tid / TJ, lj = tid % TJ, lane = tid & 63, wv = tid >> 6;
@@ -123,9 +126,10 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
-    double2 *Ms = A.mscr + (size_t)blockIdx.x * 3 * n * n;  // [c][r]: M column-major
-    double2 *Gs = Ms + n * n;                                // [i][j]: G^r row-major
-    double2 *Ts = Gs + n * n;                                // [c][r]: T column-major
+    double2 *Ms = A.mscr + (size_t)blockIdx.x * 2 * n * n;  // [c][r]: M column-major
+    double2 *Ts = Ms + n * n;                                // [c][r]: T column-major
+    // G^r[q_r][p_c] = M[r][c]: G^r(i, j) = M(piv[i], qinv[j]), read from Ms
+    auto GR = [&](int i, int j) { return Ms[L.qinv[j] * n + L.piv[i]]; };
     const int colbase = lj * n + li;                         // (r, c) = (li + TI a, lj + TJ b) at colbase + TJ b n + TI a
 
     for (int i = tid; i < n; i += NT) L.posl[i] = A.pos[i];
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         __syncthreads();
     }
 
-    // ---- 3. scratch copies of M: Ms (column-major), Gs (G^r, natural) ------------
+    // ---- 3. scratch copy of M: Ms (column-major) ------------------------------------
     for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
 #pragma unroll
     for (int a = 0; a < RA; a++)
@@ -294,7 +298,6 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
             const int r = li + TI * a, c = lj + TJ * b;
             if (r < n && c < n) {
                 Ms[colbase + TJ * b * n + TI * a] = M[a][b];
-                Gs[L.qinv[r] * n + L.piv[c]] = M[a][b];
             }
         }
     __threadfence_block();
@@ -502,7 +505,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         __syncthreads();
     }
 
-    // G^r from Gs rows (natural order)
+    // G^r from Ms through the pivot maps
     if (init2 || scf2) {
         for (int e0 = tid; e0 < nA2; e0 += NT * EB) {
             double x0[EB], x1[EB], o0[EB], o1[EB];
@@ -510,7 +513,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
 #pragma unroll
             for (int u = 0; u < EB; u++) {
                 const int e = min(e0 + NT * u, nA2 - 1), ar = e / nA, bb = e - ar * nA;
-                g[u] = Gs[L.actl[ar] * n + L.actl[bb]];
+                g[u] = GR(L.actl[ar], L.actl[bb]);
                 if (scf2) {
                     x0[u] = xe[e]; x1[u] = xe[e + nA2];
                     o0[u] = oe[e]; o1[u] = oe[e + nA2];
@@ -550,12 +553,12 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
             }
         }
     } else if (A.mode == QS_M_SCF1) {
-        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = Gs[L.actl[k2] * n + L.actl[k2]].y;
+        for (int k2 = tid; k2 < nA; k2 += NT) A.diagR[(size_t)iw * nA + k2] = GR(L.actl[k2], L.actl[k2]).y;
     } else {
-        for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = Gs[i * n + i].y;
+        for (int i = tid; i < n; i += NT) A.dosd[(size_t)iw * n + i] = GR(i, i).y;
         for (int e = tid; e < nCL * nCL; e += NT) {
             const int k2 = e / nCL, a2 = e - k2 * nCL;
-            L.grcl[e] = Gs[A.cl[k2] * n + A.cl[a2]];
+            L.grcl[e] = GR(A.cl[k2], A.cl[a2]);
         }
         __syncthreads();
         if (tid == 0) {
@@ -608,7 +611,7 @@ size_t qs_dyson_wave_lds_bytes(int n, int nCL) {
     return wlayout_bytes(n, nCL, R, R, T, T);
 }
 
-size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)3 * n * n * sizeof(double2); }
+size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)2 * n * n * sizeof(double2); }
 
 int qs_dyson_wave_max_norb() { return 96; }
 

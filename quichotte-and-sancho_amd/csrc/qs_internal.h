@@ -36,7 +36,7 @@ struct DysonArgs {
     double2 *atomL;          // FINAL: [iw][atom_tot]
     double *trUD;            // FINAL: [iw][2] current integrand up / down
     int *status;             // bit 0: singular pivot met
-    double2 *mscr;           // one-wavefront kernel: 3 norb^2 complex of scratch per energy
+    double2 *mscr;           // small-LDS kernel: 2 norb^2 complex of scratch per energy (M, T)
     int mscr_energies;       //   for this many energies (launches are chunked to it)
     void *big;               // large-Norb path: its QsBigWork (library handle and chunk buffers)
 };
