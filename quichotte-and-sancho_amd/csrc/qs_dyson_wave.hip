@@ -11,8 +11,8 @@
 // elements (li + TI a, lj + TJ b) of the matrix in registers (norb <= TI RA,
 // norb <= TJ RB).  The workgroup keeps only TI-row stages in LDS (about 60 KB at
 // norb = 80 against 125 KB for k_dyson), so two energies share a CU and each
-// hides the other's barrier and memory latency; the full matrices of the
-// Keldysh products go through a per-energy scratch slab in HBM.
+// hides the other's barrier and memory latency; M, the operand of both Keldysh
+// products, goes through a per-energy scratch slab in HBM.
 //
 // Steps:
 //  1. A = w + Hemb - Sigma1 - Sigma2^r in registers.
@@ -27,9 +27,10 @@
 //  4. T = M S'' with S''[c][c'] = SL[p_c][p_c'] as outer products over c, TI
 //     values of c at a time: one round of loads brings TI columns of M (from
 //     Ms) and builds TI rows of S'' into two LDS chunks, then TI rank-1
-//     updates read them from LDS.  T goes to the slab (Ts).
-//  5. P = T M^dagger the same way (TI columns of T and of M per round);
-//     G^<[q_r][q_r'] = P[r][r'].
+//     updates read them from LDS.  T stays in registers.
+//  5. P = T M^dagger the same way: per round, TI columns of T go from registers
+//     (they are one register column of every thread) and TI columns of M from
+//     Ms into the chunks; G^<[q_r][q_r'] = P[r][r'].
 //  6. Epilogues by mode from TI-row LDS stages of G^< and from Ms, with the
 //     loads of x and Fold batched ahead of the stores.
 #include "qs_device.h"
@@ -126,8 +127,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
-    double2 *Ms = A.mscr + (size_t)blockIdx.x * 2 * n * n;  // [c][r]: M column-major
-    double2 *Ts = Ms + n * n;                                // [c][r]: T column-major
+    double2 *Ms = A.mscr + (size_t)blockIdx.x * n * n;  // [c][r]: M column-major
     // G^r[q_r][p_c] = M[r][c]: G^r(i, j) = M(piv[i], qinv[j]), read from Ms
     auto GR = [&](int i, int j) { return Ms[L.qinv[j] * n + L.piv[i]]; };
     const int colbase = lj * n + li;                         // (r, c) = (li + TI a, lj + TJ b) at colbase + TJ b n + TI a
@@ -360,37 +360,37 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int a = 0; a < RA; a++)
-#pragma unroll
-        for (int b = 0; b < RB; b++) {
-            const int r = li + TI * a, c = lj + TJ * b;
-            if (r < n && c < n) Ts[colbase + TJ * b * n + TI * a] = T[a][b];
-        }
-    __threadfence_block();
-    __syncthreads();
 
     // ---- 5. P = T M^dagger --------------------------------------------------------
-    // P = sum_c T(:, c) conj(M(:, c))^T, TI steps at a time: columns c of T
-    // (from Ts) to stage[kk][*] and of M (from Ms) to chunk[kk][*].
+    // P = sum_c T(:, c) conj(M(:, c))^T, TJ steps at a time.  Columns k0 .. k0 + TJ - 1
+    // of T are this thread's register column bb (TI = TJ): they go from registers to
+    // stage[kk][*], and the columns of M from Ms to chunk[kk][*].
     double2 P[RA][RB];
 #pragma unroll
     for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < RB; b++) P[a][b] = make_double2(0.0, 0.0);
-    for (int k0 = 0; k0 < n; k0 += TI) {
-        const int kc = min(TI, n - k0);
-        double2 tv[RA], mv[RA];
+#pragma unroll
+    for (int bb = 0; bb < RB; bb++) {
+        const int k0 = TJ * bb;
+        if (k0 >= n) break;
+        const int kc = min(TJ, n - k0);
+        double2 mv[RA];
 #pragma unroll
         for (int q = 0; q < RA; q++) {
             const int e = tid + NT * q, ec = e < kc * n ? e : 0;
-            tv[q] = Ts[k0 * n + ec];
             mv[q] = Ms[k0 * n + ec];
         }
 #pragma unroll
+        for (int a = 0; a < RA; a++)
+            if (lj < kc && li + TI * a < n) {
+                L.stage[lj * n + li + TI * a].x = T[a][bb].x;
+                L.stage[lj * n + li + TI * a].y = T[a][bb].y;
+            }
+#pragma unroll
         for (int q = 0; q < RA; q++) {
-            L.stage[tid + NT * q].x = tv[q].x; L.stage[tid + NT * q].y = tv[q].y;
-            L.chunk[tid + NT * q].x = mv[q].x; L.chunk[tid + NT * q].y = mv[q].y;
+            L.chunk[tid + NT * q].x = mv[q].x;
+            L.chunk[tid + NT * q].y = mv[q].y;
         }
         __syncthreads();
         for (int kk = 0; kk < kc; kk++) {
@@ -611,7 +611,7 @@ size_t qs_dyson_wave_lds_bytes(int n, int nCL) {
     return wlayout_bytes(n, nCL, R, R, T, T);
 }
 
-size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)2 * n * n * sizeof(double2); }
+size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)n * n * sizeof(double2); }
 
 int qs_dyson_wave_max_norb() { return 96; }
 
