@@ -34,8 +34,14 @@ using zd = std::complex<double>;
 constexpr double PI = 3.14159265358979323846;
 constexpr int MAXH_DEV = 32;
 
-// energies per launch of the one-wavefront Dyson kernel (bounds its scratch)
-constexpr int kWaveChunk = 8192;
+constexpr size_t kWaveSlab = (size_t)8 << 30;  // small-LDS Dyson scratch: energies per launch up to 8 GB of M
+
+// energies per launch of the small-LDS Dyson kernel (its slab holds one M per energy)
+int wave_chunk(int n, int nw) {
+    const size_t per = qs_dyson_wave_scratch_bytes(n);
+    const size_t c = per ? kWaveSlab / per : (size_t)nw;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nw, c));
+}
 constexpr size_t kBigBudget = (size_t)2 << 30;  // chunk buffers of the large-Norb Dyson path
 
 // stages timed with HIP events on the handle's stream (qs_profile)
@@ -301,7 +307,7 @@ int setup_device(qs_handle *h) {
     if ((rc = dalloc(h, &h->d_diagL, (size_t)nw * std::max(nA, 1)))) return rc;
     if ((rc = dalloc(h, &h->d_partials, (size_t)nw * 4))) return rc;
     if (n <= qs_dyson_wave_max_norb()) {
-        h->mscr_energies = std::min(nw, kWaveChunk);
+        h->mscr_energies = wave_chunk(n, nw);
         if ((rc = dalloc(h, &h->d_mscr, h->mscr_energies * qs_dyson_wave_scratch_bytes(n) / sizeof(double2)))) return rc;
     }
     // the large-Norb Dyson path, needed above the small-LDS kernel's size or when forced
@@ -1302,7 +1308,7 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
         const int64_t nAs = nA / 2, ncolS = (12 * (nAs * (nAs + 1) / 2) + 7) / 8 * 8;
         b += (nw + Nf) * ncolS * 16;
     }
-    if (n <= qs_dyson_wave_max_norb()) b += std::min<int64_t>(nw, kWaveChunk) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
+    if (n <= qs_dyson_wave_max_norb()) b += (int64_t)wave_chunk((int)n, (int)nw) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
     else b += (int64_t)qs_dyson_big_bytes((int)n, (int)nw, kBigBudget);
     return b + ((int64_t)64 << 20);
 }
