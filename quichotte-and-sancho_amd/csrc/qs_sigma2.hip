@@ -78,10 +78,31 @@ __device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
             for (int t = 0; t < T; t++)
                 if (!(t & (1 << e))) {
                     const int m = j + ((t & ((1 << e) - 1)) << LH);  // position inside the level's group
-                    double2 w = twl[tsw(m << tws)];
+                    const double2 u = v[t], y = v[t + (1 << e)];
+                    // first sweep (H = 1, j = 0): m is a compile-time constant and the
+                    // angle 2 pi m / 2^(e+1) a multiple of pi / 4; w = 1 and w = -/+ i
+                    // need only additions
+                    const int q8 = S0 == 1 ? (m << (2 - e)) & 7 : -1;  // angle in units of pi / 4
+                    if (q8 == 0) {
+                        v[t] = make_double2(u.x + y.x, u.y + y.y);
+                        v[t + (1 << e)] = make_double2(u.x - y.x, u.y - y.y);
+                        continue;
+                    }
+                    if (q8 == 2) {  // w = -i (forward), +i (inverse)
+                        const double2 wy = SIGN < 0 ? make_double2(y.y, -y.x) : make_double2(-y.y, y.x);
+                        v[t] = make_double2(u.x + wy.x, u.y + wy.y);
+                        v[t + (1 << e)] = make_double2(u.x - wy.x, u.y - wy.y);
+                        continue;
+                    }
+                    double2 w;
+                    if (q8 > 0) {  // e^{-i pi q8 / 4}, q8 = 1 or 3
+                        constexpr double R = 0.70710678118654752440;
+                        w = make_double2(q8 == 1 ? R : -R, -R);
+                    } else {
+                        w = twl[tsw(m << tws)];
+                    }
                     if (SIGN > 0) w.y = -w.y;
                     // (u + w y, u - w y): four FMAs for the sum, the difference as 2u - sum
-                    const double2 u = v[t], y = v[t + (1 << e)];
                     const double sr = fma(w.x, y.x, fma(-w.y, y.y, u.x));
                     const double si = fma(w.x, y.y, fma(w.y, y.x, u.y));
                     v[t] = make_double2(sr, si);
