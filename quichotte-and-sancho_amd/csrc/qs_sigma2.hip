@@ -45,31 +45,76 @@ __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned
 // barriers.  The arithmetic is exactly that of the radix-2 levels.  Every size
 // is a compile-time constant, so the element addresses of a task are one base
 // plus immediate offsets.
-// Layout: element x of sequence q sits at q * ld + px(x), px(x) = x + x / 8,
-// ld = px(L) + 1.  The pad after every 8 elements puts the 8 lanes of the
-// first group (stride-8 blocks) in distinct banks, and the odd sequence stride
-// does the same for the column-interleaved staging accesses.
-__host__ __device__ constexpr int px(int x) { return x + (x >> 3); }
-// The twiddle table is read at strides 2^k (k up to 5) by the 8 lanes that share
-// an LDS cycle; entry m sits at tsw(m), which permutes each aligned group of 8 so
-// that those reads fall in distinct bank groups (tsw(m) = m for m < 8).
-__device__ __forceinline__ int tsw(int m) { return m ^ (((m >> 3) ^ (m >> 6)) & 7); }
-__host__ __device__ constexpr int seq_stride(int L) { return L + (L >> 3) + 1; }
+//
+// LDS banks (gfx950, measured with a one-wavefront probe of ds_read_b128 /
+// ds_write_b128): a 16-byte read is served in four phases of 16 lanes (lanes
+// 0-3, 12-15, 20-23, 24-27; the other 16 of lanes 0-31; the same two sets + 32)
+// over 16 slots of 16 bytes, a 16-byte write in eight phases of 8 consecutive
+// lanes over 8 slots; each further distinct slot that a phase puts on one bank
+// costs a cycle.
+// Layout: element x of sequence s sits at slot s * LD + (x ^ sx(x) ^ ss(s)).
+// sx XORs a few shifted bit fields of x into its low four bits; every field
+// comes from bits >= 3, and bits >= 4 where it changes bit 3, so x -> slot is
+// one-to-one inside each aligned run of 16, and sx is linear over the bits of x.
+// The fields, the sequence term and the pad of LD were chosen (by a search over
+// these kernels' access patterns under the bank model above) so that the
+// butterfly sweeps, the bit-reversed loads, the column-interleaved reads and
+// the row accesses of the middle pass put each phase on distinct banks (the
+// middle pass at N2 = 256 keeps about 64 conflict cycles per wavefront, against
+// about 2300 cycles of LDS work).
+struct LdsLayout { int a1, m1, a2, m2, a3, m3, k, pad; };
+__host__ __device__ constexpr LdsLayout lds_layout(int LG, int NSEQ) {
+    if (NSEQ == 12)  // the middle pass: 12 columns of one site pair
+        return LG >= 8 ? LdsLayout{3, 7, 4, 11, 5, 7, 5, 0} : LdsLayout{3, 7, 6, 11, 0, 0, 1, 8};
+    return LG == 7 ? LdsLayout{3, 7, 6, 3, 0, 0, 9, 0} : LdsLayout{3, 7, 4, 11, 0, 0, 1, 8};
+}
+template <int LG, int NSEQ>
+struct Lay {
+    static constexpr LdsLayout P = lds_layout(LG, NSEQ);
+    static constexpr int L = 1 << LG, LD = L + P.pad, M = L < 16 ? L - 1 : 15;
+    __host__ __device__ static constexpr int sx(int x) {
+        return (((x >> P.a1) & P.m1) ^ ((x >> P.a2) & P.m2) ^ ((x >> P.a3) & P.m3)) & M;
+    }
+    __host__ __device__ static constexpr int ss(int s) { return (s * P.k) & M; }
+    __host__ __device__ static constexpr int kx(int x) { return x ^ sx(x); }
+    __host__ __device__ static constexpr int at(int s, int x) { return s * LD + (kx(x) ^ ss(s)); }
+    static constexpr size_t bytes() { return ((size_t)L / 2 + (size_t)NSEQ * LD) * sizeof(double2); }
+};
+// x -> s LD + (kx(x) ^ ss(s)) maps [0, L) one-to-one onto a run of L slots
+template <int LG, int NSEQ>
+constexpr bool lay_one_to_one() {
+    using Y = Lay<LG, NSEQ>;
+    for (int s = 0; s < NSEQ; s++) {
+        bool seen[Y::L] = {};
+        for (int x = 0; x < Y::L; x++) {
+            const int v = Y::kx(x) ^ Y::ss(s);
+            if (v < 0 || v >= Y::L || seen[v]) return false;
+            seen[v] = true;
+        }
+    }
+    return true;
+}
+// The twiddle table is read at strides 2^k by lanes whose tasks differ in j;
+// entry m sits at tsw(m), which keeps those reads on distinct banks.
+__device__ __forceinline__ int tsw(int m) { return m ^ ((m >> 4) & 15); }
 
 template <int G, int LG, int S0, int NSEQ, int SIGN>
 __device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
+    using Y = Lay<LG, NSEQ>;
     constexpr int T = 1 << G, LT = LG - G, LH = S0 - 1, H = 1 << LH;  // tasks per sequence 2^LT, element stride H
-    constexpr int TOTAL = NSEQ << LT, LD = seq_stride(1 << LG);
-    // px(x0 + t H) = px(x0) + t (H + H / 8): x0 is a multiple of 8 when H = 1
-    constexpr int STEP = H + (H >> 3);
+    constexpr int TOTAL = NSEQ << LT;
 #pragma unroll 1
     for (int task = threadIdx.x; task < TOTAL; task += NT) {
         const int seq = task >> LT, r = task & ((1 << LT) - 1);
         const int blk = r >> LH, j = r & (H - 1);
-        double2 *b = buf + seq * LD + px((blk << (LH + G)) + j);
+        // x0 and t H have no bit in common and sx is linear, so element x0 + t H
+        // sits at seq LD + (o ^ kx(t H))
+        const int x0 = (blk << (LH + G)) + j;
+        const int o = Y::kx(x0) ^ Y::ss(seq);
+        double2 *b = buf + seq * Y::LD;
         double2 v[T];
 #pragma unroll
-        for (int t = 0; t < T; t++) v[t] = b[t * STEP];
+        for (int t = 0; t < T; t++) v[t] = b[o ^ Y::kx(t << LH)];
 #pragma unroll
         for (int e = 0; e < G; e++) {
             // radix-2 level S0 + e, half-length H 2^e: w = e^{-2 pi i m / (2 H 2^e)} = twl[m L / (2 H 2^e)]
@@ -110,7 +155,7 @@ __device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
                 }
         }
 #pragma unroll
-        for (int t = 0; t < T; t++) b[t * STEP] = v[t];
+        for (int t = 0; t < T; t++) b[o ^ Y::kx(t << LH)] = v[t];
     }
     __syncthreads();
 }
@@ -183,7 +228,8 @@ template <int LG1>
 __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, double2 *__restrict__ Z,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    constexpr int N1 = 1 << LG1, LD = seq_stride(N1), HALF = CB * N1 / 2;
+    using Y = Lay<LG1, CB>;
+    constexpr int N1 = 1 << LG1, HALF = CB * N1 / 2;
     constexpr int QS = (CB * N1 + NT - 1) / NT, QI = (HALF + NT - 1) / NT;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = p.ncol, n2 = blockIdx.y, nblk = ncol / CB;
@@ -205,7 +251,7 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
 #pragma unroll
         for (int q = 0; q < QS; q++) {
             const int t = threadIdx.x + q * NT;
-            if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = q < QI ? v[q] : make_double2(0.0, 0.0);
+            if (t < CB * N1) buf[Y::at(t & 7, bitrev(t >> 3, LG1))] = q < QI ? v[q] : make_double2(0.0, 0.0);
         }
         __syncthreads();
         if (cb + 1 < cb1) fetch(cb + 1, v);
@@ -222,7 +268,7 @@ __global__ __launch_bounds__(NT) void k_fft_f1(const double2 *__restrict__ Gt, d
             const int t = threadIdx.x + i * NT;
             if (t < CB * N1) {
                 const int col = t & 7, k1 = t >> 3;
-                Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[col * LD + px(k1)], w[i]);
+                Z[((size_t)k1 * p.N2 + n2) * ncol + cb * CB + col] = cmul(buf[Y::at(col, k1)], w[i]);
             }
         }
         __syncthreads();  // the next block overwrites buf
@@ -309,9 +355,10 @@ template <int LG2>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_fft_mid(double2 *__restrict__ Z,
                                                                                           Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    constexpr int N2 = 1 << LG2, ld = seq_stride(N2);  // N2 <= NT
+    using Y = Lay<LG2, PS>;
+    constexpr int N2 = 1 << LG2;  // N2 <= NT
     const int N1 = p.N1, nAs = p.nAs, ncol = p.ncol;
-    double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: [s (12)][ld]
+    double2 *twl = (double2 *)lds_raw, *buf = twl + N2 / 2;  // buf: 12 sequences of N2, slots Y::at(s, x)
     // Work item w = (k1, pair), pair fastest.  Workgroup L of the dispatch runs on
     // XCD L % 8, so the items are dealt out in 8 contiguous ranges: neighbouring
     // pairs, whose 192-byte column groups share 128-byte lines, then meet in one
@@ -353,7 +400,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 #pragma unroll
         for (int q = 0; q < PS; q++) {
             const int e = threadIdx.x + q * NT;
-            if (e < PS * N2) buf[(e % PS) * ld + px(bitrev(e / PS, LG2))] = pre[q];
+            if (e < PS * N2) buf[Y::at(e % PS, bitrev(e / PS, LG2))] = pre[q];
         }
     };
     // twiddle e^{+2 pi i row n2 / Nf} between the inverse row DFT and the
@@ -370,7 +417,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             const int e = threadIdx.x + q * NT;
             if (e < PS * N2) {
                 const int s = e % PS, n2 = e / PS;
-                Z[((size_t)row * N2 + n2) * ncol + cb + s] = cmul(buf[s * ld + px(n2)], w[q]);
+                Z[((size_t)row * N2 + n2) * ncol + cb + s] = cmul(buf[Y::at(s, n2)], w[q]);
             }
         }
     };
@@ -392,7 +439,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         double2 A[PS];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < PS; s++) A[s] = buf[s * ld + px(t)];
+            for (int s = 0; s < PS; s++) A[s] = buf[Y::at(s, t)];
         __syncthreads();
         put_row();
         __syncthreads();
@@ -401,14 +448,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         double2 Q[PS];
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < PS; s++) Q[s] = buf[s * ld + px(m2)];
+            for (int s = 0; s < PS; s++) Q[s] = buf[Y::at(s, m2)];
         __syncthreads();
         if (t < N2) {
             double2 O[PS];
             pair_out(Q, A, O);
-            const int bm = px(bitrev(m2, LG2));
+            const int bm = bitrev(m2, LG2);
 #pragma unroll
-            for (int s = 0; s < PS; s++) buf[s * ld + bm] = O[s];
+            for (int s = 0; s < PS; s++) buf[Y::at(s, bm)] = O[s];
             pair_out(A, Q, O);
 #pragma unroll
             for (int s = 0; s < PS; s++) A[s] = O[s];
@@ -419,7 +466,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, LG2))] = A[s];
+            for (int s = 0; s < PS; s++) buf[Y::at(s, bitrev(t, LG2))] = A[s];
         __syncthreads();
     } else {
         // k1 = 0 or N1/2: the partner position is in the same row
@@ -428,13 +475,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         if (t < N2) {
             double2 P[PS], Q[PS];
 #pragma unroll
-            for (int s = 0; s < PS; s++) { P[s] = buf[s * ld + px(t)]; Q[s] = buf[s * ld + px(m2)]; }
+            for (int s = 0; s < PS; s++) { P[s] = buf[Y::at(s, t)]; Q[s] = buf[Y::at(s, m2)]; }
             pair_out(P, Q, O);
         }
         __syncthreads();
         if (t < N2)
 #pragma unroll
-            for (int s = 0; s < PS; s++) buf[s * ld + px(bitrev(t, LG2))] = O[s];
+            for (int s = 0; s < PS; s++) buf[Y::at(s, bitrev(t, LG2))] = O[s];
         __syncthreads();
     }
     lds_fft<LG2, PS, +1>(buf, twl);
@@ -447,7 +494,8 @@ template <int LG1>
 __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, double2 *__restrict__ Sig,
                                                Sigma2Plan p) {
     extern __shared__ __align__(16) char lds_raw[];
-    constexpr int N1 = 1 << LG1, LD = seq_stride(N1), QS = (CB * N1 + NT - 1) / NT;
+    using Y = Lay<LG1, CB>;
+    constexpr int N1 = 1 << LG1, QS = (CB * N1 + NT - 1) / NT;
     double2 *twl = (double2 *)lds_raw, *buf = twl + N1 / 2;
     const int ncol = p.ncol, n2 = blockIdx.y, nblk = ncol / CB;
     const int cb0 = blockIdx.x * BPW, cb1 = min(cb0 + BPW, nblk);
@@ -467,7 +515,7 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 #pragma unroll
         for (int q = 0; q < QS; q++) {
             const int t = threadIdx.x + q * NT;
-            if (t < CB * N1) buf[(t & 7) * LD + px(bitrev(t >> 3, LG1))] = v[q];
+            if (t < CB * N1) buf[Y::at(t & 7, bitrev(t >> 3, LG1))] = v[q];
         }
         __syncthreads();
         if (cb + 1 < cb1) fetch(cb + 1, v);
@@ -476,7 +524,7 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
         for (int i = 0; i < QS; i++) {
             const int t = threadIdx.x + i * NT;
             const int col = t & 7, n1 = t >> 3, n = n2 + p.N2 * n1;
-            if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[col * LD + px(n1)];
+            if (t < CB * N1 && n < p.nw) Sig[(size_t)n * ncol + cb * CB + col] = buf[Y::at(col, n1)];
         }
         __syncthreads();  // the next block overwrites buf
     }
@@ -500,7 +548,8 @@ hipError_t launch_f1(const Sigma2Plan &p, const double2 *Gt, double2 *Z, hipStre
         (void)hipFuncSetAttribute((const void *)k_fft_f1<LG1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    const size_t lds = ((size_t)(1 << LG1) / 2 + (size_t)CB * seq_stride(1 << LG1)) * sizeof(double2);
+    static_assert(lay_one_to_one<LG1, CB>(), "LDS layout");
+    const size_t lds = Lay<LG1, CB>::bytes();
     hipLaunchKernelGGL(k_fft_f1<LG1>, dim3((p.ncol / CB + BPW - 1) / BPW, p.N2), dim3(NT), lds, s, Gt, Z, p);
     return hipGetLastError();
 }
@@ -511,7 +560,8 @@ hipError_t launch_i2(const Sigma2Plan &p, const double2 *Z, double2 *Sig, hipStr
         (void)hipFuncSetAttribute((const void *)k_fft_i2<LG1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    const size_t lds = ((size_t)(1 << LG1) / 2 + (size_t)CB * seq_stride(1 << LG1)) * sizeof(double2);
+    static_assert(lay_one_to_one<LG1, CB>(), "LDS layout");
+    const size_t lds = Lay<LG1, CB>::bytes();
     hipLaunchKernelGGL(k_fft_i2<LG1>, dim3((p.ncol / CB + BPW - 1) / BPW, p.N2), dim3(NT), lds, s, Z, Sig, p);
     return hipGetLastError();
 }
@@ -522,7 +572,8 @@ hipError_t launch_mid(const Sigma2Plan &p, double2 *Z, hipStream_t s) {
         (void)hipFuncSetAttribute((const void *)k_fft_mid<LG2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    const size_t lds = ((size_t)(1 << LG2) / 2 + (size_t)PS * seq_stride(1 << LG2)) * sizeof(double2);
+    static_assert(lay_one_to_one<LG2, PS>(), "LDS layout");
+    const size_t lds = Lay<LG2, PS>::bytes();
     const int npairs = p.nAs * (p.nAs + 1) / 2;
     hipLaunchKernelGGL(k_fft_mid<LG2>, dim3(npairs, p.N1 / 2 + 1), dim3(NT), lds, s, Z, p);
     return hipGetLastError();
