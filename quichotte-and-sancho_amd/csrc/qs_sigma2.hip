@@ -110,11 +110,13 @@ __device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
         // x0 and t H have no bit in common and sx is linear, so element x0 + t H
         // sits at seq LD + (o ^ kx(t H))
         const int x0 = (blk << (LH + G)) + j;
-        const int o = Y::kx(x0) ^ Y::ss(seq);
-        double2 *b = buf + seq * Y::LD;
+        // byte offsets, so that each element address is one XOR (and add)
+        const int o16 = (Y::kx(x0) ^ Y::ss(seq)) << 4;
+        char *b = (char *)(buf + seq * Y::LD);
+        auto el = [&](int t) -> double2 & { return *(double2 *)(b + (o16 ^ (Y::kx(t << LH) << 4))); };
         double2 v[T];
 #pragma unroll
-        for (int t = 0; t < T; t++) v[t] = b[o ^ Y::kx(t << LH)];
+        for (int t = 0; t < T; t++) v[t] = el(t);
 #pragma unroll
         for (int e = 0; e < G; e++) {
             // radix-2 level S0 + e, half-length H 2^e: w = e^{-2 pi i m / (2 H 2^e)} = twl[m L / (2 H 2^e)]
@@ -155,7 +157,7 @@ __device__ __forceinline__ void fft_group(double2 *buf, const double2 *twl) {
                 }
         }
 #pragma unroll
-        for (int t = 0; t < T; t++) b[o ^ Y::kx(t << LH)] = v[t];
+        for (int t = 0; t < T; t++) el(t) = v[t];
     }
     __syncthreads();
 }
@@ -386,40 +388,55 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     load_twl<N2>(twl, p.tw, p.Nf);
     // A row is PS N2 <= PS NT values: one round of loads into registers, then
     // into LDS at bit-reversed positions.  Row k1p is fetched while row k1 is
-    // transformed.
-    double2 pre[PS];
+    // transformed.  The row goes in two passes of power-of-two shape: columns
+    // 0..7 with 8 lanes per n2, then columns 8..11 with 4 lanes per n2.  A lane
+    // keeps its column, and its n2 = n2a + (NT / 8) u (or n2b + (NT / 4) u) has
+    // the lane part and the step in disjoint bits; bit reversal and the LDS
+    // swizzle are linear over those bits, so every slot is a per-lane base XOR a
+    // compile-time constant, and every global offset a per-lane base plus a
+    // uniform step.
+    constexpr int NA = (8 * N2 + NT - 1) / NT, NB = (4 * N2 + NT - 1) / NT, NE = NA + NB;  // NE <= PS
+    const int sa = t & 7, n2a = t >> 3, sb = 8 + (t & 3), n2b = t >> 2;
+    const int oga = n2a * ncol + sa, ogb = n2b * ncol + sb;  // offsets in a row of Z from its column base
+    const int lra = sa * Y::LD * 16, lrb = sb * Y::LD * 16;  // byte offsets of the lane's sequence in buf
+    const int pba = (Y::kx(bitrev(n2a, LG2)) ^ Y::ss(sa)) << 4, pbb = (Y::kx(bitrev(n2b, LG2)) ^ Y::ss(sb)) << 4;
+    const int nba = (Y::kx(n2a) ^ Y::ss(sa)) << 4, nbb = (Y::kx(n2b) ^ Y::ss(sb)) << 4;
+    char *bufc = (char *)buf;
+    auto e_n2 = [&](int u) { return u < NA ? n2a + (NT / 8) * u : n2b + (NT / 4) * (u - NA); };
+    auto e_og = [&](int u) { return u < NA ? oga + (NT / 8) * u * ncol : ogb + (NT / 4) * (u - NA) * ncol; };
+    // slot of element u at bit-reversed (put) or natural (store) position
+    auto e_put = [&](int u) -> double2 & {
+        return u < NA ? *(double2 *)(bufc + lra + (pba ^ (Y::kx(bitrev((NT / 8) * u, LG2)) << 4)))
+                      : *(double2 *)(bufc + lrb + (pbb ^ (Y::kx(bitrev((NT / 4) * (u - NA), LG2)) << 4)));
+    };
+    auto e_nat = [&](int u) -> double2 & {
+        return u < NA ? *(double2 *)(bufc + lra + (nba ^ (Y::kx((NT / 8) * u) << 4)))
+                      : *(double2 *)(bufc + lrb + (nbb ^ (Y::kx((NT / 4) * (u - NA)) << 4)));
+    };
+    double2 pre[NE];
     auto fetch_row = [&](int row) {
+        const double2 *Zr = Z + (size_t)row * N2 * ncol + cb;
 #pragma unroll
-        for (int q = 0; q < PS; q++) {
-            const int e = threadIdx.x + q * NT;
-            pre[q] = make_double2(0.0, 0.0);
-            if (e < PS * N2) pre[q] = Z[((size_t)row * N2 + e / PS) * ncol + cb + e % PS];
+        for (int u = 0; u < NE; u++) {
+            pre[u] = make_double2(0.0, 0.0);
+            if (e_n2(u) < N2) pre[u] = Zr[e_og(u)];
         }
     };
     auto put_row = [&]() {
 #pragma unroll
-        for (int q = 0; q < PS; q++) {
-            const int e = threadIdx.x + q * NT;
-            if (e < PS * N2) buf[Y::at(e % PS, bitrev(e / PS, LG2))] = pre[q];
-        }
+        for (int u = 0; u < NE; u++)
+            if (e_n2(u) < N2) e_put(u) = pre[u];
     };
     // twiddle e^{+2 pi i row n2 / Nf} between the inverse row DFT and the
     // inverse pass over k1
     auto store_row = [&](int row) {
-        double2 w[PS];
+        double2 w[NE];
 #pragma unroll
-        for (int q = 0; q < PS; q++) {
-            const int n2 = (threadIdx.x + q * NT) / PS;
-            w[q] = n2 < N2 ? cconj(p.tw[row * n2]) : make_double2(0.0, 0.0);
-        }
+        for (int u = 0; u < NE; u++) w[u] = e_n2(u) < N2 ? cconj(p.tw[row * e_n2(u)]) : make_double2(0.0, 0.0);
+        double2 *Zr = Z + (size_t)row * N2 * ncol + cb;
 #pragma unroll
-        for (int q = 0; q < PS; q++) {
-            const int e = threadIdx.x + q * NT;
-            if (e < PS * N2) {
-                const int s = e % PS, n2 = e / PS;
-                Z[((size_t)row * N2 + n2) * ncol + cb + s] = cmul(buf[Y::at(s, n2)], w[q]);
-            }
-        }
+        for (int u = 0; u < NE; u++)
+            if (e_n2(u) < N2) Zr[e_og(u)] = cmul(e_nat(u), w[u]);
     };
     // outputs at position x from its values X (at +l) and the partner's Y (at -l):
     // [Sigma^r_IJ, Sigma^<_IJ, Sigma^r_JI], src/self_energies.f90:175-198
