@@ -221,6 +221,57 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
     }
 }
 
+// Same gather with the energy's slice staged in LDS (3 nA^2 doubles; nA <= 82
+// fits one CU's 160 KB): one 1024-thread workgroup per CU, persistent over the
+// energies.  The slice comes in with 16-byte loads that read each line of x
+// once, the gather runs from LDS, and Gt rows go out as contiguous 16-byte
+// stores.  The next energy's loads are in flight during the current gather;
+// each thread keeps its columns' source pairs in registers for every energy.
+constexpr int PL_NT = 1024, PL_QL = 10, PL_QC = 11;  // slice <= PL_QL NT double2, ncol <= PL_QC NT
+__global__ __launch_bounds__(PL_NT) void k_permute_lds(const double *__restrict__ x, double2 *__restrict__ Gt,
+                                                       const int2 *__restrict__ src, int nA, int ncol, int nw) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double *xs = (double *)lds_raw;
+    const int tid = threadIdx.x, nh = 3 * nA * nA / 2;  // double2 per energy (nA is even)
+    int2 sc[PL_QC];
+#pragma unroll
+    for (int u = 0; u < PL_QC; u++) {
+        const int c = tid + PL_NT * u;
+        sc[u] = c < ncol ? src[c] : make_int2(0, -1);
+    }
+    double2 v[PL_QL];
+    auto fetch = [&](int iw) {
+        const double2 *xb = (const double2 *)x + (size_t)min(iw, nw - 1) * nh;
+#pragma unroll
+        for (int q = 0; q < PL_QL; q++) {
+            const int i = tid + PL_NT * q;
+            v[q] = i < nh ? xb[i] : make_double2(0.0, 0.0);
+        }
+    };
+    fetch(blockIdx.x);
+    for (int iw = blockIdx.x; iw < nw; iw += gridDim.x) {
+#pragma unroll
+        for (int q = 0; q < PL_QL; q++) {
+            const int i = tid + PL_NT * q;
+            if (i < nh) ((double2 *)xs)[i] = v[q];
+        }
+        __syncthreads();
+        if (iw + (int)gridDim.x < nw) fetch(iw + gridDim.x);
+        double2 *g = Gt + (size_t)iw * ncol;
+#pragma unroll
+        for (int u = 0; u < PL_QC; u++) {
+            const int c = tid + PL_NT * u;
+            // real part +/- xs[|sc.x| - 1] (0: zero), imaginary part xs[sc.y] (< 0: zero)
+            const int ir = sc[u].x > 0 ? sc[u].x - 1 : (sc[u].x < 0 ? -sc[u].x - 1 : 0);
+            const int ii = sc[u].y >= 0 ? sc[u].y : 0;
+            const double xr = xs[ir], xi = xs[ii];
+            const double re = sc[u].x > 0 ? xr : (sc[u].x < 0 ? -xr : 0.0);
+            if (c < ncol) g[c] = make_double2(re, sc[u].y >= 0 ? xi : 0.0);
+        }
+        __syncthreads();  // the next slice overwrites xs
+    }
+}
+
 // ---- F1 -----------------------------------------------------------------------
 // grid: (ncol / (CB BPW), N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
 // Thread work is unrolled over constant bounds (QS = CB N1 / NT elements per
@@ -551,6 +602,23 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
                                 hipStream_t s) {
+    const size_t slice = (size_t)3 * nA * nA * sizeof(double);
+    if (nA % 2 == 0 && slice <= 160 * 1024 && 3 * nA * nA / 2 <= PL_QL * PL_NT && ncol <= PL_QC * PL_NT) {
+        static int ncu = 0;
+        if (!ncu) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+                return hipErrorInvalidValue;
+            hipError_t e = hipFuncSetAttribute((const void *)k_permute_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024);
+            if (e != hipSuccess) return e;
+        }
+        const int blocks = nw < ncu ? nw : ncu;  // one per CU (the slice takes most of its LDS)
+        hipLaunchKernelGGL(k_permute_lds, dim3(blocks), dim3(PL_NT), slice, s, x, Gt, (const int2 *)blk_src, nA, ncol,
+                           nw);
+        return hipGetLastError();
+    }
     const int groups = (nw + EPW - 1) / EPW, blocks = groups < 65536 ? groups : 65536;
     hipLaunchKernelGGL(k_permute_in, dim3(blocks), dim3(NT), 0, s, x, Gt, (const int2 *)blk_src, nA, ncol, nw);
     return hipGetLastError();
