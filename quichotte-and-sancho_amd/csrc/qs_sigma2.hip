@@ -32,7 +32,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int CB = 8;   // columns per block of the passes over n1 / k1
-constexpr int BPW = 4;  // column blocks per workgroup in those passes
+constexpr int BPW = 8;  // column blocks per workgroup in those passes
 
 __device__ __forceinline__ int bitrev(int v, int bits) { return __brev((unsigned)v) >> (32 - bits); }
 
