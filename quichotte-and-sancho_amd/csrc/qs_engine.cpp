@@ -549,7 +549,8 @@ int tune_dyson(qs_handle *h) {
 }
 
 // scf_residual2 on the device: d_x -> d_F; stats[3] = max|F|, |F|^2, |F-Fold|^2
-int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, Sanity *san) {
+// gt_ready: Gt already holds x in the FFT input layout (written by the fused Broyden update)
+int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, Sanity *san, bool gt_ready = false) {
     const int nA = h->nA;
     h->V_scf = V;
     const double pp = h->p.delta / (2.0 * PI);
@@ -557,9 +558,11 @@ int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, S
     HIPCHK(h, qs_launch_equal_time_sigma1(h->d_x, h->d_et_src, h->d_act_site, h->d_U_act, h->d_gt0, h->d_sig1, nA,
                                           h->nAs, h->nw, pp, h->d_scratch, h->stream));
     prof_end(h, ST_EQT, t0);
-    t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, nA, h->ncolS, h->nw, h->stream));
-    prof_end(h, ST_PERMUTE, t0);
+    if (!gt_ready) {
+        t0 = prof_begin(h);
+        HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, nA, h->ncolS, h->nw, h->stream));
+        prof_end(h, ST_PERMUTE, t0);
+    }
     Sigma2Plan sp{};
     sp.nA = nA; sp.nAs = h->nAs; sp.nw = h->nw; sp.Nf = h->Nf; sp.N1 = h->N1; sp.N2 = h->N2;
     sp.log2N1 = h->log2N1; sp.log2N2 = h->log2N2; sp.delta = h->p.delta; sp.tw = h->d_tw; sp.Upair = h->d_Upair;
@@ -747,11 +750,13 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
     int nstored = 0, nh = 0, rc;
     double fmax = 1.79769313486231570e308;
     *converged = 0;
+    bool gt_ready = false;  // the last update also wrote the FFT input of the next residual
     for (int its = 1; its <= h->p.maxscf; its++) {
         *n_iter = its;
         double st[3];
         int ab = 0;
-        if ((rc = residual2_device(h, V, st, &ab, last))) return rc;
+        if ((rc = residual2_device(h, V, st, &ab, last, gt_ready))) return rc;
+        gt_ready = false;
         if (ab) break;
         fmax = st[0];
         const double rms = std::sqrt(st[1]) / std::sqrt((double)n);
@@ -807,13 +812,21 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
             }
         }
         hipEvent_t t1 = prof_begin(h);
-        if (sp)
+        // Gt shares memory with Sigma2, which the final sweep reads after the last
+        // step: the step that can end the loop at maxscf keeps the plain update
+        const bool fuse = sp && its < h->p.maxscf && qs_update_gt_fits(h->nA, h->ncolS, n, nh_use);
+        if (fuse)
+            HIPCHK(h, qs_launch_broyden_update_f32_gt(n, h->d_x, h->d_F, (const float *)h->d_hist_U, gam.data(), nh_use,
+                                                      alpha, h->d_xold, h->d_Gt, h->d_blk_src, h->nA, h->ncolS,
+                                                      h->nw, h->stream));
+        else if (sp)
             HIPCHK(h, qs_launch_broyden_update_f32(n, h->d_x, h->d_F, (const float *)h->d_hist_U, gam.data(), nh_use,
                                                    alpha, h->d_xold, h->stream));
         else
             HIPCHK(h, qs_launch_broyden_update_f64(n, h->d_x, h->d_F, (const double *)h->d_hist_U, gam.data(), nh_use,
                                                    alpha, h->d_xold, h->stream));
         prof_end(h, ST_UPDATE, t1);
+        gt_ready = fuse;
         std::swap(h->d_x, h->d_xold);  // x <- x_new, xold <- x
         std::swap(h->d_F, h->d_Fold);  // Fold <- F
     }

@@ -77,6 +77,12 @@ struct Sigma2Plan {
 // x (packed, reference layout) -> Gt [nw][ncol] site-pair layout
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
                                 hipStream_t s);
+// Broyden update (FP32 history) that also writes the next residual's Gt from x_new;
+// qs_update_gt_fits tells whether the shape allows it (energy slice in LDS, nh <= 16)
+bool qs_update_gt_fits(int nA, int ncol, int64_t n, int nh);
+hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const double *F, const float *Us,
+                                           const double *gamma, int nh, double alpha, double *xnew, double2 *Gt,
+                                           const int *blk_src, int nA, int ncol, int nw, hipStream_t s);
 // parts: bit 0 forward pass over n1, bit 1 middle pass (row DFTs + 2x2 algebra),
 // bit 2 inverse pass over k1; the three in order make one Sigma2 evaluation.
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,

@@ -272,6 +272,67 @@ __global__ __launch_bounds__(PL_NT) void k_permute_lds(const double *__restrict_
     }
 }
 
+// Broyden update fused with the gather of the next residual's FFT input:
+// x_new = x + alpha F - sum_k gamma_k U_k over the FP32 history (the arithmetic
+// of the plain update kernel, so x_new is the same to the bit), computed on whole
+// energy slices; each slice also goes to LDS and out to Gt as k_permute_lds
+// writes it.  One workgroup per CU, persistent over the energies.
+struct GamH { double g[16]; };
+__global__ __launch_bounds__(PL_NT) void k_update_f32_gt(int64_t n, const double *__restrict__ x,
+                                                         const double *__restrict__ F, const float *__restrict__ Us,
+                                                         GamH gm, int nh, double alpha, double *__restrict__ xnew,
+                                                         double2 *__restrict__ Gt, const int2 *__restrict__ src,
+                                                         int nA, int ncol, int nw) {
+    extern __shared__ __align__(16) char lds_raw[];
+    double *xs = (double *)lds_raw;
+    const int tid = threadIdx.x, nb = 3 * nA * nA, nq = nb / 4;  // 4 | nb (nA is even)
+    for (int iw = blockIdx.x; iw < nw; iw += gridDim.x) {
+        // the energy's slices from uniform bases, lane offsets 32-bit
+        const int64_t base = (int64_t)iw * nb;
+        const double4 *xb = (const double4 *)(x + base), *fb = (const double4 *)(F + base);
+        double4 *xnb = (double4 *)(xnew + base);
+        for (int q = tid; q < nq; q += PL_NT) {
+            const double4 xv = xb[q];
+            const double4 f = fb[q];
+            float4 hv[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (k < nh) hv[k] = ((const float4 *)(Us + (size_t)k * n + base))[q];
+            double v0 = xv.x + alpha * f.x, v1 = xv.y + alpha * f.y, v2 = xv.z + alpha * f.z, v3 = xv.w + alpha * f.w;
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (k < nh) {
+                    v0 = v0 - (double)hv[k].x * gm.g[k];
+                    v1 = v1 - (double)hv[k].y * gm.g[k];
+                    v2 = v2 - (double)hv[k].z * gm.g[k];
+                    v3 = v3 - (double)hv[k].w * gm.g[k];
+                }
+            xnb[q] = make_double4(v0, v1, v2, v3);
+            ((double2 *)xs)[2 * q] = make_double2(v0, v1);
+            ((double2 *)xs)[2 * q + 1] = make_double2(v2, v3);
+        }
+        __syncthreads();
+        double2 *g = Gt + (size_t)iw * ncol;
+        // the column map comes from L2 each time: registers go to the history loads
+        int2 sc[PL_QC];
+#pragma unroll
+        for (int u = 0; u < PL_QC; u++) {
+            const int c = tid + PL_NT * u;
+            sc[u] = c < ncol ? src[c] : make_int2(0, -1);
+        }
+#pragma unroll
+        for (int u = 0; u < PL_QC; u++) {
+            const int c = tid + PL_NT * u;
+            const int ir = sc[u].x > 0 ? sc[u].x - 1 : (sc[u].x < 0 ? -sc[u].x - 1 : 0);
+            const int ii = sc[u].y >= 0 ? sc[u].y : 0;
+            const double xr = xs[ir], xi = xs[ii];
+            const double re = sc[u].x > 0 ? xr : (sc[u].x < 0 ? -xr : 0.0);
+            if (c < ncol) g[c] = make_double2(re, sc[u].y >= 0 ? xi : 0.0);
+        }
+        __syncthreads();  // the next slice overwrites xs
+    }
+}
+
 // ---- F1 -----------------------------------------------------------------------
 // grid: (ncol / (CB BPW), N2).  LDS: twiddles (N1 / 2) | CB padded sequences.
 // Thread work is unrolled over constant bounds (QS = CB N1 / NT elements per
@@ -600,19 +661,60 @@ __global__ __launch_bounds__(NT) void k_fft_i2(const double2 *__restrict__ Z, do
 
 }  // namespace
 
+namespace {
+// an energy's packed slice fits the LDS-staged gather
+bool slice_fits_lds(int nA, int ncol) {
+    const size_t slice = (size_t)3 * nA * nA * sizeof(double);
+    return nA % 2 == 0 && slice <= 160 * 1024 && 3 * nA * nA / 2 <= PL_QL * PL_NT && ncol <= PL_QC * PL_NT;
+}
+// compute units of the current device (0 on failure); the persistent kernels take one workgroup per CU
+int cu_count() {
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            ncu = v;
+    }
+    return ncu;
+}
+}  // namespace
+
+bool qs_update_gt_fits(int nA, int ncol, int64_t n, int nh) { return slice_fits_lds(nA, ncol) && n % 4 == 0 && nh <= 16; }
+
+hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const double *F, const float *Us,
+                                           const double *gamma, int nh, double alpha, double *xnew, double2 *Gt,
+                                           const int *blk_src, int nA, int ncol, int nw, hipStream_t s) {
+    if (!qs_update_gt_fits(nA, ncol, n, nh) || (int64_t)3 * nA * nA * nw != n) return hipErrorInvalidValue;
+    const int ncu = cu_count();
+    if (ncu <= 0) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_update_f32_gt, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    GamH gm;
+    for (int k = 0; k < 16; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
+    const int blocks = nw < ncu ? nw : ncu;
+    hipLaunchKernelGGL(k_update_f32_gt, dim3(blocks), dim3(PL_NT), (size_t)3 * nA * nA * sizeof(double), s, n, x, F,
+                       Us, gm, nh, alpha, xnew, Gt, (const int2 *)blk_src, nA, ncol, nw);
+    return hipGetLastError();
+}
+
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
                                 hipStream_t s) {
     const size_t slice = (size_t)3 * nA * nA * sizeof(double);
-    if (nA % 2 == 0 && slice <= 160 * 1024 && 3 * nA * nA / 2 <= PL_QL * PL_NT && ncol <= PL_QC * PL_NT) {
-        static int ncu = 0;
-        if (!ncu) {
-            int dev = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
-                return hipErrorInvalidValue;
+    if (slice_fits_lds(nA, ncol)) {
+        const int ncu = cu_count();
+        if (ncu <= 0) return hipErrorInvalidValue;
+        static bool attr = false;
+        if (!attr) {
             hipError_t e = hipFuncSetAttribute((const void *)k_permute_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                160 * 1024);
             if (e != hipSuccess) return e;
+            attr = true;
         }
         const int blocks = nw < ncu ? nw : ncu;  // one per CU (the slice takes most of its LDS)
         hipLaunchKernelGGL(k_permute_lds, dim3(blocks), dim3(PL_NT), slice, s, x, Gt, (const int2 *)blk_src, nA, ncol,
