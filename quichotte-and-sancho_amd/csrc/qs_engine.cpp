@@ -90,6 +90,7 @@ struct qs_handle {
     double2 *d_tw = nullptr;
     // device state
     double *d_x = nullptr, *d_F = nullptr, *d_Fold = nullptr, *d_xold = nullptr;
+    double *d_dx = nullptr;  // optional: x - xold from the fused Broyden update (FP32 history)
     void *d_hist_dF = nullptr, *d_hist_U = nullptr;
     double2 *d_Gt = nullptr, *d_Z = nullptr;  // Gt doubles as the Sigma2 output
     int ncolS = 0;                            // site-pair layout columns: 12 per unordered active site pair
@@ -421,6 +422,21 @@ int setup_device(qs_handle *h) {
         }
         if ((rc = dalloc(h, &h->d_Gt, (size_t)nw * h->ncolS))) return rc;
         if ((rc = dalloc(h, &h->d_Z, (size_t)h->Nf * h->ncolS))) return rc;
+        // x - xold for the history pass (saves it reading x and xold): only where the
+        // fused update runs, and only with 4 GB of device memory to spare after it
+        if (h->p.single_history && nhist > 0 && qs_update_gt_fits(h->nA, h->ncolS, h->n_vec, std::min(nhist, 16))) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > nv * sizeof(double) + ((size_t)4 << 30)) {
+                void *p = nullptr;
+                if (hipMalloc(&p, nv * sizeof(double)) == hipSuccess) {
+                    h->allocs.push_back(p);
+                    h->dev_bytes += (int64_t)(nv * sizeof(double));
+                    h->d_dx = (double *)p;
+                } else {
+                    (void)hipGetLastError();
+                }
+            }
+        }
         if ((rc = dalloc(h, &h->d_gt0, (size_t)4 * h->nAs))) return rc;
     }
     HIPCHK(h, hipDeviceSynchronize());
@@ -751,6 +767,7 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
     double fmax = 1.79769313486231570e308;
     *converged = 0;
     bool gt_ready = false;  // the last update also wrote the FFT input of the next residual
+    bool dx_ready = false;  // ... and x - xold into d_dx
     for (int its = 1; its <= h->p.maxscf; its++) {
         *n_iter = its;
         double st[3];
@@ -778,7 +795,8 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
         if (need_proj) {
             hipEvent_t t0 = prof_begin(h);
             if (sp)
-                HIPCHK(h, qs_launch_broyden_hist_f32(n, h->d_F, h->d_Fold, h->d_x, h->d_xold, slot >= 0 ? 1.0 / fnorm : 0.0,
+                HIPCHK(h, qs_launch_broyden_hist_f32(n, h->d_F, h->d_Fold, h->d_x, h->d_xold,
+                                                     dx_ready ? h->d_dx : nullptr, slot >= 0 ? 1.0 / fnorm : 0.0,
                                                      alpha, (float *)h->d_hist_dF, (float *)h->d_hist_U, slot, nh,
                                                      h->d_scratch, nblocks, h->stream));
             else
@@ -817,8 +835,8 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
         const bool fuse = sp && its < h->p.maxscf && qs_update_gt_fits(h->nA, h->ncolS, n, nh_use);
         if (fuse)
             HIPCHK(h, qs_launch_broyden_update_f32_gt(n, h->d_x, h->d_F, (const float *)h->d_hist_U, gam.data(), nh_use,
-                                                      alpha, h->d_xold, h->d_Gt, h->d_blk_src, h->nA, h->ncolS,
-                                                      h->nw, h->stream));
+                                                      alpha, h->d_xold, h->d_dx, h->d_Gt, h->d_blk_src, h->nA,
+                                                      h->ncolS, h->nw, h->stream));
         else if (sp)
             HIPCHK(h, qs_launch_broyden_update_f32(n, h->d_x, h->d_F, (const float *)h->d_hist_U, gam.data(), nh_use,
                                                    alpha, h->d_xold, h->stream));
@@ -827,6 +845,7 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
                                                    alpha, h->d_xold, h->stream));
         prof_end(h, ST_UPDATE, t1);
         gt_ready = fuse;
+        dx_ready = fuse && h->d_dx;
         std::swap(h->d_x, h->d_xold);  // x <- x_new, xold <- x
         std::swap(h->d_F, h->d_Fold);  // Fold <- F
     }

@@ -77,12 +77,13 @@ struct Sigma2Plan {
 // x (packed, reference layout) -> Gt [nw][ncol] site-pair layout
 hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
                                 hipStream_t s);
-// Broyden update (FP32 history) that also writes the next residual's Gt from x_new;
-// qs_update_gt_fits tells whether the shape allows it (energy slice in LDS, nh <= 16)
+// Broyden update (FP32 history) that also writes the next residual's Gt from x_new,
+// and x_new - x into dx when dx is not null; qs_update_gt_fits tells whether the
+// shape allows it (energy slice in LDS, nh <= 16)
 bool qs_update_gt_fits(int nA, int ncol, int64_t n, int nh);
 hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const double *F, const float *Us,
-                                           const double *gamma, int nh, double alpha, double *xnew, double2 *Gt,
-                                           const int *blk_src, int nA, int ncol, int nw, hipStream_t s);
+                                           const double *gamma, int nh, double alpha, double *xnew, double *dx,
+                                           double2 *Gt, const int *blk_src, int nA, int ncol, int nw, hipStream_t s);
 // parts: bit 0 forward pass over n1, bit 1 middle pass (row DFTs + 2x2 algebra),
 // bit 2 inverse pass over k1; the three in order make one Sigma2 evaluation.
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
@@ -97,9 +98,10 @@ hipError_t qs_launch_equal_time_sigma1(const double *x, const int *srcidx, const
 hipError_t qs_launch_reduce_cols(const double *in, int ncols, int nrows, double *out, double *scratch,
                                  hipStream_t s);
 hipError_t qs_launch_reduce_partials(const double *partials, int nblocks, double *out, hipStream_t s);
+// dx (may be null): x - xold as the last update wrote it; read in place of x and xold
 hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *Fold, const double *x,
-                                      const double *xold, double rf, double alpha, float *dFs, float *Us,
-                                      int slot, int nh, double *partials, int nblocks, hipStream_t s);
+                                      const double *xold, const double *dx, double rf, double alpha, float *dFs,
+                                      float *Us, int slot, int nh, double *partials, int nblocks, hipStream_t s);
 hipError_t qs_launch_broyden_hist_f64(int64_t n, const double *F, const double *Fold, const double *x,
                                       const double *xold, double fnorm, double alpha, double *dF, double *U,
                                       int slot, int nh, double *partials, int nblocks, hipStream_t s);

@@ -276,13 +276,15 @@ __global__ __launch_bounds__(PL_NT) void k_permute_lds(const double *__restrict_
 // x_new = x + alpha F - sum_k gamma_k U_k over the FP32 history (the arithmetic
 // of the plain update kernel, so x_new is the same to the bit), computed on whole
 // energy slices; each slice also goes to LDS and out to Gt as k_permute_lds
-// writes it.  One workgroup per CU, persistent over the energies.
+// writes it.  With dx, x_new - x also goes out: the next history pass reads it
+// in place of x and xold (the same FP64 difference).  One workgroup per CU,
+// persistent over the energies.
 struct GamH { double g[16]; };
 __global__ __launch_bounds__(PL_NT) void k_update_f32_gt(int64_t n, const double *__restrict__ x,
                                                          const double *__restrict__ F, const float *__restrict__ Us,
                                                          GamH gm, int nh, double alpha, double *__restrict__ xnew,
-                                                         double2 *__restrict__ Gt, const int2 *__restrict__ src,
-                                                         int nA, int ncol, int nw) {
+                                                         double *__restrict__ dx, double2 *__restrict__ Gt,
+                                                         const int2 *__restrict__ src, int nA, int ncol, int nw) {
     extern __shared__ __align__(16) char lds_raw[];
     double *xs = (double *)lds_raw;
     const int tid = threadIdx.x, nb = 3 * nA * nA, nq = nb / 4;  // 4 | nb (nA is even)
@@ -308,6 +310,7 @@ __global__ __launch_bounds__(PL_NT) void k_update_f32_gt(int64_t n, const double
                     v3 = v3 - (double)hv[k].w * gm.g[k];
                 }
             xnb[q] = make_double4(v0, v1, v2, v3);
+            if (dx) ((double4 *)(dx + base))[q] = make_double4(v0 - xv.x, v1 - xv.y, v2 - xv.z, v3 - xv.w);
             ((double2 *)xs)[2 * q] = make_double2(v0, v1);
             ((double2 *)xs)[2 * q + 1] = make_double2(v2, v3);
         }
@@ -683,8 +686,8 @@ int cu_count() {
 bool qs_update_gt_fits(int nA, int ncol, int64_t n, int nh) { return slice_fits_lds(nA, ncol) && n % 4 == 0 && nh <= 16; }
 
 hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const double *F, const float *Us,
-                                           const double *gamma, int nh, double alpha, double *xnew, double2 *Gt,
-                                           const int *blk_src, int nA, int ncol, int nw, hipStream_t s) {
+                                           const double *gamma, int nh, double alpha, double *xnew, double *dx,
+                                           double2 *Gt, const int *blk_src, int nA, int ncol, int nw, hipStream_t s) {
     if (!qs_update_gt_fits(nA, ncol, n, nh) || (int64_t)3 * nA * nA * nw != n) return hipErrorInvalidValue;
     const int ncu = cu_count();
     if (ncu <= 0) return hipErrorInvalidValue;
@@ -699,7 +702,7 @@ hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const dou
     for (int k = 0; k < 16; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
     const int blocks = nw < ncu ? nw : ncu;
     hipLaunchKernelGGL(k_update_f32_gt, dim3(blocks), dim3(PL_NT), (size_t)3 * nA * nA * sizeof(double), s, n, x, F,
-                       Us, gm, nh, alpha, xnew, Gt, (const int2 *)blk_src, nA, ncol, nw);
+                       Us, gm, nh, alpha, xnew, dx, Gt, (const int2 *)blk_src, nA, ncol, nw);
     return hipGetLastError();
 }
 

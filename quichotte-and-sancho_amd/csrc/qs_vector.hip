@@ -234,10 +234,12 @@ __global__ __launch_bounds__(NT) void k_update_f64(int64_t n, const double *__re
 // FP32 history, 4 consecutive elements per thread: every column is read with
 // 16-byte loads, so each thread keeps ~30 independent loads in flight.
 // n4 = n / 4 quads; the n % 4 tail is handled by the scalar kernel.
-template <int MH>
+// DX: x - xold comes from dx (written by the fused update) instead of x and xold
+template <int MH, bool DX>
 __global__ __launch_bounds__(NT) void k_hist_f32x4(int64_t n, int64_t n4, const double *__restrict__ F,
                                                    const double *__restrict__ Fold, const double *__restrict__ x,
-                                                   const double *__restrict__ xold, double rf, double alpha,
+                                                   const double *__restrict__ xold, const double *__restrict__ dx,
+                                                   double rf, double alpha,
                                                    float *__restrict__ dFs, float *__restrict__ Us, int slot, int nh,
                                                    double *__restrict__ part) {
     __shared__ double red[NT / QS_WAVE][2 * MAXH];
@@ -250,14 +252,18 @@ __global__ __launch_bounds__(NT) void k_hist_f32x4(int64_t n, int64_t n4, const 
         float4 df = make_float4(0.f, 0.f, 0.f, 0.f);
         if (slot >= 0) {
             const double4 fo = *(const double4 *)(Fold + i);
-            const double4 xv = *(const double4 *)(x + i);
-            const double4 xo = *(const double4 *)(xold + i);
+            double4 dd;
+            if (DX) {
+                dd = *(const double4 *)(dx + i);
+            } else {
+                const double4 xv = *(const double4 *)(x + i);
+                const double4 xo = *(const double4 *)(xold + i);
+                dd = make_double4(xv.x - xo.x, xv.y - xo.y, xv.z - xo.z, xv.w - xo.w);
+            }
             const double d0 = f.x - fo.x, d1 = f.y - fo.y, d2 = f.z - fo.z, d3 = f.w - fo.w;
             df = make_float4((float)(d0 * rf), (float)(d1 * rf), (float)(d2 * rf), (float)(d3 * rf));
-            const float4 u = make_float4((float)(alpha * d0 * rf + (xv.x - xo.x) * rf),
-                                         (float)(alpha * d1 * rf + (xv.y - xo.y) * rf),
-                                         (float)(alpha * d2 * rf + (xv.z - xo.z) * rf),
-                                         (float)(alpha * d3 * rf + (xv.w - xo.w) * rf));
+            const float4 u = make_float4((float)(alpha * d0 * rf + dd.x * rf), (float)(alpha * d1 * rf + dd.y * rf),
+                                         (float)(alpha * d2 * rf + dd.z * rf), (float)(alpha * d3 * rf + dd.w * rf));
             *(float4 *)(dFs + (size_t)slot * n + i) = df;
             *(float4 *)(Us + (size_t)slot * n + i) = u;
         }
@@ -357,12 +363,15 @@ hipError_t qs_launch_reduce_partials(const double *partials, int nblocks, double
 }
 
 hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *Fold, const double *x,
-                                      const double *xold, double rf, double alpha, float *dFs, float *Us, int slot,
-                                      int nh, double *partials, int nblocks, hipStream_t s) {
+                                      const double *xold, const double *dx, double rf, double alpha, float *dFs,
+                                      float *Us, int slot, int nh, double *partials, int nblocks, hipStream_t s) {
     if (nh > MAXH) return hipErrorInvalidValue;
-    if (n % 4 == 0 && nh <= 16)
-        hipLaunchKernelGGL(k_hist_f32x4<16>, dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold, rf, alpha, dFs,
-                           Us, slot, nh, partials);
+    if (n % 4 == 0 && nh <= 16 && dx)
+        hipLaunchKernelGGL((k_hist_f32x4<16, true>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold, dx, rf,
+                           alpha, dFs, Us, slot, nh, partials);
+    else if (n % 4 == 0 && nh <= 16)
+        hipLaunchKernelGGL((k_hist_f32x4<16, false>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold,
+                           nullptr, rf, alpha, dFs, Us, slot, nh, partials);
     else if (nh <= 16)
         hipLaunchKernelGGL(k_hist_f32<16>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us, slot,
                            nh, partials);
