@@ -177,6 +177,11 @@ void prof_collect(qs_handle *h) {
     h->prof.pending.clear();
 }
 
+// the order-2 FP32-history path keeps x - xold from the fused Broyden update
+bool uses_dx(const qs_problem &p, int nA, int ncolS, int64_t n_vec) {
+    return p.order == 2 && p.single_history && p.nhist > 0 && qs_update_gt_fits(nA, ncolS, n_vec, std::min(p.nhist, 16));
+}
+
 template <class T>
 int dalloc(qs_handle *h, T **ptr, size_t count) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
@@ -422,19 +427,17 @@ int setup_device(qs_handle *h) {
         }
         if ((rc = dalloc(h, &h->d_Gt, (size_t)nw * h->ncolS))) return rc;
         if ((rc = dalloc(h, &h->d_Z, (size_t)h->Nf * h->ncolS))) return rc;
-        // x - xold for the history pass (saves it reading x and xold): only where the
-        // fused update runs, and only with 4 GB of device memory to spare after it
-        if (h->p.single_history && nhist > 0 && qs_update_gt_fits(h->nA, h->ncolS, h->n_vec, std::min(nhist, 16))) {
-            size_t fr = 0, tot = 0;
-            if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > nv * sizeof(double) + ((size_t)4 << 30)) {
-                void *p = nullptr;
-                if (hipMalloc(&p, nv * sizeof(double)) == hipSuccess) {
-                    h->allocs.push_back(p);
-                    h->dev_bytes += (int64_t)(nv * sizeof(double));
-                    h->d_dx = (double *)p;
-                } else {
-                    (void)hipGetLastError();
-                }
+        // x - xold for the history pass (saves it reading x and xold), where the fused
+        // update runs; counted in qs_footprint_bytes.  Without it (allocation
+        // failure) the history pass reads x and xold.
+        if (uses_dx(h->p, h->nA, h->ncolS, h->n_vec)) {
+            void *p = nullptr;
+            if (hipMalloc(&p, nv * sizeof(double)) == hipSuccess) {
+                h->allocs.push_back(p);
+                h->dev_bytes += (int64_t)(nv * sizeof(double));
+                h->d_dx = (double *)p;
+            } else {
+                (void)hipGetLastError();
             }
         }
         if ((rc = dalloc(h, &h->d_gt0, (size_t)4 * h->nAs))) return rc;
@@ -1339,6 +1342,7 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
         if (p->nhist > 0) b += 2 * nv * (int64_t)p->nhist * (p->single_history ? 4 : 8);
         const int64_t nAs = nA / 2, ncolS = (12 * (nAs * (nAs + 1) / 2) + 7) / 8 * 8;
         b += (nw + Nf) * ncolS * 16;
+        if (uses_dx(*p, (int)nA, (int)ncolS, nv)) b += nv * 8;
     }
     if (n <= qs_dyson_wave_max_norb()) b += (int64_t)wave_chunk((int)n, (int)nw) * (int64_t)qs_dyson_wave_scratch_bytes((int)n);
     else b += (int64_t)qs_dyson_big_bytes((int)n, (int)nw, kBigBudget);
