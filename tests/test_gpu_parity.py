@@ -218,3 +218,25 @@ def test_each_dyson_launch_shape_full_solve(order, variant, monkeypatch):
     Go, Gg = o.g_less_obs(), g.g_less_obs()
     assert np.abs(Gg - Go).max() <= 1e-10 * np.abs(Go).max()
     assert rel_max(g.dosd(), o.dosd()) < 1e-10
+
+
+@pytest.mark.parametrize("single", [True, False])
+def test_solve_stopped_at_maxscf_matches_oracle(single):
+    """A solve that ends at maxscf without converging: the final sweep uses the
+    last residual's Sigma2 (the fused FP32 update must not overwrite it on that
+    step), as the reference's does."""
+    prob = helix_problem(4, 2, 0.8, order=2, n_omega=400, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
+                         gamma=1.0, mu=2.0, tolf=1e-14, tolf_order1=1e-10, nhist=8, voltages=[0.3],
+                         single_history=single)
+    prob.maxscf = 3
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    ro = o.solve(0.3, True)
+    rg = g.solve_scf(0.3, True)
+    assert not ro["converged"] and not rg.converged
+    assert ro["n_iter"] == rg.n_iter == 3
+    Jo, Jg = np.array(o.current()), np.array(g.current())
+    assert np.all(np.abs(Jg - Jo) <= 1e-10 * np.abs(Jo) + 1e-12), (Jg, Jo)
+    Go, Gg = o.g_less_obs(), g.g_less_obs()
+    assert np.abs(Gg - Go).max() <= 1e-10 * np.abs(Go).max()
+    assert rel_max(g.dosd(), o.dosd()) < 1e-10
