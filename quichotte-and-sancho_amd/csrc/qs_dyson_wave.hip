@@ -186,8 +186,14 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         if (li + TI * a >= n) used |= 1u << a;
 
     // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
-    // step k = -1 only posts the candidates of column 0
+    // step k = -1 only posts the candidates of column 0.
+    // A SIMD holds waves of two energies.  A step's dependency chain (pivot pick,
+    // reciprocal, next candidates and their posting) runs at wave priority 3 and
+    // the rank-1 update at 0, so the other energy's bulk work fills this one's
+    // stalls instead of delaying its chain (in-box A/B: 37.2/37.8 -> 36.2/36.2 ms
+    // per sweep at 80 orbitals and 2^16 energies).
     for (int k = -1; k < n; k++) {
+        __builtin_amdgcn_s_setprio(3);
         if (k >= 0) {
             const int buf = k & 1, kb = k / TJ, kl = k % TJ;
             const double2 *pcl = L.pcol + buf * TI * RA;
@@ -232,10 +238,12 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
                         for (int a = 0; a < RA; a++) M[a][b] = make_double2(0.0, 0.0);
                     }
             }
+            __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int a = 0; a < RA; a++)
 #pragma unroll
                 for (int b = 0; b < RB; b++) cfms(M[a][b], fa[a], sp[b]);
+            __builtin_amdgcn_s_setprio(3);
             if (li == tp) {
 #pragma unroll
                 for (int a = 0; a < RA; a++)
@@ -289,6 +297,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         __syncthreads();
     }
 
+    __builtin_amdgcn_s_setprio(0);
     // ---- 3. scratch copy of M: Ms (column-major) ------------------------------------
     for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
 #pragma unroll
