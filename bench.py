@@ -270,7 +270,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (one bias point is one "
-              "GPU's work; see the sharding notes in DESIGN.md)" % args.gpus, file=sys.stderr)
+              "GPU's work; several GPUs would each take their own bias points or energy ranges)" % args.gpus,
+              file=sys.stderr)
         return 2
     if args.serve:
         return serve_bench(args)
