@@ -228,17 +228,36 @@ __global__ __launch_bounds__(NT) void k_permute_in(const double *__restrict__ x,
 // stores.  The next energy's loads are in flight during the current gather;
 // each thread keeps its columns' source pairs in registers for every energy.
 constexpr int PL_NT = 1024, PL_QL = 10, PL_QC = 11;  // slice <= PL_QL NT double2, ncol <= PL_QC NT
+// this thread's columns c = tid + PL_NT u of the site-pair layout: their source pairs
+__device__ __forceinline__ void load_colmap(const int2 *__restrict__ src, int2 (&sc)[PL_QC], int ncol, int tid) {
+#pragma unroll
+    for (int u = 0; u < PL_QC; u++) {
+        const int c = tid + PL_NT * u;
+        sc[u] = c < ncol ? src[c] : make_int2(0, -1);
+    }
+}
+// one energy's Gt row from its packed slice xs (in LDS): column c takes its real
+// part from +/- xs[|sc.x| - 1] (sc.x = 0: zero) and its imaginary part from xs[sc.y]
+// (sc.y < 0: zero); the row goes out as contiguous 16-byte stores
+__device__ __forceinline__ void gather_gt(const double *xs, const int2 (&sc)[PL_QC], double2 *__restrict__ g, int ncol,
+                                          int tid) {
+#pragma unroll
+    for (int u = 0; u < PL_QC; u++) {
+        const int c = tid + PL_NT * u;
+        const int ir = sc[u].x > 0 ? sc[u].x - 1 : (sc[u].x < 0 ? -sc[u].x - 1 : 0);
+        const int ii = sc[u].y >= 0 ? sc[u].y : 0;
+        const double xr = xs[ir], xi = xs[ii];
+        const double re = sc[u].x > 0 ? xr : (sc[u].x < 0 ? -xr : 0.0);
+        if (c < ncol) g[c] = make_double2(re, sc[u].y >= 0 ? xi : 0.0);
+    }
+}
 __global__ __launch_bounds__(PL_NT) void k_permute_lds(const double *__restrict__ x, double2 *__restrict__ Gt,
                                                        const int2 *__restrict__ src, int nA, int ncol, int nw) {
     extern __shared__ __align__(16) char lds_raw[];
     double *xs = (double *)lds_raw;
     const int tid = threadIdx.x, nh = 3 * nA * nA / 2;  // double2 per energy (nA is even)
     int2 sc[PL_QC];
-#pragma unroll
-    for (int u = 0; u < PL_QC; u++) {
-        const int c = tid + PL_NT * u;
-        sc[u] = c < ncol ? src[c] : make_int2(0, -1);
-    }
+    load_colmap(src, sc, ncol, tid);
     double2 v[PL_QL];
     auto fetch = [&](int iw) {
         const double2 *xb = (const double2 *)x + (size_t)min(iw, nw - 1) * nh;
@@ -257,17 +276,7 @@ __global__ __launch_bounds__(PL_NT) void k_permute_lds(const double *__restrict_
         }
         __syncthreads();
         if (iw + (int)gridDim.x < nw) fetch(iw + gridDim.x);
-        double2 *g = Gt + (size_t)iw * ncol;
-#pragma unroll
-        for (int u = 0; u < PL_QC; u++) {
-            const int c = tid + PL_NT * u;
-            // real part +/- xs[|sc.x| - 1] (0: zero), imaginary part xs[sc.y] (< 0: zero)
-            const int ir = sc[u].x > 0 ? sc[u].x - 1 : (sc[u].x < 0 ? -sc[u].x - 1 : 0);
-            const int ii = sc[u].y >= 0 ? sc[u].y : 0;
-            const double xr = xs[ir], xi = xs[ii];
-            const double re = sc[u].x > 0 ? xr : (sc[u].x < 0 ? -xr : 0.0);
-            if (c < ncol) g[c] = make_double2(re, sc[u].y >= 0 ? xi : 0.0);
-        }
+        gather_gt(xs, sc, Gt + (size_t)iw * ncol, ncol, tid);
         __syncthreads();  // the next slice overwrites xs
     }
 }
@@ -315,23 +324,10 @@ __global__ __launch_bounds__(PL_NT) void k_update_f32_gt(int64_t n, const double
             ((double2 *)xs)[2 * q + 1] = make_double2(v2, v3);
         }
         __syncthreads();
-        double2 *g = Gt + (size_t)iw * ncol;
         // the column map comes from L2 each time: registers go to the history loads
         int2 sc[PL_QC];
-#pragma unroll
-        for (int u = 0; u < PL_QC; u++) {
-            const int c = tid + PL_NT * u;
-            sc[u] = c < ncol ? src[c] : make_int2(0, -1);
-        }
-#pragma unroll
-        for (int u = 0; u < PL_QC; u++) {
-            const int c = tid + PL_NT * u;
-            const int ir = sc[u].x > 0 ? sc[u].x - 1 : (sc[u].x < 0 ? -sc[u].x - 1 : 0);
-            const int ii = sc[u].y >= 0 ? sc[u].y : 0;
-            const double xr = xs[ir], xi = xs[ii];
-            const double re = sc[u].x > 0 ? xr : (sc[u].x < 0 ? -xr : 0.0);
-            if (c < ncol) g[c] = make_double2(re, sc[u].y >= 0 ? xi : 0.0);
-        }
+        load_colmap(src, sc, ncol, tid);
+        gather_gt(xs, sc, Gt + (size_t)iw * ncol, ncol, tid);
         __syncthreads();  // the next slice overwrites xs
     }
 }
