@@ -21,7 +21,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "qs_hip.h"
@@ -531,6 +534,14 @@ Sanity diag_sanity(const double *sumImGL, const double *sumImGR, int nb, double 
 // Pick the Dyson launch shape for this size: time each variant on a slice of
 // the grid (mode FINAL, which every order runs) and keep the fastest.
 // QS_DYSON_VARIANT=<k> in the environment forces variant k.
+// The choice is kept for the process, per (orbitals, contact orbitals, slice
+// length): a handle created while other work runs (a serving loop) then takes
+// the shape an earlier handle of that size measured, and gives the same
+// results bit for bit.
+struct TuneRecord { int variant; double ms[4]; };
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, TuneRecord> g_tune;
+
 int tune_dyson(qs_handle *h) {
     const int nv = qs_dyson_variants(h->norb);
     h->dyson_variant = 0;
@@ -543,6 +554,16 @@ int tune_dyson(qs_handle *h) {
         // a shape this size cannot use: time the ones it can
     }
     const int count = std::min(h->nw, 2048);
+    const auto key = std::make_tuple(h->norb, h->nCL, count);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        const auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            h->dyson_variant = it->second.variant;
+            for (int i = 0; i < 4; i++) h->dyson_tune_ms[i] = it->second.ms[i];
+            return QS_OK;
+        }
+    }
     DysonArgs a = dyson_args(h, QS_M_FINAL, false, 0.0);
     hipEvent_t e0, e1;
     HIPCHK(h, hipEventCreate(&e0));
@@ -564,6 +585,12 @@ int tune_dyson(qs_handle *h) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        TuneRecord r{h->dyson_variant, {}};
+        for (int i = 0; i < 4; i++) r.ms[i] = h->dyson_tune_ms[i];
+        g_tune.emplace(key, r);  // the first measurement of a size stays
+    }
     return QS_OK;
 }
 
