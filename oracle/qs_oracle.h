@@ -109,6 +109,18 @@ void orc_broyden(orc_residual_fn fn, void *ctx, double *x, int64_t n, double alp
                  int maxscf, double tolf, int single_history, int32_t *converged, int32_t *n_iter,
                  double *fmax, double *log_fmax, double *log_rms);
 
+/* SANCHO (sancho_oracle.c): complete_input_gf + build_self_energies
+ * (SANCHO/src/green_functions.f90:163-173, SANCHO/src/self_energies.f90:38-207).
+ * GR, GL (na,na,nw) complex; U (na); outputs (na,na,nw) complex, Sig3* only for
+ * order 3.  hbar = 1.  Returns 0, or -1 on bad arguments or allocation failure. */
+int orc_sancho_sigma(int32_t na, int32_t nw, double delta, const double *U, int32_t order, const double *GR,
+                     const double *GL, double *S2R, double *S2L, double *S2G, double *S3R, double *S3L,
+                     double *S3G);
+/* the bubbles alone: chi (4, na, na, 2nw-1) in the order G, L, R, A with m = -(nw-1).. at m + nw - 1
+ * (innermost), then a, then b; phi likewise with n = 2.. at n - 2 (order 3 only) */
+int orc_sancho_bubbles(int32_t na, int32_t nw, double delta, int32_t order, const double *GR, const double *GL,
+                       double *chi, double *phi);
+
 #ifdef __cplusplus
 }
 #endif

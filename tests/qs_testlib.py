@@ -259,6 +259,8 @@ def oracle_lib():
         L.orc_check_diag_sanity.argtypes = [P, P, ctypes.c_int, ctypes.c_int, D, P, P, P, P, P, P]
         L.orc_fermi.restype = D
         L.orc_fermi.argtypes = [D, D, D, D]
+        L.orc_sancho_sigma.argtypes = [I32, I32, D, P, I32, P, P, P, P, P, P, P, P]
+        L.orc_sancho_bubbles.argtypes = [I32, I32, D, I32, P, P, P, P]
         _lib = L
     return _lib
 
@@ -454,3 +456,59 @@ def random_problem(norb, n_omega, order=2, U=1.0, seed=0, gamma=0.3, **kw):
     delta = (ev[-1] - ev[0] + 2.0) / n_omega
     return Problem(H, GL, GR, Us, order, 300.0, 0.1, 0, 0, delta, [0.0], n_omega=n_omega,
                    w_init=ev[0] - 1.0, **kw)
+
+
+# --------------------------------------------------------------------------
+# SANCHO self-energies
+# --------------------------------------------------------------------------
+def sancho_inputs(na, nw, seed=0, wlo=-2.0, whi=2.0, gamma=0.4, eta=0.05, mu=0.1, T=0.05):
+    """Physical input Green's functions of a random na-site model on nw energies:
+    G^r(w) = (w - H + i(Gamma + eta)/2)^-1 and G^< = G^r (i f(w) Gamma) G^a, so
+    G^a = (G^r)^dagger and G^< is anti-Hermitian, as SCF inputs are.  Returns
+    G^r, G^< as (na, na, nw) complex Fortran-ordered arrays, U (na) and delta."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((na, na)) + 1j * rng.standard_normal((na, na))
+    H = 0.3 * (A + A.conj().T)
+    B = rng.standard_normal((na, na)) + 1j * rng.standard_normal((na, na))
+    Gam = gamma * (B @ B.conj().T) / na
+    delta = (whi - wlo) / nw
+    w = wlo + delta * np.arange(1, nw + 1)
+    GR = np.empty((na, na, nw), dtype=complex, order="F")
+    GL = np.empty((na, na, nw), dtype=complex, order="F")
+    for k in range(nw):
+        r = np.linalg.inv(w[k] * np.eye(na) - H + 0.5j * (Gam + eta * np.eye(na)))
+        f = 1.0 / (np.exp((w[k] - mu) / T) + 1.0)
+        GR[:, :, k] = r
+        GL[:, :, k] = r @ (1j * f * Gam) @ r.conj().T
+    U = 0.5 + rng.random(na)
+    return GR, GL, U, delta
+
+
+def sancho_oracle(GR, GL, U, delta, order):
+    """orc_sancho_sigma: (S2R, S2L, S2G[, S3R, S3L, S3G]) as (na, na, nw) arrays"""
+    na, nw = GR.shape[0], GR.shape[2]
+    GR = np.asfortranarray(GR, dtype=complex)
+    GL = np.asfortranarray(GL, dtype=complex)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    outs = [np.zeros((na, na, nw), dtype=complex, order="F") for _ in range(6 if order == 3 else 3)]
+    p = [ptr(o) for o in outs] + [None] * (6 - len(outs))
+    rc = oracle_lib().orc_sancho_sigma(na, nw, delta, ptr(U), order, ptr(GR), ptr(GL), *p)
+    if rc != 0:
+        raise RuntimeError("orc_sancho_sigma failed: %d" % rc)
+    return outs
+
+
+def sancho_oracle_bubbles(GR, GL, delta, order):
+    """orc_sancho_bubbles: chi[q, a, b, m + nw - 1] (q = G, L, R, A) and phi[q, a, b, n - 2]"""
+    na, nw = GR.shape[0], GR.shape[2]
+    L = 2 * nw - 1
+    GR = np.asfortranarray(GR, dtype=complex)
+    GL = np.asfortranarray(GL, dtype=complex)
+    chi = np.zeros((4, na, na, L), dtype=complex)
+    phi = np.zeros((4, na, na, L), dtype=complex)
+    rc = oracle_lib().orc_sancho_bubbles(na, nw, delta, order, ptr(GR), ptr(GL), ptr(chi),
+                                         ptr(phi) if order == 3 else None)
+    if rc != 0:
+        raise RuntimeError("orc_sancho_bubbles failed: %d" % rc)
+    # the C arrays are (q, b, a, m) in C order; return (q, a, b, m)
+    return chi.transpose(0, 2, 1, 3), phi.transpose(0, 2, 1, 3)
