@@ -237,6 +237,85 @@ def serve_bench(args):
     return 0
 
 
+def chain_gf(N, dw, up, delta, E=2.0, t=-0.4, gamma=1.0, mu=2.0, T=300.0):
+    """Non-interacting G^r, G^< of SANCHO's chain (E_CC on every site, t_hop
+    between neighbours, GammaL on site 1, GammaR on site N) on its grid
+    w_init = Ev(1) - dw, n_omega = int((Ev(N) + up - w_init) / delta)
+    (SANCHO/src/sancho.f90:84-91), V = 0."""
+    H = np.diag(np.full(N, E)) + np.diag(np.full(N - 1, t), 1) + np.diag(np.full(N - 1, t), -1)
+    ev = np.linalg.eigvalsh(H)
+    w_init = ev[0] - dw
+    nw = int((ev[-1] + up - w_init) / delta)
+    GamL = np.zeros((N, N))
+    GamR = np.zeros((N, N))
+    GamL[0, 0] = gamma
+    GamR[-1, -1] = gamma
+    beta = 1.0 / (KB * T)
+    GR = np.empty((N, N, nw), complex, order="F")
+    GL = np.empty((N, N, nw), complex, order="F")
+    for k in range(nw):
+        w = w_init + (k + 1) * delta
+        r = np.linalg.inv(w * np.eye(N) - H + 0.5j * (GamL + GamR))
+        f = 1.0 / (np.exp(min((w - mu) * beta, 700.0)) + 1.0)
+        GR[:, :, k] = r
+        GL[:, :, k] = r @ (1j * f * (GamL + GamR)) @ r.conj().T
+    return GR, GL, delta, nw
+
+
+SANCHO_WORKLOADS = {
+    # name: (N_sites, dw, up, order): the reference's 8-site order-3 run and 20-site order-2 example
+    "chain_N8_order3": (8, 3.0, 3.0, 3),
+    "chain_N20_order2": (20, 2.0, 2.0, 2),
+}
+
+
+def sancho_bench(args):
+    """SANCHO's self-energies (complete_input_gf + build_self_energies,
+    SANCHO/src/self_energies.f90:38-207) on device arrays: builds per second.
+    Algorithmic flops per build, na sites, nw energies (complex multiply-add = 8):
+    chi and phi 48 na^2 nw^2 each, Sigma2 32 na^2 nw^2, Sigma3 64 na^2 nw^2,
+    the internal-site sums 56 na^3 (2nw - 1) each (order 3)."""
+    import importlib
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    N, dw, up, order = SANCHO_WORKLOADS[args.sancho]
+    GR, GL, delta, nw = chain_gf(N, dw, up, 0.01)
+    U = np.full(N, 0.8)
+    flops = 48 * N * N * nw * nw + 32 * N * N * nw * nw
+    if order == 3:
+        flops += 48 * N * N * nw * nw + 64 * N * N * nw * nw + 2 * 56 * N ** 3 * (2 * nw - 1)
+    s = pkg.SanchoSelfEnergies(N, nw, order)
+    nbytes = 16 * N * N * nw
+    gr, gl = pkg.DeviceBuffer(nbytes), pkg.DeviceBuffer(nbytes)
+    gr.upload(GR.ravel(order="F"))
+    gl.upload(GL.ravel(order="F"))
+    outs = [pkg.DeviceBuffer(nbytes) for _ in range(s.n_outputs)]
+    # inputs resident in HBM; each build is synchronous (ends with a stream synchronize)
+    for _ in range(args.warmup):
+        s.build_device(gr, gl, outs, U, delta)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.build_device(gr, gl, outs, U, delta)
+    dt = (time.perf_counter() - t0) / args.steps
+    line = {"metric": "SANCHO self-energy builds/s", "value": 1.0 / dt, "unit": "builds/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "SANCHO %s: %d sites, %d energies, order %d, non-interacting chain G" %
+                                   (args.sancho, N, nw, order)},
+            "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": flops / dt / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                         "note": "all kernels of one build over its whole call time, FP64 vector peak"}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        from qs_testlib import sancho_oracle
+        t0 = time.perf_counter()
+        sancho_oracle(GR, GL, U, delta, order)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 1.0 / tc, "unit": "builds/s", "cores": args.cpu_threads, "kind": "port",
+                                "sample": "one full build of the same workload (oracle/sancho_oracle.c, OpenMP)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def copy_calibration():
     """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch scale
     the same counters read on the kernels of the path to HBM bytes."""
@@ -265,6 +344,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--maxscf", type=int, default=0,
                     help="cap on SCF steps per bias point (counter-collection passes; 0 = the workload's 300)")
+    ap.add_argument("--sancho", default="", choices=[""] + list(SANCHO_WORKLOADS),
+                    help="time SANCHO's order-2/3 self-energy build on this workload instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 2 GiB five times (scale for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -275,6 +356,8 @@ def main():
         return 2
     if args.serve:
         return serve_bench(args)
+    if args.sancho:
+        return sancho_bench(args)
     if args.calibrate_copy:
         return copy_calibration()
     import importlib

@@ -129,6 +129,26 @@ const char *qs_stage_name(int32_t stage);
  * apply to this size); returns the number of shapes */
 int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax);
 
+/* ---- SANCHO: interaction self-energies of orders 2 and 3 ----------------
+ * The work arrays of SANCHO's SelfEnergies module (SANCHO/src/self_energies.f90:23-35,
+ * alloc_sigma_work :209-230) for na sites, n_omega energies and order 2 or 3 on
+ * one device.  hbar = 1 (SANCHO/common/constants.f90:18). */
+typedef struct qs_sancho qs_sancho;
+int qs_sancho_create(int32_t na, int32_t n_omega, int32_t order, int32_t device, qs_sancho **out);
+void qs_sancho_destroy(qs_sancho *s);
+const char *qs_sancho_last_error(const qs_sancho *s);
+int64_t qs_sancho_device_bytes(const qs_sancho *s);
+/* complete_input_gf + build_self_energies (SANCHO/src/green_functions.f90:163-173,
+ * SANCHO/src/self_energies.f90:38-46): G^a, G^> from G^r, G^<, then the bubbles,
+ * Sig2R/L/G and, at order 3, Sig3R/L/G.  GR, GL and the Sig arrays are
+ * (na, na, n_omega) complex128, column-major (G_in%r, G_in%L, Sig2R, ...);
+ * U_site (na) and delta are host values.  on_device = 0: the arrays are host
+ * memory; 1: device memory on the handle's device.  Synchronous.  Calls on one
+ * handle must not overlap. */
+int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
+                            double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L,
+                            double *Sig3G, int32_t on_device);
+
 #ifdef __cplusplus
 }
 #endif

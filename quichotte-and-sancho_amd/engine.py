@@ -6,6 +6,10 @@ ScfEngine mirrors the reference's SCF module interface for one bias point:
   g_less_obs(), dosd()      the final-sweep observables (src/scf.f90:134-140)
   init2(V), residual2(V,x)  the order-2 seams (src/scf.f90:116-122,158-188)
   sigma()                   Sigma1 and Sigma2 of the last residual
+
+SanchoSelfEnergies mirrors SANCHO's SelfEnergies module: build_self_energies()
+(SANCHO/src/self_energies.f90:38-46) after complete_input_gf
+(SANCHO/src/green_functions.f90:163-173), for orders 2 and 3.
 """
 import ctypes
 import os
@@ -81,6 +85,13 @@ def load_library(path=SO):
     L.qs_dyson_choice.argtypes = [P, P, P, I32]
     L.qs_stage_name.argtypes = [I32]
     L.qs_stage_name.restype = ctypes.c_char_p
+    L.qs_sancho_create.argtypes = [I32, I32, I32, I32, ctypes.POINTER(P)]
+    L.qs_sancho_destroy.argtypes = [P]
+    L.qs_sancho_last_error.argtypes = [P]
+    L.qs_sancho_last_error.restype = ctypes.c_char_p
+    L.qs_sancho_device_bytes.argtypes = [P]
+    L.qs_sancho_device_bytes.restype = I64
+    L.qs_sancho_self_energies.argtypes = [P, D, P, P, P, P, P, P, P, P, P, I32]
     _lib = L
     return L
 
@@ -307,3 +318,119 @@ class ScfEngine:
         r = _Result()
         self._check(self.L.qs_final_sweep(self.h, float(V), ctypes.byref(r)))
         return r
+
+
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime libqs_hip.so itself runs on (found in this process's
+    mappings once the library is loaded), for callers that hand over device
+    memory.  A second runtime in the process (such as the one a framework
+    bundles) cannot share the GPU with it."""
+    global _hip
+    if _hip is None:
+        load_library()
+        path = None
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64.so" in line:
+                    cand = line.split()[-1]
+                    if path is None or "/opt/rocm" in cand:
+                        path = cand
+        if path is None:
+            raise RuntimeError("libqs_hip.so is loaded but its HIP runtime is not mapped")
+        H = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        H.hipMalloc.argtypes = [ctypes.POINTER(P), ctypes.c_size_t]
+        H.hipFree.argtypes = [P]
+        H.hipMemcpy.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int]
+        H.hipSetDevice.argtypes = [ctypes.c_int]
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """Device memory from the library's own HIP runtime (hipMalloc / hipMemcpy)."""
+
+    def __init__(self, nbytes, device=0):
+        H = hip_runtime()
+        self.H, self.nbytes, self.ptr = H, int(nbytes), ctypes.c_void_p()
+        if H.hipSetDevice(int(device)) != 0 or H.hipMalloc(ctypes.byref(self.ptr), self.nbytes) != 0:
+            raise QsError(4, "hipMalloc of %d bytes failed" % self.nbytes)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        if self.H.hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes, 1) != 0:
+            raise QsError(3, "hipMemcpy to device failed")
+
+    def download(self, a):
+        assert a.flags.c_contiguous and a.nbytes == self.nbytes
+        if self.H.hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes, 2) != 0:
+            raise QsError(3, "hipMemcpy to host failed")
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self.H.hipFree(self.ptr)
+            self.ptr = None
+
+
+class SanchoSelfEnergies:
+    """SANCHO's interaction self-energies on one device.
+
+    build(GR, GL, U_site, delta) takes G_in%r and G_in%L as (na, na, n_omega)
+    complex numpy arrays and returns Sig2R, Sig2L, Sig2G and, at order 3,
+    Sig3R, Sig3L, Sig3G, all in the reference's column-major layout.
+    build_device() does the same on device memory (DeviceBuffer, or any
+    address from the library's HIP runtime) with nothing copied to the host.
+    """
+
+    def __init__(self, na, n_omega, order, device=0):
+        L = load_library()
+        self.L = L
+        self.na, self.n_omega, self.order, self.device = int(na), int(n_omega), int(order), int(device)
+        h = ctypes.c_void_p()
+        rc = L.qs_sancho_create(self.na, self.n_omega, self.order, self.device, ctypes.byref(h))
+        if rc != 0:
+            raise QsError(rc, "qs_sancho_create(na=%d, n_omega=%d, order=%d) failed" % (na, n_omega, order))
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.L.qs_sancho_destroy(h)
+            self.h = None
+
+    @property
+    def n_outputs(self):
+        return 6 if self.order == 3 else 3
+
+    def device_bytes(self):
+        return int(self.L.qs_sancho_device_bytes(self.h))
+
+    def _call(self, delta, U_site, gr, gl, outs, on_device):
+        U = np.ascontiguousarray(U_site, dtype=np.float64)
+        if U.shape != (self.na,):
+            raise ValueError("U_site must have na = %d entries" % self.na)
+        outs = list(outs) + [None] * (6 - len(outs))
+        rc = self.L.qs_sancho_self_energies(self.h, float(delta), _ptr(U), gr, gl, *outs, on_device)
+        if rc != 0:
+            raise QsError(rc, self.L.qs_sancho_last_error(self.h).decode())
+
+    def build(self, GR, GL, U_site, delta):
+        shape = (self.na, self.na, self.n_omega)
+        GR = np.asfortranarray(GR, dtype=complex)
+        GL = np.asfortranarray(GL, dtype=complex)
+        if GR.shape != shape or GL.shape != shape:
+            raise ValueError("GR, GL must be %s" % (shape,))
+        outs = [np.zeros(shape, complex, order="F") for _ in range(self.n_outputs)]
+        self._call(delta, U_site, _ptr(GR), _ptr(GL), [_ptr(o) for o in outs], 0)
+        return outs
+
+    def build_device(self, gr, gl, outs, U_site, delta):
+        """gr, gl, outs: DeviceBuffer objects (or device addresses) of na^2 n_omega complex128 each"""
+        addr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else ctypes.c_void_p(int(b))
+        if len(outs) != self.n_outputs:
+            raise ValueError("order %d needs %d outputs" % (self.order, self.n_outputs))
+        self._call(delta, U_site, addr(gr), addr(gl), [addr(o) for o in outs], 1)

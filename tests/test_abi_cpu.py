@@ -73,3 +73,19 @@ def test_footprint_counts_the_update_difference_buffer():
     assert footprint(8, True) - base == (2 * 8 * 4 + 8) * n_vec
     assert footprint(4, False) - base == 2 * 4 * 8 * n_vec
     assert footprint(0, False) == base
+
+
+def test_sancho_create_rejects_bad_sizes_before_any_device_call(lib):
+    """argument checks come first: no HIP call is made for these"""
+    P = ctypes.c_void_p
+    lib.qs_sancho_create.argtypes = [ctypes.c_int32] * 4 + [ctypes.POINTER(P)]
+    lib.qs_sancho_last_error.restype = ctypes.c_char_p
+    lib.qs_sancho_last_error.argtypes = [P]
+    lib.qs_sancho_device_bytes.restype = ctypes.c_int64
+    lib.qs_sancho_device_bytes.argtypes = [P]
+    h = P(1)
+    for na, nw, order in [(0, 10, 2), (4, 0, 3), (4, 10, 1), (4, 10, 4), (256, 10, 2)]:
+        assert lib.qs_sancho_create(na, nw, order, 0, ctypes.byref(h)) == 2
+        assert not h.value
+    assert lib.qs_sancho_last_error(None) == b"null handle"
+    assert lib.qs_sancho_device_bytes(None) == -1

@@ -1,0 +1,83 @@
+"""SANCHO's order-2/3 self-energies on the GPU (k_sn_* kernels of libqs_hip.so)
+against the oracle (oracle/sancho_oracle.c, SANCHO/src/self_energies.f90:38-207).
+
+Floating point: |hip - oracle| <= 1e-12 max|oracle| per output array (the sums
+run in another order than the reference's loops).
+"""
+import numpy as np
+import pytest
+
+from qs_testlib import product, sancho_inputs, sancho_oracle
+
+TOL = 1e-12
+NAMES = ["Sig2R", "Sig2L", "Sig2G", "Sig3R", "Sig3L", "Sig3G"]
+
+
+def check(outs, ref, case):
+    for q, (o, r) in enumerate(zip(outs, ref)):
+        err = np.max(np.abs(o - r)) / max(np.max(np.abs(r)), 1e-300)
+        assert err < TOL, "%s %s rel err %.3e" % (case, NAMES[q], err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("na,nw,seed", [(1, 1, 0), (2, 7, 1), (3, 256, 2), (4, 333, 3), (5, 513, 4), (8, 600, 5)])
+def test_sancho_sigma_matches_oracle(order, na, nw, seed):
+    print("sancho parity order=%d na=%d nw=%d seed=%d" % (order, na, nw, seed), flush=True)
+    GR, GL, U, delta = sancho_inputs(na, nw, seed)
+    ref = sancho_oracle(GR, GL, U, delta, order)
+    s = product().SanchoSelfEnergies(na, nw, order)
+    outs = s.build(GR, GL, U, delta)
+    assert len(outs) == len(ref)
+    check(outs, ref, (order, na, nw))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 3])
+def test_sancho_device_arrays_match_host_arrays(order):
+    """on_device = 1 (device memory in and out) gives the host call's bits"""
+    pkg = product()
+    na, nw = 4, 300
+    print("sancho device/host order=%d na=%d nw=%d" % (order, na, nw), flush=True)
+    GR, GL, U, delta = sancho_inputs(na, nw, 11)
+    s = pkg.SanchoSelfEnergies(na, nw, order)
+    host = s.build(GR, GL, U, delta)
+    nbytes = 16 * na * na * nw
+    gr, gl = pkg.DeviceBuffer(nbytes), pkg.DeviceBuffer(nbytes)
+    gr.upload(GR.ravel(order="F"))
+    gl.upload(GL.ravel(order="F"))
+    outs = [pkg.DeviceBuffer(nbytes) for _ in range(s.n_outputs)]
+    s.build_device(gr, gl, outs, U, delta)
+    for h, d in zip(host, outs):
+        back = np.empty(na * na * nw, complex)
+        d.download(back)
+        np.testing.assert_array_equal(h.ravel(order="F"), back)
+
+
+@pytest.mark.gpu
+def test_sancho_repeat_calls_and_spectral_identity():
+    """a second call on the same handle with other inputs gives that input's
+    result, and the outputs keep Sigma^r - Sigma^a = Sigma^> - Sigma^<"""
+    na, nw = 3, 200
+    print("sancho repeat na=%d nw=%d" % (na, nw), flush=True)
+    s = product().SanchoSelfEnergies(na, nw, 3)
+    for seed in (21, 22):
+        GR, GL, U, delta = sancho_inputs(na, nw, seed)
+        outs = s.build(GR, GL, U, delta)
+        check(outs, sancho_oracle(GR, GL, U, delta, 3), ("repeat", seed))
+        for R, L, G in (outs[0:3], outs[3:6]):
+            lhs = R - np.conj(R.transpose(1, 0, 2))
+            assert np.max(np.abs(lhs - (G - L))) < 1e-12 * np.max(np.abs(R))
+
+
+@pytest.mark.gpu
+def test_sancho_rejects_bad_arguments():
+    pkg = product()
+    with pytest.raises(pkg.QsError):
+        pkg.SanchoSelfEnergies(4, 100, 4)
+    s = pkg.SanchoSelfEnergies(2, 10, 2)
+    GR, GL, U, delta = sancho_inputs(2, 10, 0)
+    with pytest.raises(pkg.QsError):
+        s.build(GR, GL, U, -1.0)
+    with pytest.raises(ValueError):
+        s.build(GR[:, :, :5], GL[:, :, :5], U, delta)
