@@ -121,6 +121,7 @@ __global__ __launch_bounds__(SN_NT) void k_sn_sum(SnLaunch P) {
             lo = max(lo, 0);
             hi = min(hi, min(SN_TK, S.nx - k0));
             const int yl0 = S.sig > 0 ? SN_NT - 1 - t : t + SN_TK - 1, ys1 = S.sig > 0 ? 1 : -1;
+#pragma unroll 4
             for (int kk = lo; kk < hi; kk++) {
                 const int yl = yl0 + ys1 * kk;
                 double2 xv[NX], yv[NX];

@@ -21,7 +21,8 @@ def check(outs, ref, case):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("order", [2, 3])
-@pytest.mark.parametrize("na,nw,seed", [(1, 1, 0), (2, 7, 1), (3, 256, 2), (4, 333, 3), (5, 513, 4), (8, 600, 5)])
+@pytest.mark.parametrize("na,nw,seed", [(1, 1, 0), (2, 7, 1), (3, 256, 2), (4, 333, 3), (5, 513, 4), (8, 600, 5),
+                                        (12, 700, 6)])
 def test_sancho_sigma_matches_oracle(order, na, nw, seed):
     print("sancho parity order=%d na=%d nw=%d seed=%d" % (order, na, nw, seed), flush=True)
     GR, GL, U, delta = sancho_inputs(na, nw, seed)
