@@ -82,3 +82,29 @@ def test_sancho_rejects_bad_arguments():
         s.build(GR, GL, U, -1.0)
     with pytest.raises(ValueError):
         s.build(GR[:, :, :5], GL[:, :, :5], U, delta)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["order2_second_order", "order3_third_order", "order2_asymmetric_leads",
+                                  "legacy_natoms", "order2_custom_template"])
+def test_sancho_cases_with_hip_self_energies(case):
+    """SANCHO's own regression cases (SANCHO/tests/cases/) with the GPU
+    self-energies inside the reference's voltage sweep (tests/sancho_scf.py):
+    Volt_Current and populations within the reference's tolerance."""
+    import os
+    from qs_testlib import assert_close_table, read_table
+    from sancho_scf import SanchoSweep
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sancho", case)
+    print("sancho case %s with HIP self-energies" % case, flush=True)
+    handle = {}
+
+    def hip_sigma(GR, GL, U, delta, order):
+        if "s" not in handle:
+            handle["s"] = product().SanchoSelfEnergies(GR.shape[0], GR.shape[2], order)
+        return handle["s"].build(GR, GL, U, delta)
+
+    sw = SanchoSweep(os.path.join(gold, "input.dat"), hip_sigma)
+    iv, pops = sw.run()
+    for f in os.listdir(os.path.join(gold, "reference")):
+        new = iv if f.startswith("Volt_Current") else pops
+        assert_close_table(new, read_table(os.path.join(gold, "reference", f)))
