@@ -73,7 +73,26 @@ struct SnLaunch {
     SnJob job[2];
     const double *U;
     int na;
+    int nsplit;        // workgroups sharing one output block, each over its own run of tiles
+    double2 *part;     // nsplit > 1: raw partial sums [split][job][q][pair][p]
 };
+
+// acc of output p of pair (a, b), scaled and stored as job J asks
+template <int NX>
+__device__ __forceinline__ void sn_store(const SnLaunch &P, const SnJob &J, int a, int b, int p, const double2 *acc) {
+    const int na = P.na;
+    if (!J.fortran) {
+        const size_t o = ((size_t)a * na + b) * J.np + p;
+#pragma unroll
+        for (int q = 0; q < NX; q++) J.out[q][o] = cscale(acc[q], J.pp);
+    } else {
+        // (c U_a U_b pp) acc, as second_order_sigma / third_order_sigma scale it
+        const double2 s = cscale(J.cs, P.U[a] * P.U[b] * J.pp);
+        const size_t o = a + (size_t)na * b + (size_t)na * na * p;
+#pragma unroll
+        for (int q = 0; q < NX; q++) J.out[q][o] = cmul(s, acc[q]);
+    }
+}
 
 // NX = 4: the bubble sums  a0 += x0 y0, a1 += x1 y1, a2 += x2 y0 + x1 y2, a3 += x3 y0 + x1 y3
 //   (chi: x = G, L, R, A at (a,b), y = L, G, A, R at (b,a); phi: y = G, L, R, A at (a,b)).
@@ -86,8 +105,13 @@ __global__ __launch_bounds__(SN_NT) void k_sn_sum(SnLaunch P) {
     __shared__ double2 ys[NX][SN_W];
     const SnJob &J = P.job[blockIdx.z];
     const int na = P.na, pair = blockIdx.y, a = pair / na, b = pair % na;
-    const int p0 = blockIdx.x * SN_NT;
+    const int split = blockIdx.x % P.nsplit, p0 = (blockIdx.x / P.nsplit) * SN_NT;
     if (p0 >= J.np) return;  // uniform over the workgroup, before any barrier
+    // the job's tiles, numbered over its segments in order; this workgroup sums [tlo, thi)
+    int ntile = 0;
+    for (int s = 0; s < J.nseg; s++) ntile += (J.seg[s].nx + SN_TK - 1) / SN_TK;
+    const int tlo = (int)((long)ntile * split / P.nsplit), thi = (int)((long)ntile * (split + 1) / P.nsplit);
+    int tix = 0;
     const int t = threadIdx.x, p = p0 + t;
     double2 acc[NX];
 #pragma unroll
@@ -96,6 +120,8 @@ __global__ __launch_bounds__(SN_NT) void k_sn_sum(SnLaunch P) {
         const SnSeg &S = J.seg[s];
         const size_t xo = (size_t)pair * S.nx, yo = (size_t)(S.ytr ? b * na + a : pair) * S.ny;
         for (int k0 = 0; k0 < S.nx; k0 += SN_TK) {
+            const int mine = tix++;
+            if (mine < tlo || mine >= thi) continue;  // another split's tile
             // y index of (k, p) is yb + local, local = kk - t + NT - 1 (sig > 0) or t - kk + TK - 1
             const int yb = S.sig > 0 ? k0 - p0 - (SN_NT - 1) + S.c : p0 - k0 - (SN_TK - 1) + S.c;
             if (yb + (SN_W - 2) < 0 || yb >= S.ny) continue;  // the window misses the grid
@@ -142,17 +168,32 @@ __global__ __launch_bounds__(SN_NT) void k_sn_sum(SnLaunch P) {
         }
     }
     if (p >= J.np) return;
-    if (!J.fortran) {
-        const size_t o = (size_t)pair * J.np + p;
-#pragma unroll
-        for (int q = 0; q < NX; q++) J.out[q][o] = cscale(acc[q], J.pp);
+    if (P.nsplit == 1) {
+        sn_store<NX>(P, J, a, b, p, acc);
     } else {
-        // (c U_a U_b pp) acc, as second_order_sigma / third_order_sigma scale it
-        const double2 s = cscale(J.cs, P.U[a] * P.U[b] * J.pp);
-        const size_t o = a + (size_t)na * b + (size_t)na * na * p;
+        const size_t base = ((size_t)(split * gridDim.z + blockIdx.z) * NX * na * na + pair) * J.np + p;
+        const size_t qs = (size_t)na * na * J.np;
 #pragma unroll
-        for (int q = 0; q < NX; q++) J.out[q][o] = cmul(s, acc[q]);
+        for (int q = 0; q < NX; q++) P.part[base + q * qs] = acc[q];
     }
+}
+
+// sum of the nsplit partials of each output, in split order, then stored as k_sn_sum would
+template <int NX>
+__global__ __launch_bounds__(256) void k_sn_reduce(SnLaunch P) {
+    const SnJob &J = P.job[blockIdx.z];
+    const int na = P.na, pair = blockIdx.y, p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= J.np) return;
+    const size_t qs = (size_t)na * na * J.np;
+    double2 acc[NX];
+#pragma unroll
+    for (int q = 0; q < NX; q++) acc[q] = make_double2(0.0, 0.0);
+    for (int sp = 0; sp < P.nsplit; sp++) {
+        const size_t base = ((size_t)(sp * gridDim.z + blockIdx.z) * NX * na * na + pair) * J.np + p;
+#pragma unroll
+        for (int q = 0; q < NX; q++) acc[q] = cadd(acc[q], P.part[base + q * qs]);
+    }
+    sn_store<NX>(P, J, pair / na, pair % na, p, acc);
 }
 
 // S and P: sums over the internal site k with U_k (self_energies.f90:160-175), pointwise
@@ -187,8 +228,10 @@ __global__ __launch_bounds__(256) void k_sn_sp(int na, int L, const double *__re
 }  // namespace
 
 struct qs_sancho {
-    int na = 0, nw = 0, order = 0, device = 0;
+    int na = 0, nw = 0, order = 0, device = 0, cus = 256;
     hipStream_t stream = nullptr;
+    double2 *part = nullptr;  // partial sums of split launches
+    size_t part_elems = 0;
     double2 *g = nullptr;     // 4 na^2 nw
     double2 *bub = nullptr;   // (4 | 14) na^2 L
     double2 *gin = nullptr;   // 2 na^2 nw: host inputs staged
@@ -214,6 +257,34 @@ int sn_fail(qs_sancho *s, int code, const char *fmt, ...) {
     } while (0)
 
 size_t sn_nbub(int order) { return order == 3 ? 14 : 4; }
+
+// A launch of fewer than three 256-output workgroups per CU is latency-bound on
+// each workgroup's serial run of tiles: split that run over up to 4 workgroups
+// (never more than its tiles), which write partials for k_sn_reduce.
+int sn_nsplit(int np, int na, int nz, int ntile, int cus) {
+    const long wg = (long)((np + 255) / 256) * na * na * nz;
+    if (wg >= 3L * cus) return 1;
+    long n = (3L * cus + wg - 1) / wg;
+    n = n < 4 ? n : 4;
+    n = n < ntile ? n : ntile;
+    return n > 1 ? (int)n : 1;
+}
+size_t sn_part_elems(int nq, int np, int na, int nz, int ntile, int cus) {
+    const int n = sn_nsplit(np, na, nz, ntile, cus);
+    return n > 1 ? (size_t)n * nz * nq * na * na * np : 0;
+}
+template <int NX>
+hipError_t sn_launch(qs_sancho *s, SnLaunch &P, int np, int nz, int ntile, hipStream_t st) {
+    const int na = s->na, nb = (np + 255) / 256;
+    const int n = sn_nsplit(np, na, nz, ntile, s->cus);
+    P.nsplit = 1, P.part = nullptr;
+    if (n > 1 && (size_t)n * nz * NX * na * na * np <= s->part_elems) P.nsplit = n, P.part = s->part;
+    hipLaunchKernelGGL(k_sn_sum<NX>, dim3(nb * P.nsplit, na * na, nz), dim3(SN_NT), 0, st, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || P.nsplit == 1) return e;
+    hipLaunchKernelGGL(k_sn_reduce<NX>, dim3(nb, na * na, nz), dim3(256), 0, st, P);
+    return hipGetLastError();
+}
 }  // namespace
 
 extern "C" {
@@ -228,14 +299,22 @@ int qs_sancho_create(int32_t na, int32_t n_omega, int32_t order, int32_t device,
     s->na = na, s->nw = n_omega, s->order = order, s->device = device;
     const size_t nG = (size_t)na * na * n_omega, nB = (size_t)na * na * (2 * (size_t)n_omega - 1);
     int rc = QS_OK;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    const int L = 2 * n_omega - 1, nz = order == 3 ? 2 : 1;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         rc = QS_ERR_HIP;
-    else if (hipMalloc(&s->g, 4 * nG * sizeof(double2)) != hipSuccess ||
-             hipMalloc(&s->bub, sn_nbub(order) * nB * sizeof(double2)) != hipSuccess ||
-             hipMalloc(&s->gin, 2 * nG * sizeof(double2)) != hipSuccess ||
-             hipMalloc(&s->out, 6 * nG * sizeof(double2)) != hipSuccess ||
-             hipMalloc(&s->U, na * sizeof(double)) != hipSuccess)
+    if (rc == QS_OK && (hipMalloc(&s->g, 4 * nG * sizeof(double2)) != hipSuccess ||
+                        hipMalloc(&s->bub, sn_nbub(order) * nB * sizeof(double2)) != hipSuccess ||
+                        hipMalloc(&s->gin, 2 * nG * sizeof(double2)) != hipSuccess ||
+                        hipMalloc(&s->out, 6 * nG * sizeof(double2)) != hipSuccess ||
+                        hipMalloc(&s->U, na * sizeof(double)) != hipSuccess))
         rc = QS_ERR_NOMEM;
+    if (rc == QS_OK) {  // partials of the bubble and self-energy launches, when they split
+        const size_t e4 = sn_part_elems(4, L, na, nz, (n_omega + 255) / 256, s->cus);
+        const size_t e3 = sn_part_elems(3, n_omega, na, nz, (L + 255) / 256, s->cus);
+        s->part_elems = e4 > e3 ? e4 : e3;
+        if (s->part_elems && hipMalloc(&s->part, s->part_elems * sizeof(double2)) != hipSuccess) rc = QS_ERR_NOMEM;
+    }
     if (rc != QS_OK) {
         qs_sancho_destroy(s);
         return rc;
@@ -253,6 +332,7 @@ void qs_sancho_destroy(qs_sancho *s) {
     (void)hipFree(s->gin);
     (void)hipFree(s->out);
     (void)hipFree(s->U);
+    (void)hipFree(s->part);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -262,7 +342,8 @@ const char *qs_sancho_last_error(const qs_sancho *s) { return s ? s->err.c_str()
 int64_t qs_sancho_device_bytes(const qs_sancho *s) {
     if (!s) return -1;
     const int64_t nG = (int64_t)s->na * s->na * s->nw, nB = (int64_t)s->na * s->na * (2 * (int64_t)s->nw - 1);
-    return (int64_t)sizeof(double2) * (12 * nG + (int64_t)sn_nbub(s->order) * nB) + (int64_t)sizeof(double) * s->na;
+    return (int64_t)sizeof(double2) * (12 * nG + (int64_t)sn_nbub(s->order) * nB + (int64_t)s->part_elems) +
+           (int64_t)sizeof(double) * s->na;
 }
 
 int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
@@ -304,8 +385,7 @@ int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, co
         f = c;
         f.seg[0] = SnSeg{{gG, gL, gR, gA}, {gG, gL, gR, gA}, nw, nw, -1, 0, 0};
         for (int q = 0; q < 4; q++) f.out[q] = B(4 + q);
-        hipLaunchKernelGGL(k_sn_sum<4>, dim3((L + SN_NT - 1) / SN_NT, na * na, o3 ? 2 : 1), dim3(SN_NT), 0, st, P);
-        SNCHK(s, hipGetLastError());
+        SNCHK(s, sn_launch<4>(s, P, L, o3 ? 2 : 1, (nw + SN_TK - 1) / SN_TK, st));
     }
     if (o3) {
         hipLaunchKernelGGL(k_sn_sp, dim3((L + 255) / 256, na * na, 2), dim3(256), 0, st, na, L, (const double *)s->U,
@@ -327,8 +407,7 @@ int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, co
         j3.seg[0] = SnSeg{{B(9), B(8), B(10), nullptr}, {gL, gG, gR, nullptr}, L, nw, -1, nw - 1, 0};
         j3.seg[1] = SnSeg{{B(12), B(11), B(13), nullptr}, {gG, gL, gA, nullptr}, L, nw, +1, 0, 1};
         j3.out[0] = o[4], j3.out[1] = o[5], j3.out[2] = o[3];
-        hipLaunchKernelGGL(k_sn_sum<3>, dim3((nw + SN_NT - 1) / SN_NT, na * na, o3 ? 2 : 1), dim3(SN_NT), 0, st, P);
-        SNCHK(s, hipGetLastError());
+        SNCHK(s, sn_launch<3>(s, P, nw, o3 ? 2 : 1, (L + SN_TK - 1) / SN_TK, st));
     }
     if (!on_device)
         for (int q = 0; q < (o3 ? 6 : 3); q++)
