@@ -17,6 +17,9 @@
 // which is exactly the reference's loop bounds (k1 = max(1, 1+m) .. min(nw, nw+m)
 // and the like).  Tiles whose window misses the grid are skipped.  x reads are
 // LDS broadcasts; y reads by consecutive lanes hit consecutive 16-byte slots.
+// A launch too small to fill the GPU splits each output block's run of tiles
+// over several workgroups, which write partial sums; k_sn_reduce adds them in
+// a fixed order, so the results do not depend on scheduling.
 //
 // Layouts in HBM (na sites, nw energies, L = 2 nw - 1):
 //   g   [X][a][b][k]   X = R, A, L, G (frequency-first copies of the completed input)
