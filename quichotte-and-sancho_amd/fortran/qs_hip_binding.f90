@@ -47,5 +47,20 @@ module qs_hip_binding
          real(c_double), intent(in) :: x(*); real(c_double), intent(out) :: F(*)
          integer(c_int32_t), intent(out) :: abort
       end function
+      ! SANCHO's order-2/3 self-energies: complete_input_gf + build_self_energies
+      ! (SANCHO/src/green_functions.f90:163-173, SANCHO/src/self_energies.f90:38-46)
+      integer(c_int) function qs_sancho_create(na, nw, order, device, h) bind(C, name='qs_sancho_create')
+         import; integer(c_int32_t), value :: na, nw, order, device; type(c_ptr), intent(out) :: h
+      end function
+      subroutine qs_sancho_destroy(h) bind(C, name='qs_sancho_destroy')
+         import; type(c_ptr), value :: h
+      end subroutine
+      integer(c_int) function qs_sancho_self_energies(h, delta, U, GR, GL, S2R, S2L, S2G, S3R, S3L, S3G, &
+                                                      on_device) bind(C, name='qs_sancho_self_energies')
+         import; type(c_ptr), value :: h; real(c_double), value :: delta; real(c_double), intent(in) :: U(*)
+         complex(c_double_complex), intent(in) :: GR(*), GL(*)
+         complex(c_double_complex) :: S2R(*), S2L(*), S2G(*), S3R(*), S3L(*), S3G(*)
+         integer(c_int32_t), value :: on_device
+      end function
    end interface
 end module qs_hip_binding

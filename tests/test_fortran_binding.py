@@ -72,3 +72,42 @@ def test_fortran_loop_matches_oracle(tmp_path):
         assert abs(row[1] - ju) <= 1e-10 * abs(ju) + 1e-11
         assert abs(row[2] - jd) <= 1e-10 * abs(jd) + 1e-11
         assert int(row[3]) == 1 and int(row[4]) == ro["n_iter"]
+
+
+def build_sancho(tmp):
+    if not os.path.exists(FC):
+        pytest.skip("no Fortran compiler")
+    subprocess.run([FC, "-O2", "-c", os.path.join(FDIR, "qs_hip_binding.f90"), "-o", str(tmp / "b.o"),
+                    "-J" + str(tmp)], check=True)
+    exe = str(tmp / "sancho_demo")
+    subprocess.run([FC, "-O2", "-J" + str(tmp), os.path.join(FDIR, "qs_sancho_demo.f90"), str(tmp / "b.o"),
+                    "-L" + LIB, "-lqs_hip", "-Wl,-rpath," + LIB, "-o", exe], check=True)
+    return exe
+
+
+def test_fortran_sancho_binding_builds(tmp_path):
+    assert os.path.exists(build_sancho(tmp_path))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 3])
+def test_fortran_sancho_self_energies_match_oracle(tmp_path, order):
+    from qs_testlib import sancho_inputs, sancho_oracle
+    exe = build_sancho(tmp_path)
+    na, nw = 4, 200
+    print("fortran sancho order=%d na=%d nw=%d" % (order, na, nw), flush=True)
+    GR, GL, U, delta = sancho_inputs(na, nw, 31)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(np.array([na, nw, order], np.int32).tobytes())
+        f.write(np.array([delta], np.float64).tobytes())
+        f.write(U.astype(np.float64).tobytes())
+        f.write(np.asfortranarray(GR).tobytes(order="F"))
+        f.write(np.asfortranarray(GL).tobytes(order="F"))
+    r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nout = 6 if order == 3 else 3
+    got = np.frombuffer(open(fout, "rb").read(), np.complex128).reshape((na, na, nw, nout), order="F")
+    ref = sancho_oracle(GR, GL, U, delta, order)
+    for q in range(nout):
+        assert np.max(np.abs(got[:, :, :, q] - ref[q])) <= 1e-12 * np.max(np.abs(ref[q]))
