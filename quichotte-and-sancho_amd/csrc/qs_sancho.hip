@@ -349,10 +349,12 @@ int64_t qs_sancho_device_bytes(const qs_sancho *s) {
            (int64_t)sizeof(double) * s->na;
 }
 
-int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
-                            double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L,
-                            double *Sig3G, int32_t on_device) {
-    if (!s) return QS_ERR_ARG;
+}  // extern "C"
+
+namespace {
+int sn_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
+                     double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L, double *Sig3G,
+                     int32_t on_device) {
     if (!U_site || !GR || !GL || !Sig2R || !Sig2L || !Sig2G)
         return sn_fail(s, QS_ERR_ARG, "null input or second-order output");
     if (s->order == 3 && (!Sig3R || !Sig3L || !Sig3G)) return sn_fail(s, QS_ERR_ARG, "order 3 needs the Sig3 outputs");
@@ -417,6 +419,20 @@ int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, co
             SNCHK(s, hipMemcpyAsync(user[q], o[q], nG * sizeof(double2), hipMemcpyDeviceToHost, st));
     SNCHK(s, hipStreamSynchronize(st));
     return QS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
+                            double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L,
+                            double *Sig3G, int32_t on_device) {
+    if (!s) return QS_ERR_ARG;
+    const int rc = sn_self_energies(s, delta, U_site, GR, GL, Sig2R, Sig2L, Sig2G, Sig3R, Sig3L, Sig3G, on_device);
+    // after a failure part-way, work already queued may still read or write the
+    // caller's arrays: let it finish before handing them back
+    if (rc != QS_OK && s->stream) (void)hipStreamSynchronize(s->stream);
+    return rc;
 }
 
 }  // extern "C"
