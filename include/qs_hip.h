@@ -148,6 +148,13 @@ int64_t qs_sancho_device_bytes(const qs_sancho *s);
 int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
                             double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L,
                             double *Sig3G, int32_t on_device);
+/* measurement hooks (no counterpart in the reference), as qs_profile above:
+ * enable (1) or disable (0) HIP-event timing of the two summation stages on the
+ * handle's stream (0: bubbles chi/phi, 1: self-energies; each with its
+ * partial-sum reduction); resets the totals.  qs_sancho_profile_read fills
+ * ms[2] and count[2] and returns 2, or a negative status. */
+int qs_sancho_profile(qs_sancho *s, int32_t enable);
+int qs_sancho_profile_read(const qs_sancho *s, double *ms, int64_t *count);
 
 #ifdef __cplusplus
 }

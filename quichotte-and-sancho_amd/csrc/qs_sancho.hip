@@ -235,6 +235,11 @@ struct qs_sancho {
     hipStream_t stream = nullptr;
     double2 *part = nullptr;  // partial sums of split launches
     size_t part_elems = 0;
+    // optional HIP-event timing of the bubble and self-energy stages on the handle's stream
+    bool prof = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double prof_ms[2] = {0.0, 0.0};
+    int64_t prof_n[2] = {0, 0};
     double2 *g = nullptr;     // 4 na^2 nw
     double2 *bub = nullptr;   // (4 | 14) na^2 L
     double2 *gin = nullptr;   // 2 na^2 nw: host inputs staged
@@ -336,11 +341,32 @@ void qs_sancho_destroy(qs_sancho *s) {
     (void)hipFree(s->out);
     (void)hipFree(s->U);
     (void)hipFree(s->part);
+    for (hipEvent_t e : s->ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
 const char *qs_sancho_last_error(const qs_sancho *s) { return s ? s->err.c_str() : "null handle"; }
+
+int qs_sancho_profile(qs_sancho *s, int32_t enable) {
+    if (!s) return QS_ERR_ARG;
+    if (enable) {
+        SNCHK(s, hipSetDevice(s->device));
+        for (hipEvent_t &e : s->ev)
+            if (!e) SNCHK(s, hipEventCreate(&e));
+    }
+    s->prof = enable != 0;
+    s->prof_ms[0] = s->prof_ms[1] = 0.0;
+    s->prof_n[0] = s->prof_n[1] = 0;
+    return QS_OK;
+}
+
+int qs_sancho_profile_read(const qs_sancho *s, double *ms, int64_t *count) {
+    if (!s || !ms || !count) return -QS_ERR_ARG;
+    for (int k = 0; k < 2; k++) ms[k] = s->prof_ms[k], count[k] = s->prof_n[k];
+    return 2;
+}
 
 int64_t qs_sancho_device_bytes(const qs_sancho *s) {
     if (!s) return -1;
@@ -390,7 +416,9 @@ int sn_self_energies(qs_sancho *s, double delta, const double *U_site, const dou
         f = c;
         f.seg[0] = SnSeg{{gG, gL, gR, gA}, {gG, gL, gR, gA}, nw, nw, -1, 0, 0};
         for (int q = 0; q < 4; q++) f.out[q] = B(4 + q);
+        if (s->prof) SNCHK(s, hipEventRecord(s->ev[0], st));
         SNCHK(s, sn_launch<4>(s, P, L, o3 ? 2 : 1, (nw + SN_TK - 1) / SN_TK, st));
+        if (s->prof) SNCHK(s, hipEventRecord(s->ev[1], st));
     }
     if (o3) {
         hipLaunchKernelGGL(k_sn_sp, dim3((L + 255) / 256, na * na, 2), dim3(256), 0, st, na, L, (const double *)s->U,
@@ -412,12 +440,21 @@ int sn_self_energies(qs_sancho *s, double delta, const double *U_site, const dou
         j3.seg[0] = SnSeg{{B(9), B(8), B(10), nullptr}, {gL, gG, gR, nullptr}, L, nw, -1, nw - 1, 0};
         j3.seg[1] = SnSeg{{B(12), B(11), B(13), nullptr}, {gG, gL, gA, nullptr}, L, nw, +1, 0, 1};
         j3.out[0] = o[4], j3.out[1] = o[5], j3.out[2] = o[3];
+        if (s->prof) SNCHK(s, hipEventRecord(s->ev[2], st));
         SNCHK(s, sn_launch<3>(s, P, nw, o3 ? 2 : 1, (L + SN_TK - 1) / SN_TK, st));
+        if (s->prof) SNCHK(s, hipEventRecord(s->ev[3], st));
     }
     if (!on_device)
         for (int q = 0; q < (o3 ? 6 : 3); q++)
             SNCHK(s, hipMemcpyAsync(user[q], o[q], nG * sizeof(double2), hipMemcpyDeviceToHost, st));
     SNCHK(s, hipStreamSynchronize(st));
+    if (s->prof)
+        for (int k = 0; k < 2; k++) {
+            float ms = 0.0f;
+            SNCHK(s, hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
+            s->prof_ms[k] += ms;
+            s->prof_n[k]++;
+        }
     return QS_OK;
 }
 }  // namespace

@@ -92,6 +92,8 @@ def load_library(path=SO):
     L.qs_sancho_device_bytes.argtypes = [P]
     L.qs_sancho_device_bytes.restype = I64
     L.qs_sancho_self_energies.argtypes = [P, D, P, P, P, P, P, P, P, P, P, I32]
+    L.qs_sancho_profile.argtypes = [P, I32]
+    L.qs_sancho_profile_read.argtypes = [P, P, P]
     _lib = L
     return L
 
@@ -408,6 +410,20 @@ class SanchoSelfEnergies:
 
     def device_bytes(self):
         return int(self.L.qs_sancho_device_bytes(self.h))
+
+    def profile(self, enable=True):
+        rc = self.L.qs_sancho_profile(self.h, int(bool(enable)))
+        if rc != 0:
+            raise QsError(rc, self.L.qs_sancho_last_error(self.h).decode())
+
+    def profile_read(self):
+        """{"bubbles": (ms, builds), "self_energies": (ms, builds)} since profile(True)"""
+        ms = np.zeros(2)
+        cnt = np.zeros(2, dtype=np.int64)
+        n = self.L.qs_sancho_profile_read(self.h, _ptr(ms), _ptr(cnt))
+        if n < 0:
+            raise QsError(-n, "qs_sancho_profile_read")
+        return {"bubbles": (float(ms[0]), int(cnt[0])), "self_energies": (float(ms[1]), int(cnt[1]))}
 
     def _call(self, delta, U_site, gr, gl, outs, on_device):
         U = np.ascontiguousarray(U_site, dtype=np.float64)

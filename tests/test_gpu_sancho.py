@@ -108,3 +108,20 @@ def test_sancho_cases_with_hip_self_energies(case):
     for f in os.listdir(os.path.join(gold, "reference")):
         new = iv if f.startswith("Volt_Current") else pops
         assert_close_table(new, read_table(os.path.join(gold, "reference", f)))
+
+
+@pytest.mark.gpu
+def test_sancho_profile_counts_builds():
+    na, nw = 4, 300
+    print("sancho profile na=%d nw=%d" % (na, nw), flush=True)
+    GR, GL, U, delta = sancho_inputs(na, nw, 41)
+    s = product().SanchoSelfEnergies(na, nw, 3)
+    s.profile(True)
+    for _ in range(3):
+        s.build(GR, GL, U, delta)
+    prof = s.profile_read()
+    assert prof["bubbles"][1] == 3 and prof["self_energies"][1] == 3
+    assert prof["bubbles"][0] > 0 and prof["self_energies"][0] > 0
+    s.profile(False)
+    s.build(GR, GL, U, delta)
+    assert s.profile_read()["bubbles"][1] == 0
