@@ -98,6 +98,13 @@ def workload(name):
     return prob, w["volts"], desc
 
 
+def sweep_points(nv, steps):
+    """Indices of the voltages the timed steps solve: the voltage loop with
+    continuation (src/quichotte.f90:125-143) from a fresh start at its first
+    point; past the last point it starts over (fresh) at the first."""
+    return [k % nv for k in range(steps)]
+
+
 def stage_counts(eng, nh):
     """Algorithmic work per launch of each stage (flops or HBM bytes)."""
     nw, norb, nA, nv, Nf = eng.prob.n_omega, eng.norb, eng.nA, eng.n_vec, eng.Nf
@@ -133,13 +140,53 @@ def measured_traffic(workload_name, stage):
     return None
 
 
-def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
-    """Oracle timed on bounded samples of one SCF iteration of the same workload."""
+def host_cores():
+    """Physical cores this process may use: the distinct (package, core) pairs of
+    the CPUs in its affinity mask, capped by a cgroup CPU quota and by
+    OMP_NUM_THREADS when either is set (the host's share of a shared node)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    phys, cur = set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in list(f) + [""]:
+                if not line.strip():
+                    if cur.get("processor") in cpus:
+                        phys.add((cur.get("physical id", 0), cur.get("core id", cur["processor"])))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("processor", "physical id", "core id"):
+                    cur[k] = int(v)
+    except (OSError, ValueError):
+        pass
+    n = len(phys) or len(cpus)
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            n = min(n, max(1, int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return max(1, n)
+
+
+def cpu_baseline_child(args):
+    """Runs in its own process (OpenMP placement set before libgomp loads):
+    the FP64 oracle on the workload's host cores.  Two timings:
+      * the workload itself, extrapolated from bounded samples of one SCF
+        iteration (a full bias point of 80 spin-orbitals x 2^16 energies takes
+        the oracle tens of minutes);
+      * complete bias points of the 16-spin-orbital helix at 2^14 energies
+        (V = 0 fresh, then V = 0.1 with continuation), timed end to end."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import ctypes
 
-    from qs_testlib import _OrcProblem, oracle_lib, ptr
+    from qs_testlib import Oracle, _OrcProblem, helix_problem, oracle_lib, ptr
 
+    threads = args.cpu_threads
+    prob, volts, _ = workload(args.workload)
     L = oracle_lib()
     L.orc_bench_dyson.restype = ctypes.c_double
     L.orc_bench_dyson.argtypes = [ctypes.c_void_p]
@@ -151,6 +198,7 @@ def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
     nw = prob["n_omega"]
     nA, nAs = n, n // 2
     n_vec = 3 * nA * nA * nw
+    n_iter_avg = args.cpu_iters
     # Dyson/Keldysh: a sample of energies
     e_sample = max(threads * 8, 64)
     H = np.asfortranarray(prob["H"])
@@ -165,10 +213,9 @@ def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
     p.alpha_mix, p.nhist, p.maxscf, p.tolf, p.tolf_order1 = 0.5, 16, 1, 1e-8, 1e-8
     p.single_history, p.nthreads = 1, threads
     t_dys = L.orc_bench_dyson(ctypes.byref(p)) / e_sample
-    # Sigma2: forward transforms of 2 nA^2 columns and nAs^2 site pairs
+    # Sigma2: zero-padded forward transforms of 2 nA^2 columns and nAs^2 site pairs
     cols, pairs = max(threads * 4, 32), max(threads, 16)
     t_s2 = L.orc_bench_sigma2(nw, cols, pairs, threads)
-    # per-column / per-pair split: time columns alone
     t_cols = L.orc_bench_sigma2(nw, cols, 0, threads)
     t_col = t_cols / cols
     t_pair = max(t_s2 - t_cols, 0.0) / pairs
@@ -177,12 +224,47 @@ def cpu_baseline(prob, n_iter_avg, threads, budget_s=20.0):
     t_br = L.orc_bench_broyden(nb, 16, threads) / nb
     t_iter = t_dys * nw + t_col * 2 * nA * nA + t_pair * nAs * nAs + t_br * n_vec
     t_point = t_iter * n_iter_avg + t_dys * nw   # + the final sweep
-    sample = ("oracle (C, OpenMP %d threads) timed on %d energies of the Dyson/Keldysh sweep, %d zero-padded "
-              "length-%d FFT columns and %d site pairs of Sigma2, and one Broyden step on %d elements; scaled to "
-              "one SCF iteration (%.1f s) x %.1f iterations per bias point (the GPU run's mean) + the final sweep"
-              % (threads, e_sample, cols, 2 * nw, pairs, nb, t_iter, n_iter_avg))
-    return {"value": 1.0 / t_point, "unit": "bias points/s", "cores": threads, "kind": "port", "sample": sample,
-            "s_per_iteration": t_iter}
+    out = {"value": 1.0 / t_point, "unit": "bias points/s", "cores": threads, "kind": "port",
+           "extrapolated": True,
+           "sample": ("oracle (C, OpenMP, %d threads, OMP_PROC_BIND=close, OMP_PLACES=cores) timed on %d energies "
+                      "of the Dyson/Keldysh sweep, %d zero-padded length-%d FFT columns and %d site pairs of "
+                      "Sigma2, and one Broyden step on %d elements; EXTRAPOLATED to one SCF iteration (%.1f s) x "
+                      "%.1f iterations per bias point (the GPU run's mean) + the final sweep"
+                      % (threads, e_sample, cols, 2 * nw, pairs, nb, t_iter, n_iter_avg)),
+           "s_per_iteration": t_iter}
+    # complete bias points of the 16-spin-orbital helix (not extrapolated)
+    p16, v16, _ = workload("helix16")
+    pr = helix_problem(4, 2, 1.0, order=2, n_omega=p16["n_omega"], wlo=p16["w_init"],
+                       whi=p16["w_init"] + p16["n_omega"] * p16["delta"], E_CC=0.0, t_hop=-1.0, lamb=0.01,
+                       gamma=0.1, mu=0.0, tolf=1e-8, nhist=16, voltages=v16[:2])
+    o = Oracle(pr, nthreads=threads)
+    t0 = time.perf_counter()
+    steps, fresh = [], True
+    for V in v16[:2]:
+        r = o.solve(V, fresh)
+        fresh = not r["converged"]
+        steps.append(r["n_iter"])
+    tc = time.perf_counter() - t0
+    out["helix16_full_solve"] = {"value": 2.0 / tc, "unit": "bias points/s", "cores": threads,
+                                 "sample": "two complete bias points (V = 0 fresh, V = 0.1 continued; %d and %d SCF "
+                                           "steps) of the 16-spin-orbital helix at 2^14 energies, timed end to end"
+                                           % tuple(steps)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(args, n_iter_avg):
+    """The oracle in a child process, so that OpenMP placement is set before
+    its runtime starts; returns the cpu_baseline object (or an error note)."""
+    import subprocess
+    threads = host_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--workload", args.workload,
+           "--cpu-threads", str(threads), "--cpu-iters", "%.3f" % n_iter_avg]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"value": None, "error": (r.stderr or r.stdout)[-400:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def serve_bench(args):
@@ -344,14 +426,16 @@ def copy_calibration():
     return 0
 
 
-def main():
+def make_parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="helix80", choices=["helix80", "helix16", "helix240"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: the host's physical cores)")
+    ap.add_argument("--cpu-iters", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve", default="", help="comma-separated Poisson arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -361,12 +445,20 @@ def main():
                     help="time SANCHO's order-2/3 self-energy build on this workload instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 2 GiB five times (scale for HBM byte counters) and exit")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = make_parser().parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (one bias point is one "
               "GPU's work; several GPUs would each take their own bias points or energy ranges)" % args.gpus,
               file=sys.stderr)
         return 2
+    if args.cpu_baseline_child:
+        return cpu_baseline_child(args)
+    if args.cpu_threads <= 0:
+        args.cpu_threads = host_cores()
     if args.serve:
         return serve_bench(args)
     if args.sancho:
@@ -378,35 +470,37 @@ def main():
     prob, volts, desc = workload(args.workload)
     if args.maxscf > 0:
         prob["maxscf"] = args.maxscf
-    nsteps = args.warmup + args.steps
-    if nsteps > len(volts):
-        raise SystemExit("at most %d steps for this workload" % len(volts))
+    nv = len(volts)
     print("bench: %s" % desc, file=sys.stderr, flush=True)
     t0 = time.time()
     eng = pkg.ScfEngine(pkg.ScfProblem(**prob))
     dv, dms = eng.dyson_choice()
     print("bench: device memory %.1f GB, setup %.1f s, Dyson launch shape %d (tuned ms %s)" %
           (eng.device_bytes() / 1e9, time.time() - t0, dv, dms), file=sys.stderr, flush=True)
-    fresh = True
+    # warmup: fresh solves of sweep points; they leave no state the timed sweep uses
     for k in range(args.warmup):
         t1 = time.time()
-        r = eng.solve_scf(volts[k], fresh)
-        fresh = not r.converged
-        print("bench: warmup V=%+.2f converged=%s steps=%d %.2f s" % (volts[k], r.converged, r.n_iter,
-                                                                      time.time() - t1), file=sys.stderr, flush=True)
+        r = eng.solve_scf(volts[k % nv], True)
+        print("bench: warmup (fresh) V=%+.2f converged=%s steps=%d %.2f s" % (volts[k % nv], r.converged, r.n_iter,
+                                                                              time.time() - t1),
+              file=sys.stderr, flush=True)
     eng.profile(True)
     lat, iters, conv = [], [], []
+    fresh = True
     tstart = time.perf_counter()
-    for k in range(args.warmup, nsteps):
+    for k, i in enumerate(sweep_points(nv, args.steps)):
+        if i == 0:
+            fresh = True
         t1 = time.perf_counter()
-        r = eng.solve_scf(volts[k], fresh)
+        r = eng.solve_scf(volts[i], fresh)
         J = eng.current()
         lat.append(time.perf_counter() - t1)
         fresh = not r.converged
         iters.append(r.n_iter)
         conv.append(r.converged)
-        print("bench: V=%+.2f converged=%s steps=%d J=(%.6e, %.6e) %.3f s" % (volts[k], r.converged, r.n_iter, J[0], J[1],
-                                                                              lat[-1]), file=sys.stderr, flush=True)
+        print("bench: V=%+.2f %s converged=%s steps=%d J=(%.6e, %.6e) %.3f s" %
+              (volts[i], "fresh" if (i == 0 or (k and not conv[-2])) else "cont.", r.converged, r.n_iter, J[0], J[1],
+               lat[-1]), file=sys.stderr, flush=True)
     total = time.perf_counter() - tstart
     prof = eng.profile_read()
     eng.profile(False)
@@ -417,7 +511,10 @@ def main():
     avg_s = ms / 1e3 / nl
     if kind == "flop":
         ach = work / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        # "mfma" only where the kernel issues matrix instructions; the register-resident
+        # Dyson shapes do their FP64 arithmetic as vector FMAs (same 78.6 TFLOP/s peak)
+        roof = {"bound": "mfma" if eng.dyson_uses_mfma() else "fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s",
                 "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
     else:
         ach = work / avg_s / 1e9
@@ -441,10 +538,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "form": "plain FP64 path, one bias point at a time",
+                   "sweep": ("%d timed bias points walking the %d-point voltage sweep with continuation from a fresh "
+                             "start at V=%g%s; %d warmup points solved fresh beforehand"
+                             % (args.steps, nv, volts[0], ", starting over fresh after the last point"
+                                if args.steps > nv else "", args.warmup)),
                    "n_spin_orbitals": eng.norb, "n_omega": eng.prob.n_omega, "fft_length": eng.Nf,
-                   "dyson_launch_shape": ["256-thread workgroup, matrix in LDS", "512-thread workgroup, matrix in LDS",
-                                          "small-LDS workgroup, scratch slab",
-                                          "large-Norb chunks: batched zgetrf/zgetri + zgemm"][dv]},
+                   "dyson_launch_shape": eng.dyson_shape_name()},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "scf_steps_per_point": float(np.mean(iters)),
         "all_converged": bool(all(conv)),
@@ -452,8 +551,8 @@ def main():
         "roofline": roof,
     }
     if not args.no_cpu_baseline:
-        print("bench: cpu baseline (oracle sample, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(prob, float(np.mean(iters)), args.cpu_threads)
+        print("bench: cpu baseline (oracle, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args, float(np.mean(iters)))
     eng.close()
     print(json.dumps(line), flush=True)
     return 0

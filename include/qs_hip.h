@@ -128,6 +128,10 @@ const char *qs_stage_name(int32_t stage);
  * the timed ms of each on a slice of the grid (0 for a shape that does not
  * apply to this size); returns the number of shapes */
 int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax);
+/* a description of the launch shape the handle uses; *uses_mfma (may be NULL)
+ * is 1 when its dense contractions issue MFMA instructions, 0 when they are
+ * vector FMAs or library calls */
+const char *qs_dyson_shape(const qs_handle *h, int32_t *uses_mfma);
 
 /* ---- SANCHO: interaction self-energies of orders 2 and 3 ----------------
  * The work arrays of SANCHO's SelfEnergies module (SANCHO/src/self_energies.f90:23-35,
@@ -143,8 +147,11 @@ int64_t qs_sancho_device_bytes(const qs_sancho *s);
  * Sig2R/L/G and, at order 3, Sig3R/L/G.  GR, GL and the Sig arrays are
  * (na, na, n_omega) complex128, column-major (G_in%r, G_in%L, Sig2R, ...);
  * U_site (na) and delta are host values.  on_device = 0: the arrays are host
- * memory; 1: device memory on the handle's device.  Synchronous.  Calls on one
- * handle must not overlap. */
+ * memory; 1: device memory on the handle's device.  Synchronous: returns when
+ * the outputs are written.  The work runs on the handle's own stream, which is
+ * not ordered after any stream of the caller, so device inputs must be complete
+ * (the caller's writes to them finished) before the call.  Calls on one handle
+ * must not overlap. */
 int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
                             double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L,
                             double *Sig3G, int32_t on_device);

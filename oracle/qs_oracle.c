@@ -175,7 +175,33 @@ static void fft_rec(const zc *in, zc *out, int n, int s, const int *fac, const f
     }
 }
 
+/* Power-of-two lengths: iterative radix-2 Stockham (autosorting, no bit
+ * reversal), the same DFT as fft_rec.  Stage with half-length l and stride m
+ * (n = 2 l m): b[k + 2jm] = a[k + jm] + a[k + jm + lm],
+ * b[k + 2jm + m] = w^j (a[k + jm] - a[k + jm + lm]), w = e^{-+2 pi i m / n}. */
+static void fft_pow2(const fftplan *pl, int sign, zc *data, zc *scratch) {
+    const int n = pl->n;
+    zc *a = data, *b = scratch;
+    for (int l = n / 2, m = 1; l >= 1; l /= 2, m *= 2) {
+        for (int j = 0; j < l; j++) {
+            const zc w = pl->tw[(size_t)j * m];
+            const double wr = creal(w), wi = sign > 0 ? -cimag(w) : cimag(w);
+            const zc *a0 = a + (size_t)j * m, *a1 = a + (size_t)j * m + (size_t)l * m;
+            zc *b0 = b + (size_t)2 * j * m, *b1 = b0 + m;
+            for (int k = 0; k < m; k++) {
+                const double xr = creal(a0[k]), xi = cimag(a0[k]), yr = creal(a1[k]), yi = cimag(a1[k]);
+                const double dr = xr - yr, di = xi - yi;
+                b0[k] = (xr + yr) + I * (xi + yi);
+                b1[k] = (wr * dr - wi * di) + I * (wr * di + wi * dr);
+            }
+        }
+        zc *t = a; a = b; b = t;
+    }
+    if (a != data) memcpy(data, a, sizeof(zc) * n);
+}
+
 static void fft_exec(const fftplan *pl, int sign, zc *data, zc *scratch) {
+    if ((pl->n & (pl->n - 1)) == 0) { fft_pow2(pl, sign, data, scratch); return; }
     fft_rec(data, scratch, pl->n, 1, pl->fac, pl, sign);
     memcpy(data, scratch, sizeof(zc) * pl->n);
 }
@@ -221,7 +247,8 @@ struct orc_state {
     int64_t n_vec;
     int use_sigma;
     double V_scf;
-    int runaway_final, unphys_final;
+    int runaway_final, unphys_final, iworst_final;
+    double nworst_final, wworst_final;
     volatile int err;
 };
 
@@ -409,6 +436,7 @@ void orc_unpack_gf(const double *in, double *Grd, double *GLd, int nb, int Nw) {
     zc *Gr = (zc *)Grd, *GL = (zc *)GLd;
     int n_full = nb * nb, n_su = (n_full - nb) / 2;
     size_t nblock = 2 * (size_t)n_full + n_su + (n_su + nb);
+#pragma omp parallel for schedule(static)
     for (int iw = 0; iw < Nw; iw++) {
         size_t off = iw * nblock;
         zc *r = Gr + (size_t)iw * n_full, *l = GL + (size_t)iw * n_full;
@@ -807,6 +835,7 @@ static void sigma2_compute(orc_state *s) {
 /* ------------------------------------------------------------------------ */
 static double nrm2(int64_t n, const double *v) {
     double acc = 0;
+#pragma omp parallel for reduction(+ : acc) schedule(static)
     for (int64_t i = 0; i < n; i++) acc += v[i] * v[i];
     return sqrt(acc);
 }
@@ -861,6 +890,7 @@ void orc_broyden(orc_residual_fn fn, void *ctx, double *x, int64_t n, double alp
         fn(ctx, n, x, F, &abort_flag);
         if (abort_flag) break;
         fmax = 0;
+#pragma omp parallel for reduction(max : fmax) schedule(static)
         for (int64_t i = 0; i < n; i++) if (fabs(F[i]) > fmax) fmax = fabs(F[i]);
         double rms = nrm2(n, F) / sqrt((double)n);
         if (log_fmax) log_fmax[its - 1] = fmax;
@@ -869,6 +899,7 @@ void orc_broyden(orc_residual_fn fn, void *ctx, double *x, int64_t n, double alp
         if (its > 1 && nhist > 0) {
             int slot = nstored % nhist;
             if (sp) {
+#pragma omp parallel for schedule(static)
                 for (int64_t i = 0; i < n; i++) Fold[i] = F[i] - Fold[i];
                 double fnorm = nrm2(n, Fold);
                 if (fnorm > 0.0) {
@@ -908,8 +939,8 @@ void orc_broyden(orc_residual_fn fn, void *ctx, double *x, int64_t n, double alp
             }
         }
         nh = nstored < nhist ? nstored : nhist;
-        memcpy(xold, x, sizeof(double) * n);
-        memcpy(Fold, F, sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) { xold[i] = x[i]; Fold[i] = F[i]; }
         if (nh > 0) {
             for (int j = 0; j < nh; j++)
                 for (int i = 0; i < nh; i++) a[IDX(i, j, lda)] = gram[IDX(i, j, lda)];
@@ -980,6 +1011,7 @@ int orc_residual2(orc_state *s, double V, int64_t n, const double *x, double *F,
     orc_check_diag_sanity((double *)s->gRd, (double *)s->gLd, s->nA, s->nw, s->p.delta, &ra, &un, &iwst, &nws, &wws, NULL);
     *abort_flag = ra;
     if (ra) { memset(F, 0, sizeof(double) * n); return 0; }
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) F[i] -= x[i];
     return 0;
 }
@@ -1030,6 +1062,7 @@ static void final_sanity(orc_state *s) {   /* scf.f90:260-275 */
     int32_t ra, un, iwst; double nws, wws;
     orc_check_diag_sanity((double *)grd, (double *)gld, n, s->nw, s->p.delta, &ra, &un, &iwst, &nws, &wws, NULL);
     s->runaway_final = ra; s->unphys_final = un;
+    s->iworst_final = iwst; s->nworst_final = nws; s->wworst_final = wws;
     free(grd); free(gld);
 }
 
@@ -1101,6 +1134,10 @@ void orc_trUD(const orc_state *s, double *u, double *d) {
 }
 void orc_final_flags(const orc_state *s, int32_t *runaway, int32_t *unphysical) {
     *runaway = s->runaway_final; *unphysical = s->unphys_final;
+}
+/* worst state of final_sanity (1-based label), its occupation and spectral weight */
+void orc_final_worst(const orc_state *s, int32_t *i_worst, double *n_worst, double *w_worst) {
+    *i_worst = s->iworst_final; *n_worst = s->nworst_final; *w_worst = s->wworst_final;
 }
 void orc_get_gin(const orc_state *s, double *Gr, double *GL) {
     size_t m = (size_t)s->nA * s->nA * s->nw;

@@ -71,6 +71,7 @@ void orc_g_less_obs(const orc_state *s, double *G);                   /* (norb,n
 void orc_dosd(const orc_state *s, double *dosd);                      /* (norb,n_omega) */
 void orc_trUD(const orc_state *s, double *trU, double *trD);          /* (n_omega) each */
 void orc_final_flags(const orc_state *s, int32_t *runaway, int32_t *unphysical);
+void orc_final_worst(const orc_state *s, int32_t *i_worst, double *n_worst, double *w_worst);
 
 /* kernel-level seams */
 int orc_init2(orc_state *s, double V, double *x);                     /* sweep(M_INIT2) + pack_gf */

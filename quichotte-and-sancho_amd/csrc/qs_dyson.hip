@@ -502,13 +502,7 @@ __global__ __launch_bounds__(TI * 16) void k_dyson(DysonArgs A) {
 
 template <int TI, int RA, int RB>
 hipError_t launch_v(const DysonArgs &a, int count, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_dyson<TI, RA, RB>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    if (hipError_t e = qs_lds_attr((const void *)k_dyson<TI, RA, RB>)) return e;
     const size_t lds = qs_dyson_lds_bytes(a.norb, a.nCL);
     hipLaunchKernelGGL((k_dyson<TI, RA, RB>), dim3(count), dim3(TI * 16), lds, s, a);
     return hipGetLastError();
@@ -539,6 +533,21 @@ bool qs_dyson_shape_ok(int norb, int shape) {
 }
 
 int qs_dyson_variants(int norb) { (void)norb; return QS_DYSON_NSHAPES; }
+
+const char *qs_dyson_shape_desc(int shape, int *uses_mfma) {
+    static const char *const names[QS_DYSON_NSHAPES] = {
+        "256-thread workgroup, matrix in registers and LDS (FP64 vector FMA)",
+        "512-thread workgroup, matrix in registers and LDS (FP64 vector FMA)",
+        "16x16 workgroup, matrix in registers, scratch slab (FP64 vector FMA)",
+        "large-Norb chunks through HBM: batched zgetrf/zgetri + zgemm"};
+    static const int mfma[QS_DYSON_NSHAPES] = {0, 0, 0, 0};
+    if (shape < 0 || shape >= QS_DYSON_NSHAPES) {
+        if (uses_mfma) *uses_mfma = 0;
+        return "";
+    }
+    if (uses_mfma) *uses_mfma = mfma[shape];
+    return names[shape];
+}
 
 hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, hipStream_t s) {
     if (count <= 0) return hipSuccess;

@@ -593,13 +593,7 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
 template <int RA, int RB, int TI, int TJ>
 hipError_t launch_w(const DysonArgs &a0, int count, hipStream_t s) {
     const size_t lds = wlayout_bytes(a0.norb, a0.nCL, RA, RB, TI, TJ);
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_dyson_w<RA, RB, TI, TJ>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    if (hipError_t e = qs_lds_attr((const void *)k_dyson_w<RA, RB, TI, TJ>)) return e;
     // the scratch slab holds mscr_energies energies: launch in chunks of that many
     for (int c0 = 0; c0 < count; c0 += a0.mscr_energies) {
         DysonArgs a = a0;

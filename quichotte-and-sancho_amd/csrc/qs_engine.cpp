@@ -555,8 +555,10 @@ int tune_dyson(qs_handle *h) {
     }
     const int count = std::min(h->nw, 2048);
     const auto key = std::make_tuple(h->norb, h->nCL, count);
+    // one tuning at a time in the process: a second creator of the same size
+    // waits here and then takes the first one's record
+    std::lock_guard<std::mutex> lk(g_tune_mu);
     {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
         const auto it = g_tune.find(key);
         if (it != g_tune.end()) {
             h->dyson_variant = it->second.variant;
@@ -585,12 +587,9 @@ int tune_dyson(qs_handle *h) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        TuneRecord r{h->dyson_variant, {}};
-        for (int i = 0; i < 4; i++) r.ms[i] = h->dyson_tune_ms[i];
-        g_tune.emplace(key, r);  // the first measurement of a size stays
-    }
+    TuneRecord r{h->dyson_variant, {}};
+    for (int i = 0; i < 4; i++) r.ms[i] = h->dyson_tune_ms[i];
+    g_tune.emplace(key, r);
     return QS_OK;
 }
 
@@ -1009,6 +1008,32 @@ int final_sweep(qs_handle *h, double V) {
     return QS_OK;
 }
 
+// atoms and their intra-atom blocks of G^< (gather_atoms, src/green_functions.f90:240-257):
+// first orbital, orbital count and offset of each atom's (2m)^2 block; false if
+// the atoms' orbitals do not add up to the sites
+bool setup_atoms(qs_handle *h, const qs_problem *p) {
+    if (p->n_atoms > 0 && p->atom_norb) {
+        h->n_atoms = p->n_atoms;
+        h->atom_norb.assign(p->atom_norb, p->atom_norb + p->n_atoms);
+    } else {
+        h->n_atoms = h->nsites;
+        h->atom_norb.assign(h->nsites, 1);
+    }
+    int acc = 0, maxn = 0;
+    h->atom_first.resize(h->n_atoms);
+    for (int a = 0; a < h->n_atoms; a++) {
+        h->atom_first[a] = acc;
+        acc += h->atom_norb[a];
+        maxn = std::max(maxn, h->atom_norb[a]);
+    }
+    if (acc != h->nsites) return false;
+    h->atom_off.assign(h->n_atoms + 1, 0);
+    for (int a = 0; a < h->n_atoms; a++) h->atom_off[a + 1] = h->atom_off[a] + 4 * h->atom_norb[a] * h->atom_norb[a];
+    h->has_atomL = maxn > 1;
+    h->atom_tot = h->has_atomL ? h->atom_off[h->n_atoms] : 0;
+    return true;
+}
+
 int check_problem(const qs_problem *p, std::string &msg) {
     if (!p) { msg = "null problem"; return QS_ERR_ARG; }
     if (p->norb < 2 || (p->norb & 1)) { msg = "norb must be even and >= 2"; return QS_ERR_ARG; }
@@ -1049,21 +1074,7 @@ int qs_create(const qs_problem *p, qs_handle **out) {
     h->GamL.assign((const zd *)p->gammaL, (const zd *)p->gammaL + nn);
     h->GamR.assign((const zd *)p->gammaR, (const zd *)p->gammaR + nn);
     h->U.assign(p->U_site, p->U_site + h->norb);
-    if (p->n_atoms > 0 && p->atom_norb) {
-        h->n_atoms = p->n_atoms;
-        h->atom_norb.assign(p->atom_norb, p->atom_norb + p->n_atoms);
-    } else {
-        h->n_atoms = h->nsites;
-        h->atom_norb.assign(h->nsites, 1);
-    }
-    int acc = 0, maxn = 0;
-    h->atom_first.resize(h->n_atoms);
-    for (int a = 0; a < h->n_atoms; a++) { h->atom_first[a] = acc; acc += h->atom_norb[a]; maxn = std::max(maxn, h->atom_norb[a]); }
-    if (acc != h->nsites) { fprintf(stderr, "qs_create: atom_norb does not sum to norb/2\n"); delete h; return QS_ERR_ARG; }
-    h->atom_off.assign(h->n_atoms + 1, 0);
-    for (int a = 0; a < h->n_atoms; a++) h->atom_off[a + 1] = h->atom_off[a] + 4 * h->atom_norb[a] * h->atom_norb[a];
-    h->has_atomL = maxn > 1;
-    h->atom_tot = h->has_atomL ? h->atom_off[h->n_atoms] : 0;
+    if (!setup_atoms(h, p)) { fprintf(stderr, "qs_create: atom_norb does not sum to norb/2\n"); delete h; return QS_ERR_ARG; }
     int m = 1;
     while ((1 << m) < 2 * h->nw - 1) m++;
     h->Nf = 1 << m;
@@ -1357,12 +1368,15 @@ int64_t qs_footprint_bytes(const qs_problem *p) {
     tmp.GamL.assign((const zd *)p->gammaL, (const zd *)p->gammaL + nn);
     tmp.GamR.assign((const zd *)p->gammaR, (const zd *)p->gammaR + nn);
     tmp.U.assign(p->U_site, p->U_site + tmp.norb);
+    if (!setup_atoms(&tmp, p)) return -1;
     setup_storage(&tmp);
     const int64_t nw = tmp.nw, n = tmp.norb, nA = tmp.nA;
     int m = 1;
     while ((1 << m) < 2 * tmp.nw - 1) m++;
     const int64_t Nf = (int64_t)1 << m;
-    int64_t b = nw * (4 * (n / 2) * 16 + n * 8 + 16 + 2 * std::max<int64_t>(nA, 1) * 8 + 32);
+    // per energy: onsL, dosd, trUD, the two sanity diagonals, the Dyson partials, atom blocks
+    int64_t b = nw * (4 * (n / 2) * 16 + n * 8 + 16 + 2 * std::max<int64_t>(nA, 1) * 8 + 32 +
+                      (tmp.has_atomL ? (int64_t)tmp.atom_tot * 16 : 0));
     if (tmp.order == 2) {
         const int64_t nv = tmp.n_vec;
         b += 4 * nv * 8;
@@ -1383,6 +1397,17 @@ int qs_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes) 
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
     return QS_OK;
+}
+
+const char *qs_dyson_shape(const qs_handle *h, int32_t *uses_mfma) {
+    if (!h) {
+        if (uses_mfma) *uses_mfma = 0;
+        return "";
+    }
+    int m = 0;
+    const char *d = qs_dyson_shape_desc(h->dyson_variant, &m);
+    if (uses_mfma) *uses_mfma = m;
+    return d;
 }
 
 int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax) {

@@ -4,6 +4,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <set>
+
+// Raise a kernel's dynamic-LDS limit to the CU's 160 KiB, once per kernel and
+// process; safe to call from several host threads (handles of a serving loop
+// are created and launched concurrently).
+inline hipError_t qs_lds_attr(const void *kernel) {
+    static std::mutex mu;
+    static std::set<const void *> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.count(kernel)) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess) done.insert(kernel);
+    return e;
+}
+
 // modes of the per-energy solve, as G_at_energy's (src/scf.f90:49)
 enum { QS_M_INIT2 = 1, QS_M_SCF2 = 2, QS_M_SCF1 = 3, QS_M_FINAL = 4 };
 
@@ -60,6 +76,8 @@ QsBigWork *qs_big_create(int norb, int nw, size_t budget, size_t *bytes);
 void qs_big_destroy(QsBigWork *bw);
 hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s);
 bool qs_dyson_shape_ok(int norb, int shape);
+// what a launch shape is, and whether its products issue MFMA instructions
+const char *qs_dyson_shape_desc(int shape, int *uses_mfma);
 // one wavefront per energy (qs_dyson_wave.hip)
 hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s);
 size_t qs_dyson_wave_lds_bytes(int norb, int nCL);

@@ -695,13 +695,7 @@ hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const dou
     if (!qs_update_gt_fits(nA, ncol, n, nh) || (int64_t)3 * nA * nA * nw != n) return hipErrorInvalidValue;
     const int ncu = cu_count();
     if (ncu <= 0) return hipErrorInvalidValue;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_update_f32_gt, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    if (hipError_t e = qs_lds_attr((const void *)k_update_f32_gt)) return e;
     GamH gm;
     for (int k = 0; k < 16; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
     const int blocks = nw < ncu ? nw : ncu;
@@ -716,13 +710,7 @@ hipError_t qs_launch_permute_in(const double *x, double2 *Gt, const int *blk_src
     if (slice_fits_lds(nA, ncol)) {
         const int ncu = cu_count();
         if (ncu <= 0) return hipErrorInvalidValue;
-        static bool attr = false;
-        if (!attr) {
-            hipError_t e = hipFuncSetAttribute((const void *)k_permute_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               160 * 1024);
-            if (e != hipSuccess) return e;
-            attr = true;
-        }
+        if (hipError_t e = qs_lds_attr((const void *)k_permute_lds)) return e;
         const int blocks = nw < ncu ? nw : ncu;  // one per CU (the slice takes most of its LDS)
         hipLaunchKernelGGL(k_permute_lds, dim3(blocks), dim3(PL_NT), slice, s, x, Gt, (const int2 *)blk_src, nA, ncol,
                            nw);
@@ -737,11 +725,7 @@ namespace {
 
 template <int LG1>
 hipError_t launch_f1(const Sigma2Plan &p, const double2 *Gt, double2 *Z, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_fft_f1<LG1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+    if (hipError_t e = qs_lds_attr((const void *)k_fft_f1<LG1>)) return e;
     static_assert(lay_one_to_one<LG1, CB>(), "LDS layout");
     const size_t lds = Lay<LG1, CB>::bytes();
     hipLaunchKernelGGL(k_fft_f1<LG1>, dim3((p.ncol / CB + BPW - 1) / BPW, p.N2), dim3(NT), lds, s, Gt, Z, p);
@@ -749,11 +733,7 @@ hipError_t launch_f1(const Sigma2Plan &p, const double2 *Gt, double2 *Z, hipStre
 }
 template <int LG1>
 hipError_t launch_i2(const Sigma2Plan &p, const double2 *Z, double2 *Sig, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_fft_i2<LG1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+    if (hipError_t e = qs_lds_attr((const void *)k_fft_i2<LG1>)) return e;
     static_assert(lay_one_to_one<LG1, CB>(), "LDS layout");
     const size_t lds = Lay<LG1, CB>::bytes();
     hipLaunchKernelGGL(k_fft_i2<LG1>, dim3((p.ncol / CB + BPW - 1) / BPW, p.N2), dim3(NT), lds, s, Z, Sig, p);
@@ -761,11 +741,7 @@ hipError_t launch_i2(const Sigma2Plan &p, const double2 *Z, double2 *Sig, hipStr
 }
 template <int LG2>
 hipError_t launch_mid(const Sigma2Plan &p, double2 *Z, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_fft_mid<LG2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+    if (hipError_t e = qs_lds_attr((const void *)k_fft_mid<LG2>)) return e;
     static_assert(lay_one_to_one<LG2, PS>(), "LDS layout");
     const size_t lds = Lay<LG2, PS>::bytes();
     const int npairs = p.nAs * (p.nAs + 1) / 2;

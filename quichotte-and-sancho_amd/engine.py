@@ -83,6 +83,8 @@ def load_library(path=SO):
     L.qs_profile.argtypes = [P, I32]
     L.qs_profile_read.argtypes = [P, P, P, I32]
     L.qs_dyson_choice.argtypes = [P, P, P, I32]
+    L.qs_dyson_shape.argtypes = [P, P]
+    L.qs_dyson_shape.restype = ctypes.c_char_p
     L.qs_stage_name.argtypes = [I32]
     L.qs_stage_name.restype = ctypes.c_char_p
     L.qs_sancho_create.argtypes = [I32, I32, I32, I32, ctypes.POINTER(P)]
@@ -304,6 +306,14 @@ class ScfEngine:
         ms = np.zeros(4)
         n = self.L.qs_dyson_choice(self.h, ctypes.byref(v), _ptr(ms), 4)
         return v.value, ms[:n].tolist()
+
+    def dyson_shape_name(self):
+        return self.L.qs_dyson_shape(self.h, None).decode()
+
+    def dyson_uses_mfma(self):
+        m = ctypes.c_int32()
+        self.L.qs_dyson_shape(self.h, ctypes.byref(m))
+        return bool(m.value)
 
     def profile(self, enable=True):
         self._check(self.L.qs_profile(self.h, int(bool(enable))))

@@ -82,16 +82,19 @@ class Server:
                     item = self._admit_one()
             req, fp = item
             start = time.perf_counter() - t0
+            eng = None
             try:
                 eng = ScfEngine(req.problem)
                 r = eng.solve_scf(req.V, True)
                 ju, jd = eng.current()
-                eng.close()
                 resp = Response(req.rid, ju, jd, r.converged, r.n_iter, req.arrival, start,
                                 time.perf_counter() - t0, fp)
             except Exception as e:  # the request fails, the server goes on
                 resp = Response(req.rid, float("nan"), float("nan"), False, 0, req.arrival, start,
                                 time.perf_counter() - t0, fp, str(e))
+            finally:
+                if eng is not None:
+                    eng.close()   # its device memory is free before the budget says so
             with self.cv:
                 self.used -= fp
                 self.running -= 1

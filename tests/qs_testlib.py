@@ -243,6 +243,7 @@ def oracle_lib():
         L.orc_dosd.argtypes = [P, P]
         L.orc_trUD.argtypes = [P, P, P]
         L.orc_final_flags.argtypes = [P, P, P]
+        L.orc_final_worst.argtypes = [P, P, P, P]
         L.orc_init2.argtypes = [P, D, P]
         L.orc_residual2.argtypes = [P, D, I64, P, P, P]
         L.orc_final_sweep.argtypes = [P, D]
@@ -331,6 +332,12 @@ class Oracle:
         G = np.zeros((self.prob.norb, self.prob.norb), complex, order="F")
         self.L.orc_g_less_obs(self.h, ptr(G))
         return G
+
+    def final_worst(self):
+        """final_sanity's worst state: (1-based label, occupation, spectral weight)"""
+        i, n, w = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        self.L.orc_final_worst(self.h, ctypes.byref(i), ctypes.byref(n), ctypes.byref(w))
+        return i.value, n.value, w.value
 
     def dosd(self):
         d = np.zeros((self.prob.norb, self.prob.n_omega), order="F")

@@ -75,6 +75,31 @@ def test_footprint_counts_the_update_difference_buffer():
     assert footprint(0, False) == base
 
 
+def test_footprint_counts_the_atom_blocks():
+    """Multi-orbital atoms (dftb) keep their intra-atom blocks of G^< per energy
+    (gather_atoms, src/green_functions.f90:240-257): 16 bytes for each of the
+    sum over atoms of (2 m)^2 entries, per energy, on top of one-orbital atoms."""
+    import importlib
+
+    import numpy as np
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    prob = random_problem(16, 64, seed=4)
+
+    def footprint(atom_norb):
+        kw = dict(H=prob.H, gammaL=prob.GL, gammaR=prob.GR, U_site=prob.U_site, order=2, n_omega=prob.n_omega,
+                  w_init=prob.w_init, delta=prob.delta, beta=prob.beta, mu=prob.mu, nhist=4,
+                  atom_norb=atom_norb)
+        return pkg.footprint_bytes(pkg.ScfProblem(**kw))
+
+    base = footprint(None)
+    atoms = np.array([4, 4], dtype=np.int32)          # 8 sites as two 4-orbital atoms
+    assert footprint(atoms) - base == 16 * prob.n_omega * 2 * (2 * 4) ** 2
+    assert footprint(np.array([1] * 8, dtype=np.int32)) == base
+    import pytest
+    with pytest.raises(pkg.QsError):
+        footprint(np.array([4, 3], dtype=np.int32))   # does not add up to the 8 sites
+
+
 def test_sancho_create_rejects_bad_sizes_before_any_device_call(lib):
     """argument checks come first: no HIP call is made for these"""
     P = ctypes.c_void_p

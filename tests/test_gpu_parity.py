@@ -240,3 +240,20 @@ def test_solve_stopped_at_maxscf_matches_oracle(single):
     Go, Gg = o.g_less_obs(), g.g_less_obs()
     assert np.abs(Gg - Go).max() <= 1e-10 * np.abs(Go).max()
     assert rel_max(g.dosd(), o.dosd()) < 1e-10
+
+
+@pytest.mark.parametrize("atoms", [None, [4, 4], [2, 3, 3]])
+def test_footprint_covers_what_create_allocates(atoms):
+    """qs_footprint_bytes (serving admission) is at least what qs_create holds,
+    with multi-orbital atoms (their intra-atom G^< blocks) and without."""
+    prob = random_problem(16, 96, seed=8)
+    an = None if atoms is None else np.array(atoms, dtype=np.int32)
+    g = gpu_engine(prob, atom_norb=an)
+    import importlib
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    kw = dict(H=prob.H, gammaL=prob.GL, gammaR=prob.GR, U_site=prob.U_site, order=2, n_omega=prob.n_omega,
+              w_init=prob.w_init, delta=prob.delta, beta=prob.beta, mu=prob.mu, nhist=prob.nhist, atom_norb=an)
+    fp = pkg.footprint_bytes(pkg.ScfProblem(**kw))
+    assert fp >= g.device_bytes(), (fp, g.device_bytes())
+    assert fp - g.device_bytes() <= (64 << 20) + (1 << 20)
+    g.close()
