@@ -16,10 +16,9 @@ OBJ = os.path.join(LIB, "_obj")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_dyson_wave.hip", "qs_dyson_big.hip", "qs_sigma2.hip", "qs_vector.hip",
-           "qs_sancho.hip"]
-# the large-Norb Dyson path calls rocSOLVER (batched zgetrf / zgetri) and rocBLAS (batched zgemm)
-LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_dyson_wave.hip", "qs_dyson_big.hip", "qs_gj_mfma.hip", "qs_sigma2.hip",
+           "qs_vector.hip", "qs_sancho.hip"]
+LIBS = []
 
 
 def _compile(src, obj, verbose):

@@ -539,8 +539,8 @@ const char *qs_dyson_shape_desc(int shape, int *uses_mfma) {
         "256-thread workgroup, matrix in registers and LDS (FP64 vector FMA)",
         "512-thread workgroup, matrix in registers and LDS (FP64 vector FMA)",
         "16x16 workgroup, matrix in registers, scratch slab (FP64 vector FMA)",
-        "large-Norb chunks through HBM: batched zgetrf/zgetri + zgemm"};
-    static const int mfma[QS_DYSON_NSHAPES] = {0, 0, 0, 0};
+        "large-Norb chunks through HBM: blocked Gauss-Jordan and products on FP64 MFMA"};
+    static const int mfma[QS_DYSON_NSHAPES] = {0, 0, 0, 1};
     if (shape < 0 || shape >= QS_DYSON_NSHAPES) {
         if (uses_mfma) *uses_mfma = 0;
         return "";

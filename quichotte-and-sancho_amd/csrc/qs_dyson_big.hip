@@ -4,24 +4,23 @@
 // Same contract as k_dyson / k_dyson_w (src/scf.f90:305-415):
 //   G^r = [ w + Hemb - Sigma1 - Sigma2^r(w) ]^-1
 //   G^< = G^r(:,sup) [ i(f_L GamL + f_R GamR) + Sigma2^<(w) ] G^r(:,sup)^dagger
-// with every epilogue mode.  The energies go in chunks through HBM:
-//   k_big_build   A (column-major) and S'' = SL on the support, zero elsewhere
-//   zgetrf + zgetri  rocSOLVER, batched: the LAPACK pair the reference calls
-//                 (common/linear_algebra.f90:58-88), partial pivoting on |Re| + |Im|
-//   zgemm x 2     rocBLAS, batched: T = G^r S'', G^< = T G^r^dagger
+// with every epilogue mode.  The energies go in chunks through HBM, all in
+// hand-written kernels:
+//   k_big_build   A and S'' = SL on the support (zero elsewhere), row-major,
+//                 padded to np = a multiple of 16 with an identity block
+//   k_gj_inv      blocked Gauss-Jordan inverse with the zgetf2 pivot rule and
+//                 MFMA trailing updates (qs_gj_mfma.hip), in place of
+//                 zgetrf + zgetri (common/linear_algebra.f90:58-88)
+//   k_cgemm x 2   T = G^r S'', G^< = T G^r^dagger on MFMA (src/scf.f90:343-344)
 //   k_big_pack    the packing epilogues of the mode (as the other two kernels)
-// All matrices of a chunk are column-major n x n with leading dimension n.
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
+// The padding block is the identity in A and zero in S'', so it stays out of
+// G^r's top-left n x n block and of G^<.
 #include "qs_device.h"
 #include "qs_internal.h"
 
 struct QsBigWork {
-    rocblas_handle rb = nullptr;
-    double2 *A = nullptr, *S = nullptr, *T = nullptr;  // [chunk][n][n] each
-    rocblas_int *ipiv = nullptr, *info = nullptr;      // [chunk][n], [chunk]
-    int n = 0, chunk = 0;
+    double2 *A = nullptr, *S = nullptr, *T = nullptr;  // [chunk][np][np] each
+    int n = 0, np = 0, chunk = 0;
 };
 
 namespace {
@@ -30,17 +29,23 @@ constexpr int NT = 256;
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 // A(i, j) = w + Hemb - Sigma1 - Sigma2^r and S(i, j) = i (f_L GamL + f_R GamR) + Sigma2^<
-// on the support (zero elsewhere), for energy iwc + blockIdx.x
-__global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, double2 *__restrict__ Ab,
+// on the support (zero elsewhere), for energy iwc + blockIdx.x; row-major np x np,
+// identity (A) and zero (S) past n
+__global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, int np, double2 *__restrict__ Ab,
                                                   double2 *__restrict__ Sb) {
     const int n = A.norb, iw = iwc + blockIdx.x;
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     const bool use_s1 = A.use_sigma && A.sig1;
-    double2 *a = Ab + (size_t)blockIdx.x * n * n, *s = Sb + (size_t)blockIdx.x * n * n;
-    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
-        const int j = idx / n, i = idx - j * n;  // column-major (i, j)
+    double2 *a = Ab + (size_t)blockIdx.x * np * np, *s = Sb + (size_t)blockIdx.x * np * np;
+    for (int idx = threadIdx.x; idx < np * np; idx += NT) {
+        const int i = idx / np, j = idx - i * np;  // row-major (i, j)
+        if (i >= n || j >= n) {
+            a[idx] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+            s[idx] = make_double2(0.0, 0.0);
+            continue;
+        }
         double2 v = A.Hemb[(size_t)i * n + j];
         if (i == j) v.x += w;
         if (use_s1 && (i >> 1) == (j >> 1)) v = csub(v, A.sig1[(i >> 1) * 4 + (i & 1) * 2 + (j & 1)]);
@@ -83,15 +88,14 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
     return t;
 }
 
-// the epilogue of the mode for energy iwc + blockIdx.x from G^r and G^< (column-major)
-__global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, const double2 *__restrict__ Gr,
-                                                 const double2 *__restrict__ Gl, const rocblas_int *__restrict__ info) {
+// the epilogue of the mode for energy iwc + blockIdx.x from G^r and G^< (row-major np x np)
+__global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, int np, const double2 *__restrict__ Gr,
+                                                 const double2 *__restrict__ Gl) {
     __shared__ double red[NT / 64];
     const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL, iw = iwc + blockIdx.x, tid = threadIdx.x;
-    const double2 *gr = Gr + (size_t)blockIdx.x * n * n, *gl = Gl + (size_t)blockIdx.x * n * n;
-    auto GR = [&](int i, int j) { return gr[(size_t)j * n + i]; };
-    auto GL = [&](int i, int j) { return gl[(size_t)j * n + i]; };
-    if (tid == 0 && info[blockIdx.x] != 0) atomicOr(A.status, 1);
+    const double2 *gr = Gr + (size_t)blockIdx.x * np * np, *gl = Gl + (size_t)blockIdx.x * np * np;
+    auto GR = [&](int i, int j) { return gr[(size_t)i * np + j]; };
+    auto GL = [&](int i, int j) { return gl[(size_t)i * np + j]; };
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta);
     if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
@@ -189,40 +193,44 @@ __global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, const dou
     }
 }
 
-hipError_t rb_err(rocblas_status st) { return st == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure; }
-
 }  // namespace
 
-int qs_dyson_big_max_norb() { return 1024; }
+int qs_dyson_big_max_norb() { return qs_gj_max_np(); }
 
-size_t qs_dyson_big_bytes(int n, int nw, size_t budget) {
-    const size_t per = (size_t)3 * n * n * sizeof(double2) + (size_t)(n + 1) * sizeof(rocblas_int);
-    size_t chunk = budget / per;
+static size_t big_per_energy(int n) {
+    const size_t np = (size_t)((n + 15) / 16 * 16);
+    return 3 * np * np * sizeof(double2);
+}
+
+// energies per chunk: what the budget holds, rounded down to whole rounds of the
+// inverse kernel (one matrix per CU at a time), at most the grid
+static size_t big_chunk(int n, int nw, size_t budget) {
+    size_t chunk = budget / big_per_energy(n);
+    const size_t ncu = 256;   // MI355X compute units
+    if (chunk >= ncu) chunk = chunk / ncu * ncu;
     if (chunk < 1) chunk = 1;
     if (chunk > (size_t)nw) chunk = nw;
-    return chunk * per;
+    return chunk;
 }
+
+size_t qs_dyson_big_bytes(int n, int nw, size_t budget) { return big_chunk(n, nw, budget) * big_per_energy(n); }
 
 QsBigWork *qs_big_create(int n, int nw, size_t budget, size_t *bytes) {
     QsBigWork *bw = new QsBigWork();
     bw->n = n;
-    const size_t per = (size_t)3 * n * n * sizeof(double2) + (size_t)(n + 1) * sizeof(rocblas_int);
-    size_t chunk = budget / per;
-    if (chunk < 1) chunk = 1;
-    if (chunk > (size_t)nw) chunk = nw;
+    bw->np = (n + 15) / 16 * 16;
+    const size_t chunk = big_chunk(n, nw, budget);
     bw->chunk = (int)chunk;
-    const size_t mat = chunk * n * n * sizeof(double2);
-    bool ok = rocblas_create_handle(&bw->rb) == rocblas_status_success;
-    ok = ok && hipMalloc((void **)&bw->A, mat) == hipSuccess;
+    const size_t mat = chunk * (size_t)bw->np * bw->np * sizeof(double2);
+    bool ok = hipMalloc((void **)&bw->A, mat) == hipSuccess;
     ok = ok && hipMalloc((void **)&bw->S, mat) == hipSuccess;
     ok = ok && hipMalloc((void **)&bw->T, mat) == hipSuccess;
-    ok = ok && hipMalloc((void **)&bw->ipiv, chunk * n * sizeof(rocblas_int)) == hipSuccess;
-    ok = ok && hipMalloc((void **)&bw->info, chunk * sizeof(rocblas_int)) == hipSuccess;
     if (!ok) {
+        (void)hipGetLastError();
         qs_big_destroy(bw);
         return nullptr;
     }
-    if (bytes) *bytes = 3 * mat + chunk * (n + 1) * sizeof(rocblas_int);
+    if (bytes) *bytes = 3 * mat;
     return bw;
 }
 
@@ -231,9 +239,6 @@ void qs_big_destroy(QsBigWork *bw) {
     (void)hipFree(bw->A);
     (void)hipFree(bw->S);
     (void)hipFree(bw->T);
-    (void)hipFree(bw->ipiv);
-    (void)hipFree(bw->info);
-    if (bw->rb) (void)rocblas_destroy_handle(bw->rb);
     delete bw;
 }
 
@@ -241,28 +246,18 @@ hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s) {
     QsBigWork *bw = (QsBigWork *)a.big;
     if (count <= 0) return hipSuccess;
     if (!bw || bw->n != a.norb || a.norb > qs_dyson_big_max_norb()) return hipErrorInvalidValue;
-    const int n = a.norb;
-    const rocblas_stride sn = (rocblas_stride)n * n;
-    hipError_t e = rb_err(rocblas_set_stream(bw->rb, s));
-    if (e != hipSuccess) return e;
-    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-    auto *A = (rocblas_double_complex *)bw->A, *S = (rocblas_double_complex *)bw->S, *T = (rocblas_double_complex *)bw->T;
+    const int np = bw->np;
+    hipError_t e;
     for (int c0 = 0; c0 < count; c0 += bw->chunk) {
         const int cnt = count - c0 < bw->chunk ? count - c0 : bw->chunk, iwc = a.iw0 + c0;
-        hipLaunchKernelGGL(k_big_build, dim3(cnt), dim3(NT), 0, s, a, iwc, bw->A, bw->S);
+        hipLaunchKernelGGL(k_big_build, dim3(cnt), dim3(NT), 0, s, a, iwc, np, bw->A, bw->S);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = rb_err(rocsolver_zgetrf_strided_batched(bw->rb, n, n, A, n, sn, bw->ipiv, n, bw->info, cnt))) != hipSuccess)
-            return e;
-        if ((e = rb_err(rocsolver_zgetri_strided_batched(bw->rb, n, A, n, sn, bw->ipiv, n, bw->info, cnt))) != hipSuccess)
-            return e;
-        // T = G^r S'';  G^< = T G^r^dagger (into S)
-        if ((e = rb_err(rocblas_zgemm_strided_batched(bw->rb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, A,
-                                                      n, sn, S, n, sn, &zero, T, n, sn, cnt))) != hipSuccess)
-            return e;
-        if ((e = rb_err(rocblas_zgemm_strided_batched(bw->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose,
-                                                      n, n, n, &one, T, n, sn, A, n, sn, &zero, S, n, sn, cnt))) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(k_big_pack, dim3(cnt), dim3(NT), 0, s, a, iwc, bw->A, bw->S, bw->info);
+        // G^r into T (A is left as scratch)
+        if ((e = qs_launch_gj_inv(np, cnt, bw->A, bw->T, a.status, false, s)) != hipSuccess) return e;
+        // T1 = G^r S'' into A;  G^< = T1 G^r^dagger into S
+        if ((e = qs_launch_cgemm(np, cnt, bw->T, bw->S, bw->A, false, false, s)) != hipSuccess) return e;
+        if ((e = qs_launch_cgemm(np, cnt, bw->A, bw->T, bw->S, true, false, s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_big_pack, dim3(cnt), dim3(NT), 0, s, a, iwc, np, bw->T, bw->S);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

@@ -78,6 +78,15 @@ hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s);
 bool qs_dyson_shape_ok(int norb, int shape);
 // what a launch shape is, and whether its products issue MFMA instructions
 const char *qs_dyson_shape_desc(int shape, int *uses_mfma);
+// batched complex inverse and products on MFMA (qs_gj_mfma.hip); row-major np x np
+// matrices, np a multiple of 16 and at most qs_gj_max_np(); fp32 selects complex FP32
+int qs_gj_max_np();
+size_t qs_gj_lds_bytes(int np, bool fp32);
+// G[b] = M[b]^-1 for count matrices (M is overwritten); bit 0 of *status on an exact zero pivot
+hipError_t qs_launch_gj_inv(int np, int count, void *M, void *G, int *status, bool fp32, hipStream_t s);
+// C[b] = A[b] op(B[b]), op = identity or conjugate transpose
+hipError_t qs_launch_cgemm(int np, int count, const void *A, const void *B, void *C, bool conjB, bool fp32,
+                           hipStream_t s);
 // one wavefront per energy (qs_dyson_wave.hip)
 hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s);
 size_t qs_dyson_wave_lds_bytes(int norb, int nCL);

@@ -45,7 +45,8 @@ def test_init2_matches_oracle(norb, nw):
     assert rel_max(xg, xo) < 1e-11
 
 
-@pytest.mark.parametrize("norb,nw", [(4, 33), (8, 200), (16, 257), (80, 48), (100, 24), (128, 12)])
+@pytest.mark.parametrize("norb,nw", [(4, 33), (8, 200), (16, 257), (80, 48), (100, 24), (128, 12), (160, 8),
+                                     (240, 6), (256, 4)])
 def test_residual2_and_sigma_match_oracle(norb, nw):
     prob = random_problem(norb, nw, seed=10 + norb, U=1.3)
     o = Oracle(prob)
@@ -160,8 +161,13 @@ def test_gpu_reproduces_reference_fixture(case, shape, monkeypatch):
         assert_close_table(pops, read_table(pf))
 
 
-def test_runaway_and_errors_are_reported():
+@pytest.mark.parametrize("shape", ["tuned", "large-Norb"])
+def test_runaway_and_errors_are_reported(shape, monkeypatch):
+    """A NaN in the input reaches the inverse as NaN pivots (not exact zeros): the
+    sweep completes and the sanity check reports a runaway, in every shape."""
     import importlib
+    if shape == "large-Norb":
+        monkeypatch.setenv("QS_DYSON_VARIANT", "3")
     pkg = importlib.import_module("quichotte-and-sancho_amd")
     prob = random_problem(8, 64, seed=3)
     g = gpu_engine(prob)
@@ -256,4 +262,24 @@ def test_footprint_covers_what_create_allocates(atoms):
     fp = pkg.footprint_bytes(pkg.ScfProblem(**kw))
     assert fp >= g.device_bytes(), (fp, g.device_bytes())
     assert fp - g.device_bytes() <= (64 << 20) + (1 << 20)
+    g.close()
+
+
+@pytest.mark.parametrize("norb", [98, 112, 144, 200])
+def test_large_norb_inverse_and_products_match_oracle(norb):
+    """The hand-written large-Norb path (blocked Gauss-Jordan with MFMA updates,
+    MFMA products) at sizes that are not multiples of its 32-column panel, on a
+    random system whose pivots move rows: init2 and one residual at 1e-11."""
+    prob = random_problem(norb, 5, seed=300 + norb, U=0.7)
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    assert g.dyson_choice()[0] == 3
+    x = o.init2(-0.2)
+    xg = g.init2(-0.2)
+    assert rel_max(xg, x) < 1e-11, worst_packed(xg, x, o.nA)
+    F0, _ = o.residual2(-0.2, x)
+    x1 = x + 0.5 * F0
+    Fo, _ = o.residual2(-0.2, x1)
+    Fg, _ = g.residual2(-0.2, x1)
+    assert rel_max(Fg, Fo) < 1e-11, worst_packed(Fg, Fo, o.nA)
     g.close()
