@@ -163,6 +163,25 @@ int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, co
 int qs_sancho_profile(qs_sancho *s, int32_t enable);
 int qs_sancho_profile_read(const qs_sancho *s, double *ms, int64_t *count);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): qs_sancho_self_energies on device arrays with no handle and
+ * no allocation.  The work arrays (completed G copies, bubbles, split partial
+ * sums, U) live in a fixed buffer inside the library, qs_sancho_static_bytes()
+ * = 64 MiB, one copy per device, held until the process ends.  A size whose work
+ * arrays need more (qs_sancho_static_need) returns QS_ERR_UNSUPPORTED before any
+ * GPU work.  Runs on the calling thread's current device, where GR, GL and the
+ * Sig arrays must live, on `stream` (a hipStream_t; NULL for the default stream),
+ * and returns after that stream has finished.  Every call on a device shares
+ * that device's buffer: calls that overlap on one device race.  Results are
+ * those of qs_sancho_self_energies with on_device = 1. */
+int64_t qs_sancho_static_bytes(void);
+/* bytes of that buffer a call of these sizes uses on the current device; -1 if invalid */
+int64_t qs_sancho_static_need(int32_t na, int32_t n_omega, int32_t order);
+int qs_sancho_self_energies_static(int32_t na, int32_t n_omega, int32_t order, double delta, const double *U_site,
+                                   const double *GR, const double *GL, double *Sig2R, double *Sig2L, double *Sig2G,
+                                   double *Sig3R, double *Sig3L, double *Sig3G, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,4 +6,4 @@ include/qs_hip.h; this module binds it with ctypes.  There is no CPU fallback:
 if the library is missing or no GPU is visible, the calls raise.
 """
 from .engine import (QS_ERRORS, DeviceBuffer, QsError, SanchoSelfEnergies, ScfEngine, ScfProblem, ScfResult,  # noqa: F401
-                     device_memory, footprint_bytes, load_library)
+                     device_memory, footprint_bytes, load_library, sancho_self_energies_static)

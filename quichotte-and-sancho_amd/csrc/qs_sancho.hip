@@ -31,6 +31,7 @@
 #include <string>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "qs_device.h"
 #include "qs_hip.h"
@@ -228,6 +229,15 @@ __global__ __launch_bounds__(256) void k_sn_sp(int na, int L, const double *__re
     O[2 * nB + o] = sr;
 }
 
+// U_site into device memory from a kernel argument (no host-memory copy)
+struct SnU {
+    double u[256];
+};
+__global__ void k_sn_set_u(int na, SnU U, double *__restrict__ dst) {
+    const int i = threadIdx.x;
+    if (i < na) dst[i] = U.u[i];
+}
+
 }  // namespace
 
 struct qs_sancho {
@@ -389,7 +399,12 @@ int sn_self_energies(qs_sancho *s, double delta, const double *U_site, const dou
     const size_t nG = (size_t)na * na * nw, nB = (size_t)na * na * L;
     SNCHK(s, hipSetDevice(s->device));
     hipStream_t st = s->stream;
-    SNCHK(s, hipMemcpyAsync(s->U, U_site, na * sizeof(double), hipMemcpyHostToDevice, st));
+    {
+        SnU u{};
+        for (int i = 0; i < na; i++) u.u[i] = U_site[i];
+        hipLaunchKernelGGL(k_sn_set_u, dim3(1), dim3(256), 0, st, na, u, s->U);
+        SNCHK(s, hipGetLastError());
+    }
     const double2 *gr = (const double2 *)GR, *gl = (const double2 *)GL;
     if (!on_device) {
         SNCHK(s, hipMemcpyAsync(s->gin, GR, nG * sizeof(double2), hipMemcpyHostToDevice, st));
@@ -459,7 +474,94 @@ int sn_self_energies(qs_sancho *s, double delta, const double *U_site, const dou
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Entry point that allocates nothing: the work arrays live in a fixed buffer of
+// the library, one copy per device (held until the process ends).
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticBytes = (size_t)64 << 20;
+static __device__ __align__(256) double2 scratch[kStaticBytes / sizeof(double2)];
+
+namespace {
+std::mutex g_scratch_mu;
+double2 *g_scratch_ptr[64] = {nullptr};
+
+// the device address of this device's copy of scratch (the array's name is a host
+// stand-in: only this address may be passed to kernels or copies)
+int scratch_address(int dev, double2 **out) {
+    if (dev < 0 || dev >= 64) return QS_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return QS_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = (double2 *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return QS_OK;
+}
+// the work arrays of one call, in double2 slots: g, bubbles, split partials, U
+struct StaticLayout {
+    size_t nG, nB, part_elems, total;
+};
+StaticLayout static_layout(int na, int n_omega, int order, int cus) {
+    StaticLayout l;
+    const size_t L = 2 * (size_t)n_omega - 1;
+    l.nG = (size_t)na * na * n_omega;
+    l.nB = (size_t)na * na * L;
+    const int nz = order == 3 ? 2 : 1;
+    const size_t e4 = sn_part_elems(4, (int)L, na, nz, (n_omega + 255) / 256, cus);
+    const size_t e3 = sn_part_elems(3, n_omega, na, nz, ((int)L + 255) / 256, cus);
+    l.part_elems = e4 > e3 ? e4 : e3;
+    l.total = 4 * l.nG + sn_nbub(order) * l.nB + l.part_elems + (na + 1) / 2;
+    return l;
+}
+bool static_args_ok(int na, int n_omega, int order) {
+    return na >= 1 && na <= 255 && n_omega >= 1 && n_omega <= (1 << 24) && (order == 2 || order == 3);
+}
+int current_cus(int *dev, int *cus) {
+    if (hipGetDevice(dev) != hipSuccess) return QS_ERR_HIP;
+    if (hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, *dev) != hipSuccess) return QS_ERR_HIP;
+    return QS_OK;
+}
+}  // namespace
+
 extern "C" {
+
+int64_t qs_sancho_static_bytes(void) { return (int64_t)kStaticBytes; }
+
+int64_t qs_sancho_static_need(int32_t na, int32_t n_omega, int32_t order) {
+    if (!static_args_ok(na, n_omega, order)) return -1;
+    int dev = 0, cus = 256;
+    if (current_cus(&dev, &cus) != QS_OK) return -1;
+    return (int64_t)(sizeof(double2) * static_layout(na, n_omega, order, cus).total);
+}
+
+int qs_sancho_self_energies_static(int32_t na, int32_t n_omega, int32_t order, double delta, const double *U_site,
+                                   const double *GR, const double *GL, double *Sig2R, double *Sig2L, double *Sig2G,
+                                   double *Sig3R, double *Sig3L, double *Sig3G, void *stream) {
+    if (!static_args_ok(na, n_omega, order)) return QS_ERR_ARG;
+    int dev = 0, cus = 256;
+    if (int rc = current_cus(&dev, &cus)) return rc;
+    const StaticLayout l = static_layout(na, n_omega, order, cus);
+    if (l.total * sizeof(double2) > kStaticBytes) return QS_ERR_UNSUPPORTED;
+    double2 *base = nullptr;
+    if (int rc = scratch_address(dev, &base)) return rc;
+    // a descriptor of the work arrays inside scratch, laid out as qs_sancho_create's
+    qs_sancho w;
+    w.na = na, w.nw = n_omega, w.order = order, w.device = dev, w.cus = cus;
+    w.stream = (hipStream_t)stream;
+    w.part_elems = l.part_elems;
+    w.g = base;
+    w.bub = w.g + 4 * l.nG;
+    w.part = l.part_elems ? w.bub + sn_nbub(order) * l.nB : nullptr;
+    w.U = (double *)(w.bub + sn_nbub(order) * l.nB + l.part_elems);
+    const int rc = sn_self_energies(&w, delta, U_site, GR, GL, Sig2R, Sig2L, Sig2G, Sig3R, Sig3L, Sig3G, 1);
+    if (rc != QS_OK) (void)hipStreamSynchronize(w.stream);
+    w.stream = nullptr;   // the caller's stream: not destroyed here
+    return rc;
+}
 
 int qs_sancho_self_energies(qs_sancho *s, double delta, const double *U_site, const double *GR, const double *GL,
                             double *Sig2R, double *Sig2L, double *Sig2G, double *Sig3R, double *Sig3L,

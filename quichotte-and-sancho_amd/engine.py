@@ -94,6 +94,11 @@ def load_library(path=SO):
     L.qs_sancho_device_bytes.argtypes = [P]
     L.qs_sancho_device_bytes.restype = I64
     L.qs_sancho_self_energies.argtypes = [P, D, P, P, P, P, P, P, P, P, P, I32]
+    L.qs_sancho_static_bytes.argtypes = []
+    L.qs_sancho_static_bytes.restype = I64
+    L.qs_sancho_static_need.argtypes = [I32, I32, I32]
+    L.qs_sancho_static_need.restype = I64
+    L.qs_sancho_self_energies_static.argtypes = [I32, I32, I32, D, P, P, P, P, P, P, P, P, P, P]
     L.qs_sancho_profile.argtypes = [P, I32]
     L.qs_sancho_profile_read.argtypes = [P, P, P]
     _lib = L
@@ -460,3 +465,22 @@ class SanchoSelfEnergies:
         if len(outs) != self.n_outputs:
             raise ValueError("order %d needs %d outputs" % (self.order, self.n_outputs))
         self._call(delta, U_site, addr(gr), addr(gl), [addr(o) for o in outs], 1)
+
+
+def sancho_self_energies_static(na, n_omega, order, gr, gl, outs, U_site, delta, stream=None):
+    """qs_sancho_self_energies_static: SANCHO's self-energies on device arrays with
+    no handle and no device allocation (work arrays in the library's fixed buffer).
+    gr, gl, outs: DeviceBuffer objects or device addresses of na^2 n_omega complex128."""
+    L = load_library()
+    addr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else ctypes.c_void_p(int(b))
+    n_out = 6 if order == 3 else 3
+    if len(outs) != n_out:
+        raise ValueError("order %d needs %d outputs" % (order, n_out))
+    U = np.ascontiguousarray(U_site, dtype=np.float64)
+    if U.shape != (na,):
+        raise ValueError("U_site must have na = %d entries" % na)
+    o = [addr(b) for b in outs] + [None] * (6 - n_out)
+    rc = L.qs_sancho_self_energies_static(int(na), int(n_omega), int(order), float(delta), _ptr(U), addr(gr), addr(gl),
+                                          *o, stream)
+    if rc != 0:
+        raise QsError(rc, "qs_sancho_self_energies_static(na=%d, n_omega=%d, order=%d)" % (na, n_omega, order))
