@@ -33,6 +33,7 @@ sys.path.insert(0, REPO)
 KB = 8.6173303e-5
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def helix_H(N_ions, N_turns, R=1.4, L=3.4, hand=1.0, E=0.0, t=-1.0, lamb=0.01):
@@ -436,6 +437,10 @@ def make_parser():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: the host's physical cores)")
     ap.add_argument("--cpu-iters", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: time the mixed-precision form after the plain FP64 path on the same points")
+    ap.add_argument("--mixed-switch", type=float, default=1e-5,
+                    help="max|F| below which the mixed form leaves FP32 sweeps")
     ap.add_argument("--serve", default="", help="comma-separated Poisson arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -484,27 +489,38 @@ def main():
         print("bench: warmup (fresh) V=%+.2f converged=%s steps=%d %.2f s" % (volts[k % nv], r.converged, r.n_iter,
                                                                               time.time() - t1),
               file=sys.stderr, flush=True)
-    eng.profile(True)
-    lat, iters, conv = [], [], []
-    fresh = True
-    tstart = time.perf_counter()
-    for k, i in enumerate(sweep_points(nv, args.steps)):
-        if i == 0:
-            fresh = True
-        t1 = time.perf_counter()
-        r = eng.solve_scf(volts[i], fresh)
-        J = eng.current()
-        lat.append(time.perf_counter() - t1)
-        fresh = not r.converged
-        iters.append(r.n_iter)
-        conv.append(r.converged)
-        print("bench: V=%+.2f %s converged=%s steps=%d J=(%.6e, %.6e) %.3f s" %
-              (volts[i], "fresh" if (i == 0 or (k and not conv[-2])) else "cont.", r.converged, r.n_iter, J[0], J[1],
-               lat[-1]), file=sys.stderr, flush=True)
-    total = time.perf_counter() - tstart
-    prof = eng.profile_read()
-    eng.profile(False)
+    def timed_sweep(tag):
+        eng.profile(True)
+        lat, iters, conv, Js, n32 = [], [], [], [], []
+        fresh = True
+        tstart = time.perf_counter()
+        for k, i in enumerate(sweep_points(nv, args.steps)):
+            if i == 0:
+                fresh = True
+            t1 = time.perf_counter()
+            r = eng.solve_scf(volts[i], fresh)
+            J = eng.current()
+            lat.append(time.perf_counter() - t1)
+            fresh = not r.converged
+            iters.append(r.n_iter)
+            conv.append(r.converged)
+            Js.append(J)
+            n32.append(eng.precision_steps()[0])
+            print("bench: %s V=%+.2f %s converged=%s steps=%d (FP32 %d) J=(%.9e, %.9e) %.3f s" %
+                  (tag, volts[i], "fresh" if (i == 0 or (k and not conv[-2])) else "cont.", r.converged, r.n_iter,
+                   n32[-1], J[0], J[1], lat[-1]), file=sys.stderr, flush=True)
+        total = time.perf_counter() - tstart
+        prof = eng.profile_read()
+        eng.profile(False)
+        return total, lat, iters, conv, np.array(Js), n32, prof
+
+    plain = None
+    if args.precision == "mixed":
+        plain = timed_sweep("plain")
+        eng.set_precision(1, args.mixed_switch)
+    total, lat, iters, conv, Js, n32, prof = timed_sweep(args.precision)
     counts = stage_counts(eng, prob["nhist"])
+    counts["dyson_keldysh_fp32"] = counts["dyson_keldysh"]
     dom = max((s for s in prof if s in counts and prof[s][1] > 0), key=lambda s: prof[s][0])
     bound, work, kind = counts[dom]
     ms, nl = prof[dom]
@@ -513,9 +529,10 @@ def main():
         ach = work / avg_s / 1e12
         # "mfma" only where the kernel issues matrix instructions; the register-resident
         # Dyson shapes do their FP64 arithmetic as vector FMAs (same 78.6 TFLOP/s peak)
-        roof = {"bound": "mfma" if eng.dyson_uses_mfma() else "fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s",
-                "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+        fp32 = dom == "dyson_keldysh_fp32"
+        peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+        roof = {"bound": "mfma" if (fp32 or eng.dyson_uses_mfma()) else "fp64", "achieved": ach, "peak": peak,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None}
     else:
         ach = work / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
@@ -550,6 +567,22 @@ def main():
         "stages": stages,
         "roofline": roof,
     }
+    if plain is not None:
+        p_total, p_lat, p_iters, p_conv, p_J = plain[:5]
+        rel = np.abs(Js - p_J) / np.maximum(np.abs(p_J), 1e-300)
+        line["dtype"] = "c64 MFMA Dyson/Keldysh + f64"
+        line["config"]["form"] = ("mixed precision: complex-FP32 MFMA Dyson/Keldysh while max|F| >= %g, FP64 steps "
+                                  "to tolf/10, FP64 final sweep; the plain FP64 path timed on the same points first"
+                                  % args.mixed_switch)
+        line["plain_fp64"] = {"value": args.steps / p_total, "ms_per_step": 1e3 * p_total / args.steps,
+                              "latency_ms": {"p50": 1e3 * float(np.percentile(p_lat, 50)),
+                                             "p99": 1e3 * float(np.percentile(p_lat, 99))},
+                              "scf_steps_per_point": float(np.mean(p_iters)), "all_converged": bool(all(p_conv))}
+        line["speedup_vs_plain"] = p_total / total
+        line["fp32_steps_per_point"] = float(np.mean(n32))
+        line["rel_dI_vs_plain"] = {"max": float(rel[np.abs(p_J) > 1e-9].max()) if np.any(np.abs(p_J) > 1e-9) else 0.0,
+                                   "per_point": [[float(volts[i]), float(rel[k, 0]), float(rel[k, 1])]
+                                                 for k, i in enumerate(sweep_points(nv, args.steps))]}
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (oracle, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args, float(np.mean(iters)))

@@ -99,6 +99,20 @@ int qs_g_less_obs(const qs_handle *h, double *G);                    /* G_less_o
 int qs_dosd(const qs_handle *h, double *dosd);                       /* dosd (norb, n_omega) real */
 int qs_trUD(const qs_handle *h, double *trU, double *trD);           /* current integrand (n_omega) each */
 
+/* Precision of the order-2 SCF sweeps (no counterpart in the reference, which is
+ * FP64 throughout; replaces the FP64 zgetrf/zgetri and matmul of
+ * common/linear_algebra.f90:58-88 and src/scf.f90:318-344 in the early steps).
+ * mode 0: FP64 (the default, the drop-in path).  mode 1, mixed: the Dyson inverse
+ * and the two Keldysh products run in complex FP32 on MFMA with FP32 accumulation
+ * while max|F| >= switch_fmax (and keeps falling), then FP64 steps take the
+ * solution to tolf / 10 (a decade below the FP64 path's stopping point, so the
+ * currents agree with it to ~tolf); the Sigma2 FFT, the integrals, Broyden and the final sweep
+ * are FP64 throughout.  Mode 1 allocates the large-Norb chunk buffers if the
+ * size has none.  qs_precision_steps gives the last solve's residual counts in
+ * each precision. */
+int qs_set_precision(qs_handle *h, int32_t mode, double switch_fmax);
+int qs_precision_steps(const qs_handle *h, int32_t *fp32_steps, int32_t *fp64_steps);
+
 /* --- seams inside the SCF step (order 2) ------------------------------------ */
 /* sweep(M_INIT2) + pack_gf (src/scf.f90:116-122): the non-interacting active
  * G^r, G^< packed into the handle's x; copied to x_out (n_vec) if not NULL */

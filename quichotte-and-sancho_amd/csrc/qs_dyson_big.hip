@@ -13,6 +13,9 @@
 //                 zgetrf + zgetri (common/linear_algebra.f90:58-88)
 //   k_cgemm x 2   T = G^r S'', G^< = T G^r^dagger on MFMA (src/scf.f90:343-344)
 //   k_big_pack    the packing epilogues of the mode (as the other two kernels)
+// With DysonArgs::fp32 the matrices, the inverse and the products are complex FP32
+// (FP32 MFMA, FP32 accumulation); A and S'' are built in FP64 and rounded, and the
+// packing epilogues widen the results to FP64.
 // The padding block is the identity in A and zero in S'', so it stays out of
 // G^r's top-left n x n block and of G^<.
 #include "qs_device.h"
@@ -31,19 +34,29 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 // A(i, j) = w + Hemb - Sigma1 - Sigma2^r and S(i, j) = i (f_L GamL + f_R GamR) + Sigma2^<
 // on the support (zero elsewhere), for energy iwc + blockIdx.x; row-major np x np,
 // identity (A) and zero (S) past n
-__global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, int np, double2 *__restrict__ Ab,
-                                                  double2 *__restrict__ Sb) {
+// the matrices are stored as CT: double2 (FP64 path) or float2 (complex FP32 path)
+__device__ __forceinline__ void put(double2 *p, double2 v) { *p = v; }
+__device__ __forceinline__ void put(float2 *p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ double2 get(const double2 *p) { return *p; }
+__device__ __forceinline__ double2 get(const float2 *p) {
+    const float2 v = *p;
+    return make_double2((double)v.x, (double)v.y);
+}
+
+template <typename CT>
+__global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, int np, CT *__restrict__ Ab,
+                                                  CT *__restrict__ Sb) {
     const int n = A.norb, iw = iwc + blockIdx.x;
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     const bool use_s1 = A.use_sigma && A.sig1;
-    double2 *a = Ab + (size_t)blockIdx.x * np * np, *s = Sb + (size_t)blockIdx.x * np * np;
+    CT *a = Ab + (size_t)blockIdx.x * np * np, *s = Sb + (size_t)blockIdx.x * np * np;
     for (int idx = threadIdx.x; idx < np * np; idx += NT) {
         const int i = idx / np, j = idx - i * np;  // row-major (i, j)
         if (i >= n || j >= n) {
-            a[idx] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-            s[idx] = make_double2(0.0, 0.0);
+            put(a + idx, make_double2(i == j ? 1.0 : 0.0, 0.0));
+            put(s + idx, make_double2(0.0, 0.0));
             continue;
         }
         double2 v = A.Hemb[(size_t)i * n + j];
@@ -51,7 +64,7 @@ __global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, int np, 
         if (use_s1 && (i >> 1) == (j >> 1)) v = csub(v, A.sig1[(i >> 1) * 4 + (i & 1) * 2 + (j & 1)]);
         const int pi = A.pos[i], pj = A.pos[j];
         if (sigw && pi >= 0 && pj >= 0) v = csub(v, sigw[sig_r_index(pi, pj, A.nAs)]);
-        a[idx] = v;
+        put(a + idx, v);
         const int si = A.suppos[i], sj = A.suppos[j];
         double2 sv = make_double2(0.0, 0.0);
         if (si >= 0 && sj >= 0) {
@@ -63,7 +76,7 @@ __global__ __launch_bounds__(NT) void k_big_build(DysonArgs A, int iwc, int np, 
                 sv = cadd(sv, sig_l_value(sigw[sl.idx], sl.flip));
             }
         }
-        s[idx] = sv;
+        put(s + idx, sv);
     }
 }
 
@@ -89,13 +102,14 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
 }
 
 // the epilogue of the mode for energy iwc + blockIdx.x from G^r and G^< (row-major np x np)
-__global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, int np, const double2 *__restrict__ Gr,
-                                                 const double2 *__restrict__ Gl) {
+template <typename CT>
+__global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, int np, const CT *__restrict__ Gr,
+                                                 const CT *__restrict__ Gl) {
     __shared__ double red[NT / 64];
     const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL, iw = iwc + blockIdx.x, tid = threadIdx.x;
-    const double2 *gr = Gr + (size_t)blockIdx.x * np * np, *gl = Gl + (size_t)blockIdx.x * np * np;
-    auto GR = [&](int i, int j) { return gr[(size_t)i * np + j]; };
-    auto GL = [&](int i, int j) { return gl[(size_t)i * np + j]; };
+    const CT *gr = Gr + (size_t)blockIdx.x * np * np, *gl = Gl + (size_t)blockIdx.x * np * np;
+    auto GR = [&](int i, int j) { return get(gr + (size_t)i * np + j); };
+    auto GL = [&](int i, int j) { return get(gl + (size_t)i * np + j); };
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta);
     if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
@@ -247,17 +261,26 @@ hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     if (!bw || bw->n != a.norb || a.norb > qs_dyson_big_max_norb()) return hipErrorInvalidValue;
     const int np = bw->np;
+    const bool f32 = a.fp32 != 0;
     hipError_t e;
     for (int c0 = 0; c0 < count; c0 += bw->chunk) {
         const int cnt = count - c0 < bw->chunk ? count - c0 : bw->chunk, iwc = a.iw0 + c0;
-        hipLaunchKernelGGL(k_big_build, dim3(cnt), dim3(NT), 0, s, a, iwc, np, bw->A, bw->S);
+        if (f32)
+            hipLaunchKernelGGL(k_big_build<float2>, dim3(cnt), dim3(NT), 0, s, a, iwc, np, (float2 *)bw->A,
+                               (float2 *)bw->S);
+        else
+            hipLaunchKernelGGL(k_big_build<double2>, dim3(cnt), dim3(NT), 0, s, a, iwc, np, bw->A, bw->S);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         // G^r into T (A is left as scratch)
-        if ((e = qs_launch_gj_inv(np, cnt, bw->A, bw->T, a.status, false, s)) != hipSuccess) return e;
+        if ((e = qs_launch_gj_inv(np, cnt, bw->A, bw->T, a.status, f32, s)) != hipSuccess) return e;
         // T1 = G^r S'' into A;  G^< = T1 G^r^dagger into S
-        if ((e = qs_launch_cgemm(np, cnt, bw->T, bw->S, bw->A, false, false, s)) != hipSuccess) return e;
-        if ((e = qs_launch_cgemm(np, cnt, bw->A, bw->T, bw->S, true, false, s)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_big_pack, dim3(cnt), dim3(NT), 0, s, a, iwc, np, bw->T, bw->S);
+        if ((e = qs_launch_cgemm(np, cnt, bw->T, bw->S, bw->A, false, f32, s)) != hipSuccess) return e;
+        if ((e = qs_launch_cgemm(np, cnt, bw->A, bw->T, bw->S, true, f32, s)) != hipSuccess) return e;
+        if (f32)
+            hipLaunchKernelGGL(k_big_pack<float2>, dim3(cnt), dim3(NT), 0, s, a, iwc, np, (const float2 *)bw->T,
+                               (const float2 *)bw->S);
+        else
+            hipLaunchKernelGGL(k_big_pack<double2>, dim3(cnt), dim3(NT), 0, s, a, iwc, np, bw->T, bw->S);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

@@ -50,11 +50,11 @@ constexpr size_t kBigBudget = (size_t)2 << 30;  // chunk buffers of the large-No
 // stages timed with HIP events on the handle's stream (qs_profile)
 enum Stage {
     ST_EQT = 0, ST_PERMUTE, ST_FFT_F1, ST_FFT_MID, ST_FFT_I2, ST_DYSON, ST_REDUCE, ST_HIST, ST_UPDATE, ST_FINAL,
-    ST_COUNT
+    ST_DYSON32, ST_COUNT
 };
 const char *const kStageNames[ST_COUNT] = {"equal_time_sigma1", "permute_in",      "sigma2_fft_f1",  "sigma2_fft_mid",
                                            "sigma2_fft_i2",     "dyson_keldysh",   "reductions",     "broyden_history",
-                                           "broyden_update",    "final_sweep"};
+                                           "broyden_update",    "final_sweep",     "dyson_keldysh_fp32"};
 
 struct Prof {
     bool on = false;
@@ -122,6 +122,11 @@ struct qs_handle {
     Prof prof;
     int dyson_variant = 0;
     double dyson_tune_ms[4] = {0, 0, 0, 0};
+    // precision of the order-2 SCF sweeps: 0 = FP64; 1 = complex-FP32 Dyson/Keldysh
+    // (MFMA) while max|F| >= mixed_switch, then FP64 to the end (and the final sweep)
+    int precision = 0;
+    double mixed_switch = 1e-5;
+    int steps32 = 0, steps64 = 0;   // residuals of the last solve in each precision
 };
 
 namespace {
@@ -476,12 +481,18 @@ DysonArgs dyson_args(qs_handle *h, int mode, bool use_sigma, double V) {
     return a;
 }
 
-int run_dyson(qs_handle *h, int mode, bool use_sigma, double V) {
+// fp32: the complex-FP32 large-Norb path (mixed precision), whatever the tuned shape
+int run_dyson(qs_handle *h, int mode, bool use_sigma, double V, bool fp32 = false) {
     DysonArgs a = dyson_args(h, mode, use_sigma, V);
     HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(int), h->stream));
     hipEvent_t t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream));
-    prof_end(h, mode == QS_M_FINAL ? ST_FINAL : ST_DYSON, t0);
+    if (fp32) {
+        a.fp32 = 1;
+        HIPCHK(h, qs_launch_dyson_variant(a, h->nw, QS_DYSON_BIG, h->stream));
+    } else {
+        HIPCHK(h, qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream));
+    }
+    prof_end(h, mode == QS_M_FINAL ? ST_FINAL : (fp32 ? ST_DYSON32 : ST_DYSON), t0);
     return QS_OK;
 }
 
@@ -595,7 +606,8 @@ int tune_dyson(qs_handle *h) {
 
 // scf_residual2 on the device: d_x -> d_F; stats[3] = max|F|, |F|^2, |F-Fold|^2
 // gt_ready: Gt already holds x in the FFT input layout (written by the fused Broyden update)
-int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, Sanity *san, bool gt_ready = false) {
+int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, Sanity *san, bool gt_ready = false,
+                     bool fp32 = false) {
     const int nA = h->nA;
     h->V_scf = V;
     const double pp = h->p.delta / (2.0 * PI);
@@ -618,7 +630,7 @@ int residual2_device(qs_handle *h, double V, double stats[3], int *abort_flag, S
         prof_end(h, ST_FFT_F1 + part, t0);
     }
     h->sigma_valid = true;
-    int rc = run_dyson(h, QS_M_SCF2, true, V);
+    int rc = run_dyson(h, QS_M_SCF2, true, V, fp32);
     if (rc) return rc;
     t0 = prof_begin(h);
     HIPCHK(h, qs_launch_reduce_partials(h->d_partials, h->nw, h->d_red, h->stream));
@@ -797,18 +809,39 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
     *converged = 0;
     bool gt_ready = false;  // the last update also wrote the FFT input of the next residual
     bool dx_ready = false;  // ... and x - xold into d_dx
+    // mixed precision: FP32 sweeps while max|F| stays above the switch level and keeps
+    // falling (it stalls at the FP32 rounding floor), FP64 from then on
+    bool use32 = h->precision == 1 && h->order == 2;
+    double best32 = 1.79769313486231570e308;
+    int stall32 = 0;
+    h->steps32 = h->steps64 = 0;
     for (int its = 1; its <= h->p.maxscf; its++) {
         *n_iter = its;
         double st[3];
         int ab = 0;
-        if ((rc = residual2_device(h, V, st, &ab, last, gt_ready))) return rc;
+        if ((rc = residual2_device(h, V, st, &ab, last, gt_ready, use32))) return rc;
+        (use32 ? h->steps32 : h->steps64)++;
         gt_ready = false;
         if (ab) break;
         fmax = st[0];
         const double rms = std::sqrt(st[1]) / std::sqrt((double)n);
         if (lf) lf[its - 1] = fmax;
         if (lr) lr[its - 1] = rms;
-        if (fmax < h->p.tolf) { *converged = 1; break; }
+        if (use32) {
+            if (fmax < 0.5 * best32) {
+                best32 = fmax;
+                stall32 = 0;
+            } else {
+                stall32++;
+            }
+            // an FP32 residual never ends the loop: the next one is FP64
+            if (fmax < h->mixed_switch || fmax < h->p.tolf || stall32 >= 6) use32 = false;
+        } else if (fmax < (h->precision == 1 ? 0.1 * h->p.tolf : h->p.tolf)) {
+            // mixed precision ends a decade below tolf, so that its solution sits well
+            // inside the tolf-wide set of FP64 solutions and not just anywhere in it
+            *converged = 1;
+            break;
+        }
         bool need_proj = false;
         int slot = -1;
         double fnorm = 0.0;
@@ -1396,6 +1429,32 @@ int qs_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes) 
     if (hipMemGetInfo(&f, &t) != hipSuccess) return QS_ERR_HIP;
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
+    return QS_OK;
+}
+
+int qs_set_precision(qs_handle *h, int32_t mode, double switch_fmax) {
+    if (!h) return QS_ERR_ARG;
+    if (mode != 0 && mode != 1) return fail(h, QS_ERR_ARG, "precision mode must be 0 (FP64) or 1 (mixed)");
+    if (mode == 1 && !(switch_fmax > 0.0)) return fail(h, QS_ERR_ARG, "the switch level must be positive");
+    if (mode == 1 && h->order != 2) return fail(h, QS_ERR_STATE, "mixed precision applies to order 2");
+    if (mode == 1 && h->norb > qs_dyson_big_max_norb())
+        return fail(h, QS_ERR_UNSUPPORTED, "no complex-FP32 path above %d orbitals", qs_dyson_big_max_norb());
+    HIPCHK(h, hipSetDevice(h->p.device));
+    if (mode == 1 && !h->big) {
+        size_t bytes = 0;
+        h->big = qs_big_create(h->norb, h->nw, kBigBudget, &bytes);
+        if (!h->big) return fail(h, QS_ERR_NOMEM, "complex-FP32 Dyson workspace");
+        h->dev_bytes += (int64_t)bytes;
+    }
+    h->precision = mode;
+    if (mode == 1) h->mixed_switch = switch_fmax;
+    return QS_OK;
+}
+
+int qs_precision_steps(const qs_handle *h, int32_t *fp32_steps, int32_t *fp64_steps) {
+    if (!h) return QS_ERR_ARG;
+    if (fp32_steps) *fp32_steps = h->steps32;
+    if (fp64_steps) *fp64_steps = h->steps64;
     return QS_OK;
 }
 

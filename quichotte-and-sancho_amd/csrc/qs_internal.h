@@ -54,7 +54,8 @@ struct DysonArgs {
     int *status;             // bit 0: singular pivot met
     double2 *mscr;           // small-LDS kernel: 2 norb^2 complex of scratch per energy (M, T)
     int mscr_energies;       //   for this many energies (launches are chunked to it)
-    void *big;               // large-Norb path: its QsBigWork (library handle and chunk buffers)
+    void *big;               // large-Norb path: its QsBigWork (chunk buffers)
+    int fp32;                // large-Norb path: complex FP32 matrices, inverse and products
 };
 
 // Dyson + Keldysh at every energy of [iw0, iw0 + count).  Returns the hip error.

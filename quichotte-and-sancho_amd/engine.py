@@ -83,6 +83,8 @@ def load_library(path=SO):
     L.qs_profile.argtypes = [P, I32]
     L.qs_profile_read.argtypes = [P, P, P, I32]
     L.qs_dyson_choice.argtypes = [P, P, P, I32]
+    L.qs_set_precision.argtypes = [P, I32, D]
+    L.qs_precision_steps.argtypes = [P, P, P]
     L.qs_dyson_shape.argtypes = [P, P]
     L.qs_dyson_shape.restype = ctypes.c_char_p
     L.qs_stage_name.argtypes = [I32]
@@ -311,6 +313,17 @@ class ScfEngine:
         ms = np.zeros(4)
         n = self.L.qs_dyson_choice(self.h, ctypes.byref(v), _ptr(ms), 4)
         return v.value, ms[:n].tolist()
+
+    def set_precision(self, mode, switch_fmax=1e-5):
+        """0 = FP64; 1 = mixed (complex-FP32 MFMA Dyson/Keldysh while max|F| >=
+        switch_fmax, then FP64)"""
+        self._check(self.L.qs_set_precision(self.h, int(mode), float(switch_fmax)))
+
+    def precision_steps(self):
+        """(FP32 residuals, FP64 residuals) of the last solve"""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.L.qs_precision_steps(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def dyson_shape_name(self):
         return self.L.qs_dyson_shape(self.h, None).decode()
