@@ -113,6 +113,18 @@ int qs_trUD(const qs_handle *h, double *trU, double *trD);           /* current 
 int qs_set_precision(qs_handle *h, int32_t mode, double switch_fmax);
 int qs_precision_steps(const qs_handle *h, int32_t *fp32_steps, int32_t *fp64_steps);
 
+/* How order-2 SCF steps run (no counterpart in the reference, whose loop is
+ * host code; the steps are src/scf.f90:158-188 and common/broyden_mixing.f90:51-184).
+ * mode 0: the host makes the step's scalar decisions (sanity, Broyden slot, the
+ * Gram solve) between launches; 1: a fixed launch sequence with those decisions
+ * in device kernels, one small copy to the host per step; 2 (default): as 1, with
+ * steps from the third on replayed from two captured graphs.  Modes 1 and 2 apply
+ * to FP64 sweeps with FP32 history of at most 16 columns (the fused update); other
+ * configurations use mode 0.  qs_step_info: whether the device path applies, and
+ * how many steps of the last solve were graph replays. */
+int qs_set_step_mode(qs_handle *h, int32_t mode);
+int qs_step_info(const qs_handle *h, int32_t *device_steps_active, int32_t *graph_steps);
+
 /* --- seams inside the SCF step (order 2) ------------------------------------ */
 /* sweep(M_INIT2) + pack_gf (src/scf.f90:116-122): the non-interacting active
  * G^r, G^< packed into the handle's x; copied to x_out (n_vec) if not NULL */

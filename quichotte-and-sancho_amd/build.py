@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["qs_engine.cpp", "qs_dyson.hip", "qs_dyson_wave.hip", "qs_dyson_big.hip", "qs_gj_mfma.hip", "qs_sigma2.hip",
-           "qs_vector.hip", "qs_sancho.hip"]
+           "qs_vector.hip", "qs_control.hip", "qs_sancho.hip"]
 LIBS = []
 
 

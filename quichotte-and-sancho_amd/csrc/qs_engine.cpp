@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -61,6 +62,10 @@ struct Prof {
     std::vector<hipEvent_t> pool;
     struct Rec { int stage; hipEvent_t a, b; };
     std::vector<Rec> pending;
+    // while a graph is captured, its intervals go to capture_to; each replay
+    // records the same events again and is read with prof_collect_graph
+    std::vector<Rec> *capture_to = nullptr;
+    std::vector<Rec> graph_recs[2];
     double ms[ST_COUNT] = {0};
     int64_t count[ST_COUNT] = {0};
 };
@@ -127,6 +132,14 @@ struct qs_handle {
     int precision = 0;
     double mixed_switch = 1e-5;
     int steps32 = 0, steps64 = 0;   // residuals of the last solve in each precision
+    // order-2 SCF steps: 0 = host loop (scalars through the host each step), 1 = the
+    // step's scalar logic on the device (QsCtrl), 2 = as 1 with steps replayed from
+    // captured graphs (the default where the configuration allows it)
+    int step_mode = 2;
+    QsCtrl *d_ctl = nullptr;
+    QsCtrl *h_ctl = nullptr;        // pinned copy of the control header
+    double *d_csum = nullptr;       // column sums: 2 nA sanity + 2 MAXH history
+    int graph_steps = 0;            // steps of the last solve replayed from a graph
 };
 
 namespace {
@@ -158,18 +171,52 @@ hipEvent_t prof_event(qs_handle *h) {
     return e;
 }
 // open a timed region on the handle's stream; returns the start event (or null)
+// an event record on the handle's stream; while a graph is captured it becomes an
+// external event-record node, so each replay records the event again
+bool prof_record(qs_handle *h, hipEvent_t e) {
+    const hipError_t r = h->prof.capture_to ? hipEventRecordWithFlags(e, h->stream, hipEventRecordExternal)
+                                            : hipEventRecord(e, h->stream);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
 hipEvent_t prof_begin(qs_handle *h) {
     if (!h->prof.on) return nullptr;
     hipEvent_t a = prof_event(h);
-    if (a) (void)hipEventRecord(a, h->stream);
+    if (a && !prof_record(h, a)) {
+        h->prof.pool.push_back(a);
+        return nullptr;
+    }
     return a;
 }
 void prof_end(qs_handle *h, int stage, hipEvent_t a) {
     if (!a) return;
     hipEvent_t b = prof_event(h);
-    if (!b) return;
-    (void)hipEventRecord(b, h->stream);
-    h->prof.pending.push_back({stage, a, b});
+    if (!b || !prof_record(h, b)) {
+        h->prof.pool.push_back(a);
+        if (b) h->prof.pool.push_back(b);
+        return;
+    }
+    (h->prof.capture_to ? *h->prof.capture_to : h->prof.pending).push_back({stage, a, b});
+}
+// after a replay of graph `par` has finished: add its intervals
+void prof_collect_graph(qs_handle *h, int par) {
+    for (auto &r : h->prof.graph_recs[par]) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            h->prof.ms[r.stage] += ms;
+            h->prof.count[r.stage] += 1;
+        }
+    }
+}
+void prof_release_graph(qs_handle *h, int par) {
+    for (auto &r : h->prof.graph_recs[par]) {
+        h->prof.pool.push_back(r.a);
+        h->prof.pool.push_back(r.b);
+    }
+    h->prof.graph_recs[par].clear();
 }
 // after a stream synchronisation: fold the pending intervals into the totals
 void prof_collect(qs_handle *h) {
@@ -449,6 +496,9 @@ int setup_device(qs_handle *h) {
             }
         }
         if ((rc = dalloc(h, &h->d_gt0, (size_t)4 * h->nAs))) return rc;
+        if ((rc = dalloc(h, &h->d_ctl, 1))) return rc;
+        if ((rc = dalloc(h, &h->d_csum, (size_t)2 * std::max(nA, 1) + 2 * MAXH_DEV))) return rc;
+        HIPCHK(h, hipHostMalloc((void **)&h->h_ctl, sizeof(QsCtrl), hipHostMallocDefault));
     }
     HIPCHK(h, hipDeviceSynchronize());
     return QS_OK;
@@ -915,6 +965,156 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
     return QS_OK;
 }
 
+// whether the device-controlled steps apply: order 2, FP64 sweeps, FP32 history of at
+// most 16 columns with the fused update (x - xold kept)
+bool ctl_steps_ok(const qs_handle *h) {
+    return h->order == 2 && h->precision == 0 && h->p.single_history && h->p.nhist > 0 && h->p.nhist <= 16 &&
+           h->d_dx && h->d_ctl && qs_update_gt_fits(h->nA, h->ncolS, h->n_vec, 16);
+}
+
+// one order-2 SCF step with its scalar logic on the device: the residual
+// (scf_residual2, src/scf.f90:158-188), the sanity check, the history pass, the
+// Gram solve and the update (broyden_solve, common/broyden_mixing.f90:51-184),
+// then a copy of the control header to pinned host memory.  No host decision
+// inside, so the same sequence can be captured as a graph.
+int enqueue_ctl_step(qs_handle *h, double V) {
+    const int nA = h->nA;
+    const int64_t n = h->n_vec;
+    const double pp = h->p.delta / (2.0 * PI);
+    hipEvent_t t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_equal_time_sigma1(h->d_x, h->d_et_src, h->d_act_site, h->d_U_act, h->d_gt0, h->d_sig1, nA,
+                                          h->nAs, h->nw, pp, h->d_scratch, h->stream));
+    prof_end(h, ST_EQT, t0);
+    Sigma2Plan sp{};
+    sp.nA = nA; sp.nAs = h->nAs; sp.nw = h->nw; sp.Nf = h->Nf; sp.N1 = h->N1; sp.N2 = h->N2;
+    sp.log2N1 = h->log2N1; sp.log2N2 = h->log2N2; sp.delta = h->p.delta; sp.tw = h->d_tw; sp.Upair = h->d_Upair;
+    sp.ncol = h->ncolS;
+    for (int part = 0; part < 3; part++) {
+        t0 = prof_begin(h);
+        HIPCHK(h, qs_launch_sigma2(sp, h->d_Gt, h->d_Z, h->d_Gt, 1 << part, h->stream));
+        prof_end(h, ST_FFT_F1 + part, t0);
+    }
+    h->sigma_valid = true;
+    DysonArgs a = dyson_args(h, QS_M_SCF2, true, V);
+    HIPCHK(h, qs_launch_ctl_status(h->d_status, h->stream));
+    t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream));
+    prof_end(h, ST_DYSON, t0);
+    t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_reduce_partials(h->d_partials, h->nw, &h->d_ctl->fmax, h->stream));
+    HIPCHK(h, qs_launch_reduce_cols(h->d_diagR, nA, h->nw, h->d_csum, h->d_scratch, h->stream));
+    HIPCHK(h, qs_launch_reduce_cols(h->d_diagL, nA, h->nw, h->d_csum + nA, h->d_scratch, h->stream));
+    HIPCHK(h, qs_launch_ctl_check(h->d_ctl, h->d_csum, h->d_csum + nA, nA, h->p.delta, h->d_act, h->d_status,
+                                  h->p.nhist, h->p.tolf, h->stream));
+    prof_end(h, ST_REDUCE, t0);
+    const int nblocks = qs_vec_blocks(n);
+    t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_broyden_hist_f32_ctl(n, h->d_F, h->d_Fold, h->d_dx, h->p.alpha_mix, (float *)h->d_hist_dF,
+                                             (float *)h->d_hist_U, h->d_ctl, h->d_scratch, nblocks, h->stream));
+    prof_end(h, ST_HIST, t0);
+    HIPCHK(h, qs_launch_reduce_cols(h->d_scratch, 2 * MAXH_DEV, nblocks, h->d_csum + 2 * nA,
+                                    h->d_scratch + (size_t)nblocks * 2 * MAXH_DEV, h->stream));
+    HIPCHK(h, qs_launch_ctl_gram(h->d_ctl, h->d_csum + 2 * nA, h->p.nhist, h->stream));
+    t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_broyden_update_f32_gt_ctl(n, h->d_x, h->d_F, (const float *)h->d_hist_U, h->d_ctl,
+                                                  h->p.maxscf, h->p.alpha_mix, h->d_xold, h->d_dx, h->d_Gt,
+                                                  h->d_blk_src, nA, h->ncolS, h->nw, h->stream));
+    prof_end(h, ST_UPDATE, t0);
+    HIPCHK(h, hipMemcpyAsync(h->h_ctl, h->d_ctl, offsetof(QsCtrl, gamma), hipMemcpyDeviceToHost, h->stream));
+    return QS_OK;
+}
+
+// broyden_solve with the scalar logic on the device.  Steps 1 and 2 run as plain
+// launches; later steps replay one of two graphs captured from the same launch
+// sequence (x / xold and F / Fold trade places every step, so the buffers of
+// step k+2 are those of step k).
+int broyden_device_ctl(qs_handle *h, double V, int *converged, int *n_iter, double *fmax_out, Sanity *last,
+                       double *lf, double *lr) {
+    const int64_t n = h->n_vec;
+    int rc = QS_OK;
+    *converged = 0;
+    h->graph_steps = 0;
+    HIPCHK(h, qs_launch_ctl_reset(h->d_ctl, h->d_status, h->stream));
+    // the first residual's FFT input from x (later ones come from the fused update)
+    hipEvent_t t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, h->nA, h->ncolS, h->nw, h->stream));
+    prof_end(h, ST_PERMUTE, t0);
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    auto release = [&]() {
+        for (int p = 0; p < 2; p++) {
+            if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+            if (graph[p]) (void)hipGraphDestroy(graph[p]);
+            prof_release_graph(h, p);
+        }
+    };
+    double fmax = 1.79769313486231570e308;
+    for (int its = 1; its <= h->p.maxscf; its++) {
+        *n_iter = its;
+        const int par = its & 1;
+        if (h->step_mode == 2 && its > 2) {
+            if (!exec[par]) {
+                h->prof.capture_to = &h->prof.graph_recs[par];
+                hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    rc = enqueue_ctl_step(h, V);
+                    e = hipStreamEndCapture(h->stream, &graph[par]);
+                    if (rc == QS_OK && e == hipSuccess) e = hipGraphInstantiate(&exec[par], graph[par], nullptr, nullptr, 0);
+                }
+                h->prof.capture_to = nullptr;
+                if (rc != QS_OK || e != hipSuccess) {
+                    release();
+                    if (rc == QS_OK) rc = fail(h, QS_ERR_HIP, "SCF step graph: %s", hipGetErrorString(e));
+                    return rc;
+                }
+            }
+            hipError_t e = hipGraphLaunch(exec[par], h->stream);
+            if (e != hipSuccess) {
+                release();
+                return fail(h, QS_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+            }
+            h->graph_steps++;
+        } else if ((rc = enqueue_ctl_step(h, V))) {
+            release();
+            return rc;
+        }
+        {
+            hipError_t e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) {
+                release();
+                return fail(h, QS_ERR_HIP, "SCF step: %s", hipGetErrorString(e));
+            }
+        }
+        if (h->prof.on) {
+            if (h->step_mode == 2 && its > 2) prof_collect_graph(h, par);
+            else prof_collect(h);
+        }
+        const QsCtrl &c = *h->h_ctl;
+        if (c.singular) {
+            release();
+            return fail(h, QS_ERR_SINGULAR, "singular matrix in the Dyson inverse (zgetrf INFO != 0)");
+        }
+        last->runaway = c.abort;
+        last->unphysical = c.unphysical;
+        last->i_worst = c.i_worst;
+        last->n_worst = c.n_worst;
+        last->w_worst = c.w_worst;
+        if (c.abort) break;
+        fmax = c.fmax;
+        if (lf) lf[its - 1] = fmax;
+        if (lr) lr[its - 1] = std::sqrt(c.s2) / std::sqrt((double)n);
+        if (c.converged) {
+            *converged = 1;
+            break;
+        }
+        std::swap(h->d_x, h->d_xold);  // x <- x_new, xold <- x
+        std::swap(h->d_F, h->d_Fold);  // Fold <- F
+    }
+    release();
+    *fmax_out = fmax;
+    return QS_OK;
+}
+
 }  // namespace
 
 namespace {
@@ -1141,6 +1341,7 @@ void qs_destroy(qs_handle *h) {
     for (auto &r : h->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
     if (h->h_pin) (void)hipHostFree(h->h_pin);
+    if (h->h_ctl) (void)hipHostFree(h->h_ctl);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -1310,7 +1511,12 @@ int qs_solve_scf(qs_handle *h, double V, int32_t fresh, qs_scf_result *res, doub
         int conv = 0, nit = 0;
         double fmax = 0.0;
         Sanity last;
-        if ((rc = broyden_device(h, V, &conv, &nit, &fmax, &last, log_fmax, log_rms))) return rc;
+        if (h->step_mode > 0 && ctl_steps_ok(h)) {
+            if ((rc = broyden_device_ctl(h, V, &conv, &nit, &fmax, &last, log_fmax, log_rms))) return rc;
+        } else {
+            h->graph_steps = 0;
+            if ((rc = broyden_device(h, V, &conv, &nit, &fmax, &last, log_fmax, log_rms))) return rc;
+        }
         r.converged = conv;
         r.n_iter = nit;
         r.fmax = fmax;
@@ -1448,6 +1654,20 @@ int qs_set_precision(qs_handle *h, int32_t mode, double switch_fmax) {
     }
     h->precision = mode;
     if (mode == 1) h->mixed_switch = switch_fmax;
+    return QS_OK;
+}
+
+int qs_set_step_mode(qs_handle *h, int32_t mode) {
+    if (!h) return QS_ERR_ARG;
+    if (mode < 0 || mode > 2) return fail(h, QS_ERR_ARG, "step mode must be 0, 1 or 2");
+    h->step_mode = mode;
+    return QS_OK;
+}
+
+int qs_step_info(const qs_handle *h, int32_t *device_steps_active, int32_t *graph_steps) {
+    if (!h) return QS_ERR_ARG;
+    if (device_steps_active) *device_steps_active = (h->step_mode > 0 && ctl_steps_ok(h)) ? 1 : 0;
+    if (graph_steps) *graph_steps = h->graph_steps;
     return QS_OK;
 }
 

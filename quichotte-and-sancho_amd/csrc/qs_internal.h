@@ -94,6 +94,29 @@ size_t qs_dyson_wave_lds_bytes(int norb, int nCL);
 size_t qs_dyson_wave_scratch_bytes(int norb);
 int qs_dyson_wave_max_norb();
 
+// Device-side control of order-2 SCF steps (qs_control.hip): the scalar state of
+// broyden_solve (common/broyden_mixing.f90:51-184) and each step's outcome, kept
+// in device memory so that a whole step runs without the host (and can be a
+// captured graph).  fmax, s2, d2 are contiguous: k_reduce_partials writes them.
+struct QsCtrl {
+    int its, nstored, nh, slot;          // residuals so far; history ring bookkeeping
+    int nh_use, stop, converged, abort;  // nh_use: history columns in the update (0 after a reset)
+    int singular, unphysical, i_worst, pad;
+    double fmax, s2, d2, rf;             // max|F|, |F|^2, |F - Fold|^2 of the last residual; 1/|F - Fold|
+    double n_worst, w_worst;
+    double gamma[32];
+    double gram[32 * 32];
+};
+hipError_t qs_launch_ctl_reset(QsCtrl *c, int *status, hipStream_t s);
+hipError_t qs_launch_ctl_status(int *status, hipStream_t s);
+// after the residual: sanity from the column sums of Im G^r_kk, Im G^<_kk (label: act, 0-based),
+// its += 1, the stop decision, and the history slot / 1 / |F - Fold| of this step
+hipError_t qs_launch_ctl_check(QsCtrl *c, const double *sumR, const double *sumL, int nb, double delta,
+                               const int *label, const int *status, int nhist, double tolf, hipStream_t s);
+// after the history pass: the Gram row of the new slot, the projections and the
+// (w0^2 + Gram) gamma = cvec solve (dposv; a failed factorisation resets the history)
+hipError_t qs_launch_ctl_gram(QsCtrl *c, const double *sums, int nhist, hipStream_t s);
+
 // Second-order self-energy stage (qs_sigma2.hip)
 struct Sigma2Plan {
     int nA, nAs, nw, Nf, N1, N2, log2N1, log2N2;
@@ -112,6 +135,12 @@ bool qs_update_gt_fits(int nA, int ncol, int64_t n, int nh);
 hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const double *F, const float *Us,
                                            const double *gamma, int nh, double alpha, double *xnew, double *dx,
                                            double2 *Gt, const int *blk_src, int nA, int ncol, int nw, hipStream_t s);
+// the same with gamma and the history count from the control block; nothing when the
+// step stopped, and no Gt once its >= maxscf (the final sweep reads Sigma2 there)
+hipError_t qs_launch_broyden_update_f32_gt_ctl(int64_t n, const double *x, const double *F, const float *Us,
+                                               const QsCtrl *c, int maxscf, double alpha, double *xnew, double *dx,
+                                               double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
+                                               hipStream_t s);
 // parts: bit 0 forward pass over n1, bit 1 middle pass (row DFTs + 2x2 algebra),
 // bit 2 inverse pass over k1; the three in order make one Sigma2 evaluation.
 hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, double2 *Sig, int parts,
@@ -130,6 +159,11 @@ hipError_t qs_launch_reduce_partials(const double *partials, int nblocks, double
 hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *Fold, const double *x,
                                       const double *xold, const double *dx, double rf, double alpha, float *dFs,
                                       float *Us, int slot, int nh, double *partials, int nblocks, hipStream_t s);
+// the FP32-history pass with slot, 1/|F - Fold| and the history count from the control
+// block (dx as above; nothing is stored when the step stopped or slot < 0)
+hipError_t qs_launch_broyden_hist_f32_ctl(int64_t n, const double *F, const double *Fold, const double *dx,
+                                          double alpha, float *dFs, float *Us, const QsCtrl *c, double *partials,
+                                          int nblocks, hipStream_t s);
 hipError_t qs_launch_broyden_hist_f64(int64_t n, const double *F, const double *Fold, const double *x,
                                       const double *xold, double fnorm, double alpha, double *dF, double *U,
                                       int slot, int nh, double *partials, int nblocks, hipStream_t s);

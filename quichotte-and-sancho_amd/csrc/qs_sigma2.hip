@@ -288,12 +288,22 @@ __global__ __launch_bounds__(PL_NT) void k_permute_lds(const double *__restrict_
 // writes it.  With dx, x_new - x also goes out: the next history pass reads it
 // in place of x and xold (the same FP64 difference).  One workgroup per CU,
 // persistent over the energies.
+// ctl (not null): gamma and nh from the control block; nothing when the step
+// stopped, and no Gt once its >= maxscf
 struct GamH { double g[16]; };
 __global__ __launch_bounds__(PL_NT) void k_update_f32_gt(int64_t n, const double *__restrict__ x,
                                                          const double *__restrict__ F, const float *__restrict__ Us,
                                                          GamH gm, int nh, double alpha, double *__restrict__ xnew,
                                                          double *__restrict__ dx, double2 *__restrict__ Gt,
-                                                         const int2 *__restrict__ src, int nA, int ncol, int nw) {
+                                                         const int2 *__restrict__ src, int nA, int ncol, int nw,
+                                                         const QsCtrl *__restrict__ ctl, int maxscf) {
+    if (ctl) {
+        if (ctl->stop) return;
+        nh = ctl->nh_use;
+#pragma unroll
+        for (int k = 0; k < 16; k++) gm.g[k] = k < nh ? ctl->gamma[k] : 0.0;
+        if (ctl->its >= maxscf) Gt = nullptr;
+    }
     extern __shared__ __align__(16) char lds_raw[];
     double *xs = (double *)lds_raw;
     const int tid = threadIdx.x, nb = 3 * nA * nA, nq = nb / 4;  // 4 | nb (nA is even)
@@ -324,10 +334,12 @@ __global__ __launch_bounds__(PL_NT) void k_update_f32_gt(int64_t n, const double
             ((double2 *)xs)[2 * q + 1] = make_double2(v2, v3);
         }
         __syncthreads();
-        // the column map comes from L2 each time: registers go to the history loads
-        int2 sc[PL_QC];
-        load_colmap(src, sc, ncol, tid);
-        gather_gt(xs, sc, Gt + (size_t)iw * ncol, ncol, tid);
+        if (Gt) {
+            // the column map comes from L2 each time: registers go to the history loads
+            int2 sc[PL_QC];
+            load_colmap(src, sc, ncol, tid);
+            gather_gt(xs, sc, Gt + (size_t)iw * ncol, ncol, tid);
+        }
         __syncthreads();  // the next slice overwrites xs
     }
 }
@@ -700,7 +712,22 @@ hipError_t qs_launch_broyden_update_f32_gt(int64_t n, const double *x, const dou
     for (int k = 0; k < 16; k++) gm.g[k] = k < nh ? gamma[k] : 0.0;
     const int blocks = nw < ncu ? nw : ncu;
     hipLaunchKernelGGL(k_update_f32_gt, dim3(blocks), dim3(PL_NT), (size_t)3 * nA * nA * sizeof(double), s, n, x, F,
-                       Us, gm, nh, alpha, xnew, dx, Gt, (const int2 *)blk_src, nA, ncol, nw);
+                       Us, gm, nh, alpha, xnew, dx, Gt, (const int2 *)blk_src, nA, ncol, nw, (const QsCtrl *)nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_broyden_update_f32_gt_ctl(int64_t n, const double *x, const double *F, const float *Us,
+                                               const QsCtrl *c, int maxscf, double alpha, double *xnew, double *dx,
+                                               double2 *Gt, const int *blk_src, int nA, int ncol, int nw,
+                                               hipStream_t s) {
+    if (!c || !qs_update_gt_fits(nA, ncol, n, 16) || (int64_t)3 * nA * nA * nw != n) return hipErrorInvalidValue;
+    const int ncu = cu_count();
+    if (ncu <= 0) return hipErrorInvalidValue;
+    if (hipError_t e = qs_lds_attr((const void *)k_update_f32_gt)) return e;
+    GamH gm{};
+    const int blocks = nw < ncu ? nw : ncu;
+    hipLaunchKernelGGL(k_update_f32_gt, dim3(blocks), dim3(PL_NT), (size_t)3 * nA * nA * sizeof(double), s, n, x, F,
+                       Us, gm, 0, alpha, xnew, dx, Gt, (const int2 *)blk_src, nA, ncol, nw, c, maxscf);
     return hipGetLastError();
 }
 

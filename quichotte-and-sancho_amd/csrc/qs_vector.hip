@@ -235,13 +235,20 @@ __global__ __launch_bounds__(NT) void k_update_f64(int64_t n, const double *__re
 // 16-byte loads, so each thread keeps ~30 independent loads in flight.
 // n4 = n / 4 quads; the n % 4 tail is handled by the scalar kernel.
 // DX: x - xold comes from dx (written by the fused update) instead of x and xold
+// ctl (not null): slot, rf and nh come from the control block, and a stopped step stores nothing
 template <int MH, bool DX>
 __global__ __launch_bounds__(NT) void k_hist_f32x4(int64_t n, int64_t n4, const double *__restrict__ F,
                                                    const double *__restrict__ Fold, const double *__restrict__ x,
                                                    const double *__restrict__ xold, const double *__restrict__ dx,
                                                    double rf, double alpha,
                                                    float *__restrict__ dFs, float *__restrict__ Us, int slot, int nh,
-                                                   double *__restrict__ part) {
+                                                   double *__restrict__ part, const QsCtrl *__restrict__ ctl) {
+    if (ctl) {
+        if (ctl->stop) return;
+        slot = ctl->slot;
+        rf = ctl->rf;
+        nh = ctl->nh;
+    }
     __shared__ double red[NT / QS_WAVE][2 * MAXH];
     double g[MH], pr[MH];
 #pragma unroll
@@ -368,16 +375,25 @@ hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *
     if (nh > MAXH) return hipErrorInvalidValue;
     if (n % 4 == 0 && nh <= 16 && dx)
         hipLaunchKernelGGL((k_hist_f32x4<16, true>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold, dx, rf,
-                           alpha, dFs, Us, slot, nh, partials);
+                           alpha, dFs, Us, slot, nh, partials, (const QsCtrl *)nullptr);
     else if (n % 4 == 0 && nh <= 16)
         hipLaunchKernelGGL((k_hist_f32x4<16, false>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold,
-                           nullptr, rf, alpha, dFs, Us, slot, nh, partials);
+                           nullptr, rf, alpha, dFs, Us, slot, nh, partials, (const QsCtrl *)nullptr);
     else if (nh <= 16)
         hipLaunchKernelGGL(k_hist_f32<16>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us, slot,
                            nh, partials);
     else
         hipLaunchKernelGGL(k_hist_f32<MAXH>, dim3(nblocks), dim3(NT), 0, s, n, F, Fold, x, xold, rf, alpha, dFs, Us,
                            slot, nh, partials);
+    return hipGetLastError();
+}
+
+hipError_t qs_launch_broyden_hist_f32_ctl(int64_t n, const double *F, const double *Fold, const double *dx,
+                                          double alpha, float *dFs, float *Us, const QsCtrl *c, double *partials,
+                                          int nblocks, hipStream_t s) {
+    if (n % 4 != 0 || !dx || !c) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_hist_f32x4<16, true>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, nullptr, nullptr, dx,
+                       0.0, alpha, dFs, Us, -1, 0, partials, c);
     return hipGetLastError();
 }
 

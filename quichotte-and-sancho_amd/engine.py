@@ -83,6 +83,8 @@ def load_library(path=SO):
     L.qs_profile.argtypes = [P, I32]
     L.qs_profile_read.argtypes = [P, P, P, I32]
     L.qs_dyson_choice.argtypes = [P, P, P, I32]
+    L.qs_set_step_mode.argtypes = [P, I32]
+    L.qs_step_info.argtypes = [P, P, P]
     L.qs_set_precision.argtypes = [P, I32, D]
     L.qs_precision_steps.argtypes = [P, P, P]
     L.qs_dyson_shape.argtypes = [P, P]
@@ -318,6 +320,16 @@ class ScfEngine:
         """0 = FP64; 1 = mixed (complex-FP32 MFMA Dyson/Keldysh while max|F| >=
         switch_fmax, then FP64)"""
         self._check(self.L.qs_set_precision(self.h, int(mode), float(switch_fmax)))
+
+    def set_step_mode(self, mode):
+        """0 = host loop, 1 = device-controlled steps, 2 = device-controlled steps replayed from graphs"""
+        self._check(self.L.qs_set_step_mode(self.h, int(mode)))
+
+    def step_info(self):
+        """(device-controlled steps apply, graph replays in the last solve)"""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.L.qs_step_info(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), b.value
 
     def precision_steps(self):
         """(FP32 residuals, FP64 residuals) of the last solve"""
