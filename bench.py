@@ -479,6 +479,7 @@ def main():
     print("bench: %s" % desc, file=sys.stderr, flush=True)
     t0 = time.time()
     eng = pkg.ScfEngine(pkg.ScfProblem(**prob))
+    eng.set_step_mode(2)   # one handle in this process: steps replayed from captured graphs
     dv, dms = eng.dyson_choice()
     print("bench: device memory %.1f GB, setup %.1f s, Dyson launch shape %d (tuned ms %s)" %
           (eng.device_bytes() / 1e9, time.time() - t0, dv, dms), file=sys.stderr, flush=True)
@@ -492,6 +493,7 @@ def main():
     def timed_sweep(tag):
         eng.profile(True)
         lat, iters, conv, Js, n32 = [], [], [], [], []
+        graph_steps.clear()
         fresh = True
         tstart = time.perf_counter()
         for k, i in enumerate(sweep_points(nv, args.steps)):
@@ -506,6 +508,7 @@ def main():
             conv.append(r.converged)
             Js.append(J)
             n32.append(eng.precision_steps()[0])
+            graph_steps.append(eng.step_info()[1])
             print("bench: %s V=%+.2f %s converged=%s steps=%d (FP32 %d) J=(%.9e, %.9e) %.3f s" %
                   (tag, volts[i], "fresh" if (i == 0 or (k and not conv[-2])) else "cont.", r.converged, r.n_iter,
                    n32[-1], J[0], J[1], lat[-1]), file=sys.stderr, flush=True)
@@ -514,6 +517,7 @@ def main():
         eng.profile(False)
         return total, lat, iters, conv, np.array(Js), n32, prof
 
+    graph_steps = []
     plain = None
     if args.precision == "mixed":
         plain = timed_sweep("plain")
@@ -563,6 +567,7 @@ def main():
                    "dyson_launch_shape": eng.dyson_shape_name()},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "scf_steps_per_point": float(np.mean(iters)),
+        "graph_replayed_steps_per_point": float(np.mean(graph_steps)) if graph_steps else 0.0,
         "all_converged": bool(all(conv)),
         "stages": stages,
         "roofline": roof,

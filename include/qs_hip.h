@@ -74,6 +74,14 @@ int qs_create(const qs_problem *p, qs_handle **out);
 void qs_destroy(qs_handle *h);
 const char *qs_last_error(const qs_handle *h);
 
+/* Point an existing handle at another system of the same sizes (norb, n_omega,
+ * order, nhist, history precision, device) and the same active subspace, lead
+ * contacts and atoms: H, Gamma_L/R, U, T, mu and the grid origin/step change, no
+ * device memory is allocated and the Dyson launch shape is kept.  The next solve
+ * must be fresh.  QS_ERR_STATE if the problem does not fit the handle.  (No
+ * counterpart in the reference; a request pool reuses handles with it.) */
+int qs_reset_problem(qs_handle *h, const qs_problem *p);
+
 /* sizes of the active subspace and of the packed vector (n_vec, src/scf.f90:61-68);
  * nf is the padded FFT length used on the device (a power of two >= 2 n_omega - 1) */
 int qs_sizes(const qs_handle *h, int32_t *nA, int32_t *nS, int32_t *nCL, int32_t *nAs, int64_t *n_vec,
@@ -116,10 +124,13 @@ int qs_precision_steps(const qs_handle *h, int32_t *fp32_steps, int32_t *fp64_st
 /* How order-2 SCF steps run (no counterpart in the reference, whose loop is
  * host code; the steps are src/scf.f90:158-188 and common/broyden_mixing.f90:51-184).
  * mode 0: the host makes the step's scalar decisions (sanity, Broyden slot, the
- * Gram solve) between launches; 1: a fixed launch sequence with those decisions
- * in device kernels, one small copy to the host per step; 2 (default): as 1, with
- * steps from the third on replayed from two captured graphs.  Modes 1 and 2 apply
- * to FP64 sweeps with FP32 history of at most 16 columns (the fused update); other
+ * Gram solve) between launches; 1 (default): a fixed launch sequence with those
+ * decisions in device kernels, one small copy to the host per step; 2: as 1, with
+ * steps from the third on replayed from two captured graphs (same results as 1,
+ * bit for bit).  Mode 2 is for a process whose other threads make no HIP calls on
+ * the device while a solve runs: a capture is disturbed by such calls (the solve
+ * then goes on as mode 1), and they can fail.  Modes 1 and 2 apply to FP64 sweeps
+ * with FP32 history of at most 16 columns (the fused update); other
  * configurations use mode 0.  qs_step_info: whether the device path applies, and
  * how many steps of the last solve were graph replays. */
 int qs_set_step_mode(qs_handle *h, int32_t mode);

@@ -133,9 +133,9 @@ struct qs_handle {
     double mixed_switch = 1e-5;
     int steps32 = 0, steps64 = 0;   // residuals of the last solve in each precision
     // order-2 SCF steps: 0 = host loop (scalars through the host each step), 1 = the
-    // step's scalar logic on the device (QsCtrl), 2 = as 1 with steps replayed from
-    // captured graphs (the default where the configuration allows it)
-    int step_mode = 2;
+    // step's scalar logic on the device (QsCtrl; the default where the configuration
+    // allows it), 2 = as 1 with steps replayed from captured graphs
+    int step_mode = 1;
     QsCtrl *d_ctl = nullptr;
     QsCtrl *h_ctl = nullptr;        // pinned copy of the control header
     double *d_csum = nullptr;       // column sums: 2 nA sanity + 2 MAXH history
@@ -337,7 +337,9 @@ int setup_device(qs_handle *h) {
     int rc;
     const int n = h->norb, nA = h->nA, nA2 = nA * nA, nw = h->nw;
     HIPCHK(h, hipSetDevice(h->p.device));
-    HIPCHK(h, hipStreamCreate(&h->stream));
+    // non-blocking: work on the null stream (another handle's synchronous copies)
+    // neither waits for this stream nor disturbs a graph being captured on it
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 4096 * sizeof(double), hipHostMallocDefault));
     {
         // zero pad past n^2: the small-footprint Dyson kernel reads whole
@@ -500,7 +502,7 @@ int setup_device(qs_handle *h) {
         if ((rc = dalloc(h, &h->d_csum, (size_t)2 * std::max(nA, 1) + 2 * MAXH_DEV))) return rc;
         HIPCHK(h, hipHostMalloc((void **)&h->h_ctl, sizeof(QsCtrl), hipHostMallocDefault));
     }
-    HIPCHK(h, hipDeviceSynchronize());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return QS_OK;
 }
 
@@ -601,6 +603,7 @@ Sanity diag_sanity(const double *sumImGL, const double *sumImGR, int nb, double 
 // results bit for bit.
 struct TuneRecord { int variant; double ms[4]; };
 std::mutex g_tune_mu;
+std::mutex g_capture_mu;   // graph captures of SCF steps, one at a time in the process
 std::map<std::tuple<int, int, int>, TuneRecord> g_tune;
 
 int tune_dyson(qs_handle *h) {
@@ -1041,33 +1044,44 @@ int broyden_device_ctl(qs_handle *h, double V, int *converged, int *n_iter, doub
     prof_end(h, ST_PERMUTE, t0);
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // destroys what exists and forgets it, so that a second call is harmless
     auto release = [&]() {
         for (int p = 0; p < 2; p++) {
             if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
             if (graph[p]) (void)hipGraphDestroy(graph[p]);
+            exec[p] = nullptr;
+            graph[p] = nullptr;
             prof_release_graph(h, p);
         }
     };
     double fmax = 1.79769313486231570e308;
+    bool graphs = h->step_mode == 2;
     for (int its = 1; its <= h->p.maxscf; its++) {
         *n_iter = its;
         const int par = its & 1;
-        if (h->step_mode == 2 && its > 2) {
-            if (!exec[par]) {
-                h->prof.capture_to = &h->prof.graph_recs[par];
-                hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-                if (e == hipSuccess) {
-                    rc = enqueue_ctl_step(h, V);
-                    e = hipStreamEndCapture(h->stream, &graph[par]);
-                    if (rc == QS_OK && e == hipSuccess) e = hipGraphInstantiate(&exec[par], graph[par], nullptr, nullptr, 0);
-                }
-                h->prof.capture_to = nullptr;
-                if (rc != QS_OK || e != hipSuccess) {
-                    release();
-                    if (rc == QS_OK) rc = fail(h, QS_ERR_HIP, "SCF step graph: %s", hipGetErrorString(e));
-                    return rc;
-                }
+        if (graphs && its > 2 && !exec[par]) {
+            // one capture at a time in the process; if the capture fails anyway (work
+            // of other threads can invalidate it), the solve goes on with plain
+            // launches of the same sequence, which give the same results
+            std::lock_guard<std::mutex> lk(g_capture_mu);
+            h->prof.capture_to = &h->prof.graph_recs[par];
+            hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+            int crc = QS_OK;
+            if (e == hipSuccess) {
+                crc = enqueue_ctl_step(h, V);
+                e = hipStreamEndCapture(h->stream, &graph[par]);
+                if (crc == QS_OK && e == hipSuccess)
+                    e = hipGraphInstantiate(&exec[par], graph[par], nullptr, nullptr, 0);
             }
+            h->prof.capture_to = nullptr;
+            if (crc != QS_OK || e != hipSuccess) {
+                (void)hipGetLastError();
+                h->err.clear();
+                release();
+                graphs = false;
+            }
+        }
+        if (graphs && its > 2) {
             hipError_t e = hipGraphLaunch(exec[par], h->stream);
             if (e != hipSuccess) {
                 release();
@@ -1086,7 +1100,7 @@ int broyden_device_ctl(qs_handle *h, double V, int *converged, int *n_iter, doub
             }
         }
         if (h->prof.on) {
-            if (h->step_mode == 2 && its > 2) prof_collect_graph(h, par);
+            if (graphs && its > 2) prof_collect_graph(h, par);
             else prof_collect(h);
         }
         const QsCtrl &c = *h->h_ctl;
@@ -1654,6 +1668,56 @@ int qs_set_precision(qs_handle *h, int32_t mode, double switch_fmax) {
     }
     h->precision = mode;
     if (mode == 1) h->mixed_switch = switch_fmax;
+    return QS_OK;
+}
+
+int qs_reset_problem(qs_handle *h, const qs_problem *p) {
+    if (!h || !p) return QS_ERR_ARG;
+    std::string msg;
+    if (int rc = check_problem(p, msg)) return fail(h, rc, "%s", msg.c_str());
+    if (p->norb != h->norb || p->n_omega != h->nw || p->order != h->order || p->nhist != h->p.nhist ||
+        (p->single_history != 0) != (h->p.single_history != 0) || p->device != h->p.device)
+        return fail(h, QS_ERR_STATE, "the new problem needs other sizes than this handle's");
+    // the new system's host setup, compared with the handle's
+    qs_handle t;
+    t.p = *p;
+    t.norb = p->norb; t.nw = p->n_omega; t.nsites = p->norb / 2; t.order = p->order;
+    const size_t nn = (size_t)t.norb * t.norb;
+    t.H.assign((const zd *)p->H, (const zd *)p->H + nn);
+    t.GamL.assign((const zd *)p->gammaL, (const zd *)p->gammaL + nn);
+    t.GamR.assign((const zd *)p->gammaR, (const zd *)p->gammaR + nn);
+    t.U.assign(p->U_site, p->U_site + t.norb);
+    if (!setup_atoms(&t, p)) return fail(h, QS_ERR_ARG, "atom_norb does not sum to norb/2");
+    t.Nf = h->Nf;
+    setup_storage(&t);
+    if (t.act != h->act || t.cl != h->cl || t.sup != h->sup || t.asite != h->asite || t.site_int != h->site_int ||
+        t.atom_norb != h->atom_norb)
+        return fail(h, QS_ERR_STATE, "the new problem has another active subspace, contact set or atom structure");
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->p = *p;
+    h->H.swap(t.H); h->GamL.swap(t.GamL); h->GamR.swap(t.GamR); h->U.swap(t.U);
+    h->Hemb.swap(t.Hemb); h->GamL_s.swap(t.GamL_s); h->GamR_s.swap(t.GamR_s); h->GamL_cl.swap(t.GamL_cl);
+    HIPCHK(h, hipMemcpy(h->d_Hemb, h->Hemb.data(), h->Hemb.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_gamL_s, h->GamL_s.data(), h->GamL_s.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_gamR_s, h->GamR_s.data(), h->GamR_s.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_gamL_cl, h->GamL_cl.data(), h->GamL_cl.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (h->order == 2) {
+        std::vector<double> uact(h->nAs);
+        for (int ia = 0; ia < h->nAs; ia++) uact[ia] = h->U[2 * h->asite[ia]];
+        const double pp = h->p.delta / (2.0 * PI);
+        std::vector<double> upair((size_t)h->nAs * h->nAs);
+        for (int ia = 0; ia < h->nAs; ia++)
+            for (int ja = 0; ja < h->nAs; ja++) upair[(size_t)ia * h->nAs + ja] = uact[ia] * uact[ja] * pp * pp / h->Nf;
+        HIPCHK(h, hipMemcpy(h->d_U_act, uact.data(), uact.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->d_Upair, upair.data(), upair.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    h->x_valid = false;
+    h->sigma_valid = false;
+    h->final_valid = false;
+    h->G_less_t0.assign(nn, 0.0);
+    h->Sig1.assign(nn, 0.0);
+    h->G_less_obs.assign(nn, 0.0);
     return QS_OK;
 }
 

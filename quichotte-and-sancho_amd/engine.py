@@ -83,6 +83,7 @@ def load_library(path=SO):
     L.qs_profile.argtypes = [P, I32]
     L.qs_profile_read.argtypes = [P, P, P, I32]
     L.qs_dyson_choice.argtypes = [P, P, P, I32]
+    L.qs_reset_problem.argtypes = [P, P]
     L.qs_set_step_mode.argtypes = [P, I32]
     L.qs_step_info.argtypes = [P, P, P]
     L.qs_set_precision.argtypes = [P, I32, D]
@@ -243,6 +244,16 @@ class ScfEngine:
         self.nA, self.nS, self.nCL, self.nAs = [v.value for v in a]
         self.n_vec = nv.value
         self.Nf = nf.value
+
+    def reset(self, prob: ScfProblem):
+        """qs_reset_problem: the same handle (device memory, launch shape) for
+        another system of the same sizes and active structure; QsError code 6 if
+        it does not fit."""
+        p, keep = _make_problem(prob)
+        p.device = self.prob.device
+        self._check(self.L.qs_reset_problem(self.h, ctypes.byref(p)))
+        self.prob = prob
+        self._H, self._GL, self._GR, self._U, self._an = keep
 
     def close(self):
         if getattr(self, "h", None):

@@ -46,60 +46,113 @@ class Response:
         return self.done - self.arrival
 
 
-class Server:
-    """Admission by HBM footprint, up to `max_running` requests on the GPU at once."""
+def pool_key(p: ScfProblem):
+    """Requests with equal keys can share a handle (qs_reset_problem checks the rest)."""
+    an = None if p.atom_norb is None else tuple(int(a) for a in p.atom_norb)
+    return (p.H.shape[0], int(p.n_omega), int(p.order), int(p.nhist), bool(p.single_history), an, int(p.device))
 
-    def __init__(self, max_running=4, hbm_fraction=0.85, device=0):
+
+class Server:
+    """Admission by HBM footprint, up to `max_running` requests on the GPU at once.
+
+    Finished requests leave their handle in a pool keyed by the request's sizes;
+    a later request of the same sizes takes it (qs_reset_problem) instead of
+    allocating and tuning a new one.  Idle handles count against the budget and
+    are released, oldest first, when a request of another size needs the memory."""
+
+    def __init__(self, max_running=4, hbm_fraction=0.85, device=0, pool=True):
         free, total = device_memory(device)
         self.budget = int(free * hbm_fraction)
         self.max_running = max_running
         self.cv = threading.Condition()
-        self.used = 0
+        self.used = 0              # footprints of the running requests
+        self.idle = []             # pooled handles: (key, engine, footprint), oldest first
+        self.idle_bytes = 0
+        self.use_pool = pool
         self.running = 0
         self.pending = []          # arrived, not admitted (arrival order)
         self.closed = False
+        self.reused = 0
 
-    def _admit_one(self):
-        """Under the lock: the first pending request that fits, or None."""
+    def _admit_one(self, evict):
+        """Under the lock: the first pending request that fits, with a pooled
+        handle if one matches; idle handles to release go to `evict`."""
         if self.running >= self.max_running:
             return None
         for i, (req, fp) in enumerate(self.pending):
-            if self.used + fp <= self.budget:
+            key = pool_key(req.problem)
+            for j, (k, eng, efp) in enumerate(self.idle):
+                if k == key:
+                    self.idle.pop(j)
+                    self.idle_bytes -= efp
+                    self.pending.pop(i)
+                    self.used += efp
+                    self.running += 1
+                    return req, efp, eng
+            if self.used + fp > self.budget:
+                continue
+            while self.idle and self.used + self.idle_bytes + fp > self.budget:
+                k, eng, efp = self.idle.pop(0)
+                self.idle_bytes -= efp
+                evict.append(eng)
+            if self.used + self.idle_bytes + fp <= self.budget:
                 self.pending.pop(i)
                 self.used += fp
                 self.running += 1
-                return req, fp
+                return req, fp, None
         return None
 
     def _worker(self, t0, out):
         while True:
+            evict = []
             with self.cv:
-                item = self._admit_one()
+                item = self._admit_one(evict)
                 while item is None:
                     if self.closed and not self.pending:
                         return
                     self.cv.wait(0.05)
-                    item = self._admit_one()
-            req, fp = item
+                    item = self._admit_one(evict)
+            for e in evict:
+                e.close()
+            req, fp, eng = item
             start = time.perf_counter() - t0
-            eng = None
+            keep = False
             try:
-                eng = ScfEngine(req.problem)
+                if eng is not None:
+                    try:
+                        eng.reset(req.problem)
+                        self.reused += 1
+                    except Exception:   # another active structure: a handle of its own
+                        eng.close()
+                        eng = None
+                if eng is None:
+                    eng = ScfEngine(req.problem)
                 r = eng.solve_scf(req.V, True)
                 ju, jd = eng.current()
                 resp = Response(req.rid, ju, jd, r.converged, r.n_iter, req.arrival, start,
                                 time.perf_counter() - t0, fp)
+                keep = self.use_pool
             except Exception as e:  # the request fails, the server goes on
                 resp = Response(req.rid, float("nan"), float("nan"), False, 0, req.arrival, start,
                                 time.perf_counter() - t0, fp, str(e))
             finally:
-                if eng is not None:
+                if eng is not None and not keep:
                     eng.close()   # its device memory is free before the budget says so
             with self.cv:
                 self.used -= fp
+                if keep:
+                    self.idle.append((pool_key(req.problem), eng, fp))
+                    self.idle_bytes += fp
                 self.running -= 1
                 out.append(resp)
                 self.cv.notify_all()
+
+    def shutdown(self):
+        """Release the pooled handles."""
+        with self.cv:
+            idle, self.idle, self.idle_bytes = self.idle, [], 0
+        for _, eng, _ in idle:
+            eng.close()
 
     def run(self, requests):
         """Release each request at its arrival time; return the responses in rid order."""

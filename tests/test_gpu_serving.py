@@ -40,6 +40,7 @@ def test_mix_equals_alone_and_oracle():
         e.close()
     srv = serving.Server(max_running=4)
     out = srv.run(reqs)
+    srv.shutdown()
     assert len(out) == len(reqs)
     for r, (J, nit) in zip(out, alone):
         assert r.error == ""
@@ -64,3 +65,55 @@ def test_admission_respects_budget():
         for b in out:
             if a.rid < b.rid and a.footprint + b.footprint > srv.budget:
                 assert a.done <= b.start + 1e-6 or b.done <= a.start + 1e-6
+
+
+def test_pooled_handle_reset_equals_fresh_handle():
+    """A handle reset to another system of the same sizes (qs_reset_problem)
+    solves it to the same bits as a handle made for it; a system with another
+    active subspace is refused and leaves the handle as it was."""
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+
+    def sp(U, mu, T, V_, U0=None):
+        p = helix_problem(4, 2, U, order=2, n_omega=384, wlo=0.0, whi=4.0, E_CC=2.0, t_hop=-0.4, lamb=0.05,
+                          gamma=1.0, mu=mu, T=T, tolf=1e-10, nhist=8, voltages=[V_])
+        Us = p.U_site.copy()
+        if U0 is not None:
+            Us[U0] = 0.0
+        return pkg.ScfProblem(H=p.H, gammaL=p.GL, gammaR=p.GR, U_site=Us, order=2, n_omega=p.n_omega,
+                              w_init=p.w_init, delta=p.delta, beta=p.beta, mu=p.mu, nhist=8, tolf=1e-10)
+
+    a, b = sp(0.6, 2.0, 300.0, 0.2), sp(0.9, 1.9, 150.0, -0.3)
+    e = pkg.ScfEngine(a)
+    ra = e.solve_scf(0.2, True)
+    Ja = e.current()
+    e.reset(b)
+    rb = e.solve_scf(-0.3, True)
+    Jb = e.current()
+    f = pkg.ScfEngine(b)
+    rf = f.solve_scf(-0.3, True)
+    assert rb.n_iter == rf.n_iter and Jb == f.current()
+    assert np.array_equal(e.g_less_obs(), f.g_less_obs())
+    f.close()
+    # U = 0 on a non-contact site: a smaller active subspace than the handle's
+    with pytest.raises(pkg.QsError) as err:
+        e.reset(sp(0.6, 2.0, 300.0, 0.2, U0=4))
+    assert err.value.code == 6
+    e.reset(a)
+    ra2 = e.solve_scf(0.2, True)
+    assert ra2.n_iter == ra.n_iter and e.current() == Ja
+    e.close()
+
+
+def test_pool_reuses_handles_of_equal_sizes():
+    pkg, serving, reqs, probs = make_requests()
+    # the same request list twice, the second round arriving after the first has finished
+    again = [serving.Request(r.rid + 100, r.problem, r.V, arrival=1.0 + r.arrival) for r in reqs]
+    srv = serving.Server(max_running=4)
+    out = srv.run(reqs + again)
+    srv.shutdown()
+    assert srv.reused >= len(reqs) // 2
+    first = {r.rid: r for r in out if r.rid < 100}
+    for r in out:
+        if r.rid >= 100:
+            a = first[r.rid - 100]
+            assert r.error == "" and r.n_iter == a.n_iter and r.J_up == a.J_up and r.J_down == a.J_down
