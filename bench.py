@@ -268,29 +268,38 @@ def cpu_baseline(args, n_iter_avg):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+# largest grid per size for the serving stream: one request's footprint (x, F, Fold,
+# xold, x - xold and 16 FP32 history columns per packed element) stays near 1/4 of HBM
+SERVE_SIZES = {16: ((4, 2), 16), 48: ((8, 3), 15), 80: ((8, 5), 14), 160: ((8, 10), 12), 240: ((12, 10), 11)}
+
+
 def serve_bench(args):
-    """Poisson arrivals of single-bias solves of helix systems with 16, 48 or 80
-    spin-orbitals, grids of 2^13 or 2^14 energies, U ~ U[0.5, 3], V ~ U[-2, 2]."""
+    """Poisson arrivals of single-bias solves (the serving stream): helix systems of
+    16/48/80/160/240 spin-orbitals drawn uniformly, grids of 2^13..2^16 energies on
+    (-12, 12] drawn uniformly and capped per size (SERVE_SIZES), U ~ U[0.5, 3] eV,
+    V ~ U[-2, 2] V, fresh starts, tolf 1e-8, nhist 16."""
     import importlib
     pkg = importlib.import_module("quichotte-and-sancho_amd")
     serving = importlib.import_module("quichotte-and-sancho_amd.serving")
     rates = [float(r) for r in args.serve.split(",")]
-    shapes = {16: (4, 2), 48: (8, 3), 80: (8, 5)}
+    sizes = [int(x) for x in args.serve_sizes.split(",")]
     out = {"metric": "serving: converged bias points/s and latency", "unit": "bias points/s", "n_gpus": 1,
            "dtype": "f64", "data": "synthetic", "higher_is_better": True,
-           "config": {"workload": "helix systems of 16/48/80 spin-orbitals, 2^13/2^14 energies, U~U[0.5,3], "
-                                  "V~U[-2,2], tolf 1e-8, nhist 16, Poisson arrivals", "requests_per_rate": args.requests},
+           "config": {"workload": ("helix systems of %s spin-orbitals, grids 2^13..2^16 on (-12,12] capped per size "
+                                   "at %s, U~U[0.5,3], V~U[-2,2], fresh starts, tolf 1e-8, nhist 16 FP32 history, "
+                                   "Poisson arrivals" % (sizes, {n: "2^%d" % SERVE_SIZES[n][1] for n in sizes})),
+                      "requests_per_rate": args.requests, "max_running": args.serve_streams},
            "rates": {}}
     for rate in rates:
         rng = np.random.default_rng(int(rate))
         t, reqs = 0.0, []
         for rid in range(args.requests):
             t += rng.exponential(1.0 / rate)
-            norb = int(rng.choice(list(shapes)))
-            lg = int(rng.choice([13, 14]))
+            norb = int(rng.choice(sizes))
+            lg = min(int(rng.choice([13, 14, 15, 16])), SERVE_SIZES[norb][1])
             U = float(rng.uniform(0.5, 3.0))
             V = float(rng.uniform(-2.0, 2.0))
-            ni, nt = shapes[norb]
+            (ni, nt), _ = SERVE_SIZES[norb]
             H = helix_H(ni, nt)
             n = H.shape[0]
             GL = np.zeros((n, n), complex)
@@ -308,14 +317,26 @@ def serve_bench(args):
         t0 = time.perf_counter()
         res = srv.run(reqs)
         wall = time.perf_counter() - t0
+        srv.shutdown()
         lat = np.array([r.latency for r in res])
         ok = [r for r in res if r.converged and not r.error]
-        out["rates"][str(rate)] = {"bias_points_per_s": len(ok) / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+        errs = [r.error for r in res if r.error]
+        by_size = {}
+        for r, q in zip(sorted(res, key=lambda r: r.rid), reqs):
+            n = q.problem.H.shape[0]
+            by_size.setdefault(n, []).append(r)
+        per_size = {str(n): {"requests": len(v), "converged": sum(r.converged for r in v),
+                             "median_steps": float(np.median([r.n_iter for r in v])),
+                             "median_service_ms": 1e3 * float(np.median([r.done - r.start for r in v])),
+                             "median_latency_ms": 1e3 * float(np.median([r.latency for r in v]))}
+                    for n, v in sorted(by_size.items())}
+        out["rates"][str(rate)] = {"per_size": per_size, "bias_points_per_s": len(ok) / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
                                    "p99_ms": 1e3 * float(np.percentile(lat, 99)), "converged": len(ok),
-                                   "requests": len(res), "wall_s": wall}
-        print("bench: serve rate %g/s: %d/%d converged, %.2f points/s, p50 %.0f ms, p99 %.0f ms" %
-              (rate, len(ok), len(res), len(ok) / wall, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99)),
-              file=sys.stderr, flush=True)
+                                   "requests": len(res), "wall_s": wall, "handles_reused": srv.reused,
+                                   "errors": errs[:3]}
+        print("bench: serve rate %g/s: %d/%d converged, %.2f points/s, p50 %.0f ms, p99 %.0f ms, %d handles reused"
+              % (rate, len(ok), len(res), len(ok) / wall, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99),
+                 srv.reused), file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -444,6 +465,7 @@ def make_parser():
     ap.add_argument("--serve", default="", help="comma-separated Poisson arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-sizes", default="16,48,80,160,240", help="spin-orbital counts of the serving stream")
     ap.add_argument("--maxscf", type=int, default=0,
                     help="cap on SCF steps per bias point (counter-collection passes; 0 = the workload's 300)")
     ap.add_argument("--sancho", default="", choices=[""] + list(SANCHO_WORKLOADS),
