@@ -523,8 +523,11 @@ int qs_dyson_max_norb() { return 88; }
 
 bool qs_dyson_shape_ok(int norb, int shape) {
     // k_dyson<16, 5, 5> (WG256 at 65..80 orbitals) is switched off: it returned
-    // wrong init2 values at 80 orbitals and a GPU memory fault followed, with no
-    // cause found in its source; the other shapes cover these sizes
+    // wrong init2 values at 80 orbitals and a GPU memory fault followed.  Its
+    // source runs clean and correct in CPU builds under AddressSanitizer and
+    // ThreadSanitizer (tests/test_dyson_emu_cpu.py), so no cause is known; it is
+    // also the only shape the compiler gives 256 VGPRs + 94 AGPRs.  The other
+    // shapes cover these sizes.
     if (shape == QS_DYSON_WG256) return norb <= qs_dyson_max_norb() && (norb + 15) / 16 != 5;
     if (shape == QS_DYSON_WG512) return norb > 16 && norb <= qs_dyson_max_norb();
     if (shape == QS_DYSON_WAVE) return norb <= qs_dyson_wave_max_norb();

@@ -1,0 +1,210 @@
+// CPU build of the library's HIP kernels, for bounds and barrier checks under
+// the host sanitizers (test infrastructure, never part of the library).
+//
+// Every GPU thread of a workgroup is a host thread.  __syncthreads() is a
+// barrier over the workgroup's threads, and the wavefront operations
+// (__shfl_xor, readfirstlane) exchange values through a per-wavefront buffer
+// between two barriers over its 64 lanes, so every access the kernel makes is
+// ordered exactly as the kernel's own barriers order it.  ThreadSanitizer then
+// reports any pair of accesses of different threads that no barrier orders
+// (an LDS race), and AddressSanitizer any access past the dynamic LDS size or
+// a global buffer (the LDS of a workgroup is a heap block of exactly the
+// launch's size).  LDS starts filled with NaN bytes, so a read of LDS that
+// nothing wrote shows up in the results.  Workgroups run one after another.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__
+#define __align__(n) alignas(n)
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct double2 {
+    double x, y;
+};
+struct float2 {
+    float x, y;
+};
+inline double2 make_double2(double a, double b) { return double2{a, b}; }
+inline float2 make_float2(float a, float b) { return float2{a, b}; }
+
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorLaunchFailure = 719 };
+typedef void *hipStream_t;
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
+inline hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
+
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline long long min(long long a, long long b) { return a < b ? a : b; }
+inline long long max(long long a, long long b) { return a > b ? a : b; }
+inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+inline unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
+
+inline thread_local dim3 emu_tid, emu_bid, emu_bdim, emu_gdim;
+
+namespace emu {
+
+// generation barrier on atomics (acquire/release, which ThreadSanitizer models
+// exactly); a wait longer than 20 s means the threads do not all reach the same
+// barrier (a divergent __syncthreads), and aborts
+struct Barrier {
+    std::atomic<int> count{0};
+    std::atomic<unsigned> gen{0};
+    int n = 0;
+    void wait() {
+        const unsigned g = gen.load(std::memory_order_acquire);
+        if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+            count.store(0, std::memory_order_relaxed);
+            gen.fetch_add(1, std::memory_order_acq_rel);
+            return;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        while (gen.load(std::memory_order_acquire) == g) {
+            std::this_thread::yield();
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+                fprintf(stderr, "emu: barrier timeout (threads do not all reach one barrier)\n");
+                abort();
+            }
+        }
+    }
+};
+
+struct Wave {
+    Barrier bar;
+    uint64_t buf[64];
+};
+
+struct Block {
+    Barrier bar;
+    std::vector<Wave> waves;
+    char *lds = nullptr;
+};
+
+inline thread_local Block *cur = nullptr;
+inline hipError_t last_error = hipSuccess;
+
+inline char *lds() { return cur->lds; }
+
+template <class T>
+T exchange(T v, int src_of_lane_xor, bool first) {
+    static_assert(sizeof(T) <= 8, "wave exchange of at most 8 bytes");
+    const int t = (int)emu_tid.x, lane = t & 63;
+    Wave &w = cur->waves[t >> 6];
+    uint64_t b = 0;
+    memcpy(&b, &v, sizeof(T));
+    w.buf[lane] = b;
+    w.bar.wait();
+    const uint64_t r = w.buf[first ? 0 : (lane ^ src_of_lane_xor)];
+    w.bar.wait();
+    T out;
+    memcpy(&out, &r, sizeof(T));
+    return out;
+}
+
+template <class T>
+T exchange_from(T v, int src) {
+    const int t = (int)emu_tid.x;
+    return exchange(v, (t & 63) ^ (src & 63), false);
+}
+
+}  // namespace emu
+
+#define threadIdx emu_tid
+#define blockIdx emu_bid
+#define blockDim emu_bdim
+#define gridDim emu_gdim
+
+inline void __syncthreads() { emu::cur->bar.wait(); }
+template <class T>
+inline T __shfl_xor(T v, int o, int width = 64) {
+    (void)width;
+    return emu::exchange(v, o, false);
+}
+#define __builtin_amdgcn_readfirstlane(v) emu::exchange((v), 0, true)
+// readlane(v, l): lane l's v (l uniform); as an exchange with lane (lane ^ lane ^ l)
+#define __builtin_amdgcn_readlane(v, l) emu::exchange_from((v), (l))
+// permlane16/32_swap(x, x): x of this lane and of lane ^ 16 (32).  The kernels
+// use only the unordered pair, so the emulation returns {own, partner}.
+namespace emu {
+struct U2 {
+    unsigned v[2];
+    unsigned operator[](int i) const { return v[i]; }
+};
+}  // namespace emu
+#define __builtin_amdgcn_permlane16_swap(a, b, fi, bc) emu::U2{{(unsigned)(a), emu::exchange((unsigned)(b), 16, false)}}
+#define __builtin_amdgcn_permlane32_swap(a, b, fi, bc) emu::U2{{(unsigned)(a), emu::exchange((unsigned)(b), 32, false)}}
+inline int __double2loint(double d) {
+    uint64_t u;
+    memcpy(&u, &d, 8);
+    return (int)(uint32_t)u;
+}
+inline int __double2hiint(double d) {
+    uint64_t u;
+    memcpy(&u, &d, 8);
+    return (int)(uint32_t)(u >> 32);
+}
+inline double __hiloint2double(int hi, int lo) {
+    const uint64_t u = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    double d;
+    memcpy(&d, &u, 8);
+    return d;
+}
+#define __builtin_amdgcn_s_setprio(p) ((void)0)
+// the barrier that follows orders global memory as well (host threads share it)
+inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline int atomicOr(int *p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline hipError_t hipGetLastError() {
+    const hipError_t e = emu::last_error;
+    emu::last_error = hipSuccess;
+    return e;
+}
+
+namespace emu {
+template <class... P, class... A>
+hipError_t launch(void (*k)(P...), dim3 grid, dim3 block, size_t lds_bytes, hipStream_t, A... args) {
+    const int nt = (int)(block.x * block.y * block.z);
+    if (nt % 64 || lds_bytes > 160 * 1024) return last_error = hipErrorInvalidValue;
+    for (unsigned bx = 0; bx < grid.x; bx++) {
+        Block blk;
+        blk.bar.n = nt;
+        blk.waves = std::vector<Wave>(nt / 64);
+        for (auto &w : blk.waves) w.bar.n = 64;
+        const size_t sz = (lds_bytes + 15) & ~(size_t)15;
+        blk.lds = (char *)aligned_alloc(16, sz ? sz : 16);
+        memset(blk.lds, 0xff, sz);
+        std::vector<std::thread> th;
+        th.reserve(nt);
+        for (int t = 0; t < nt; t++)
+            th.emplace_back([&, t] {
+                cur = &blk;
+                emu_tid = dim3(t);
+                emu_bid = dim3(bx);
+                emu_bdim = block;
+                emu_gdim = grid;
+                k(args...);
+            });
+        for (auto &t : th) t.join();
+        free(blk.lds);
+    }
+    return hipSuccess;
+}
+}  // namespace emu
+
+#define hipLaunchKernelGGL(k, g, b, lds, s, ...) emu::launch(k, dim3(g), dim3(b), (size_t)(lds), s, __VA_ARGS__)

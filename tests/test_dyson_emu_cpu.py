@@ -1,0 +1,202 @@
+"""CPU builds of the register-resident Dyson/Keldysh kernels under the host
+sanitizers (no GPU).
+
+`tests/emu/hip/hip_runtime.h` runs every GPU thread of a workgroup as a host
+thread, with __syncthreads and the wavefront exchanges as barriers, so the
+kernel sources of quichotte-and-sancho_amd/csrc compile with g++ unchanged
+except for their dynamic-LDS declaration.  AddressSanitizer then checks every
+LDS and global access against the launch's exact LDS size and the buffers'
+sizes, ThreadSanitizer checks that every pair of LDS accesses of different
+threads is ordered by a barrier, and LDS starts as NaN bytes so a read of
+anything unwritten shows in the outputs.  The outputs are compared with numpy
+(init2: G^r and G^< of one energy, src/scf.f90:315-355) and between launch
+shapes, which compute the same products in the same order.
+
+k_dyson<16, 5, 5> (256 threads at 65..80 orbitals) is the shape switched off
+in the library after a wrong result and a memory fault on the GPU; these
+builds are how its source was checked for a race or an out-of-range access.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "quichotte-and-sancho_amd", "csrc")
+EMU = os.path.join(ROOT, "tests", "emu")
+LDS_DECL = "extern __shared__ __align__(16) char lds_raw[];"
+
+M_INIT2, M_SCF2, M_FINAL = 1, 2, 4
+KB = 8.617333262e-5
+
+pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ is needed for the CPU builds")
+
+
+@pytest.fixture(scope="module")
+def builds(tmp_path_factory):
+    """{name: executable}: k_dyson and k_dyson_w, each under ASan and TSan."""
+    d = tmp_path_factory.mktemp("emu")
+    srcs = {}
+    for name, fn in (("k_dyson", "qs_dyson.hip"), ("k_dyson_w", "qs_dyson_wave.hip")):
+        text = open(os.path.join(CSRC, fn)).read()
+        assert text.count(LDS_DECL) == 1
+        p = d / ("%s_cpu.hip" % name)
+        p.write_text(text.replace(LDS_DECL, "char *lds_raw = emu::lds();"))
+        srcs[name] = str(p)
+    procs, exes = [], {}
+    for name, src in srcs.items():
+        for san in ("address", "thread"):
+            exe = str(d / ("%s_%s" % (name, san)))
+            cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=" + san, "-fno-omit-frame-pointer",
+                   "-I", EMU, "-I", CSRC, '-DQS_DYSON_SOURCE="%s"' % src, os.path.join(EMU, "dyson_emu.cpp"),
+                   "-o", exe, "-pthread"]
+            if name == "k_dyson_w":
+                cmd.insert(1, "-DQS_EMU_WAVE")
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+            exes[(name, san)] = exe
+    for cmd, p in procs:
+        out, _ = p.communicate(timeout=600)
+        assert p.returncode == 0, "CPU build failed: %s\n%s" % (" ".join(cmd), out[-4000:])
+    return exes, d
+
+
+def make_problem(path, n, count, mode, use_sigma, seed=1, V=0.5):
+    """A dense random system with every orbital active and on the support."""
+    rng = np.random.default_rng(seed)
+    H = rng.uniform(-1, 1, (n, n)) + 0.1j * rng.uniform(-1, 1, (n, n))
+    H = (H + H.conj().T) / 2
+    GL = np.zeros((n, n), complex)
+    GR = np.zeros((n, n), complex)
+    GL[0, 0] = GL[1, 1] = 0.1
+    GL[0, 1] = GL[1, 0] = 0.03
+    GR[n - 2, n - 2] = GR[n - 1, n - 1] = 0.1
+    Hemb = -H + 0.5j * (GL + GR)
+    ns = n // 2
+    sig_ld = (12 * (ns * (ns + 1) // 2) + 7) // 8 * 8
+    P = dict(n=n, Hemb=Hemb, GL=GL, GR=GR, w_init=-2.0, delta=0.37, mu=0.0, beta=1.0 / (KB * 300.0), V=V)
+    with open(path, "wb") as f:
+        np.array([n, count, mode, use_sigma, 2], np.int32).tofile(f)
+        np.array([P["w_init"], P["delta"], P["mu"], P["beta"], V]).tofile(f)
+        for a in (Hemb, GL, GR):
+            np.ascontiguousarray(a, dtype=np.complex128).tofile(f)
+        np.ascontiguousarray(GL[:2, :2], dtype=np.complex128).tofile(f)
+        np.array([0, 1], np.int32).tofile(f)
+        (0.1 * (rng.uniform(-1, 1, 4 * ns) + 1j * rng.uniform(-1, 1, 4 * ns))).tofile(f)
+        if use_sigma:
+            (0.05 * (rng.standard_normal(count * sig_ld) + 1j * rng.standard_normal(count * sig_ld))).tofile(f)
+        if mode == M_SCF2:
+            rng.standard_normal(count * 3 * n * n).tofile(f)
+            rng.standard_normal(count * 3 * n * n).tofile(f)
+    return P
+
+
+def read_out(path, n, count):
+    raw = np.fromfile(path, np.uint8)
+    out, off = {}, 0
+    for name, dt, k in (("x", np.float64, count * 3 * n * n), ("partials", np.float64, count * 4),
+                        ("diagR", np.float64, count * n), ("diagL", np.float64, count * n),
+                        ("onsL", np.complex128, count * 4 * (n // 2)), ("dosd", np.float64, count * n),
+                        ("trUD", np.float64, count * 2), ("status", np.int32, 1)):
+        nb = np.dtype(dt).itemsize * k
+        out[name] = raw[off:off + nb].view(dt)
+        off += nb
+    assert off == raw.size
+    return out
+
+
+def run(exe, prob, outp, shape):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([exe, str(prob), str(outp)] + list(shape), capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0 and "Sanitizer" not in r.stderr, "%s %s: rc %d\n%s" % (
+        os.path.basename(exe), shape, r.returncode, r.stderr[-6000:])
+
+
+def init2_reference(P, count):
+    """Packed [Re G^r | Im G^r | Re G^<(i<j) | Im G^<(i<=j)] per energy (common/linear_algebra.f90:142-191)."""
+    n = P["n"]
+    xs = []
+    for iw in range(count):
+        w = P["w_init"] + (iw + 1) * P["delta"]
+        f = lambda V: 1.0 / (np.exp(min((w - P["mu"] + V) * P["beta"], 700.0)) + 1.0)
+        G = np.linalg.inv(w * np.eye(n) + P["Hemb"])
+        L = G @ (1j * (f(P["V"]) * P["GL"] + f(0.0) * P["GR"])) @ G.conj().T
+        xs.append(np.concatenate([G.real.ravel(), G.imag.ravel(), L.real[np.triu_indices(n, 1)],
+                                  L.imag[np.triu_indices(n)]]))
+    return np.concatenate(xs)
+
+
+def written(o, mode):
+    """the outputs each mode writes (the rest keep their NaN fill)"""
+    if mode == M_INIT2:
+        return ["x"]
+    if mode == M_SCF2:
+        return ["x", "partials", "diagR", "diagL"]
+    return ["onsL", "dosd", "trUD"]
+
+
+def compare(a, b, names, rtol):
+    for k in names:
+        x, y = a[k], b[k]
+        assert not np.isnan(x).any() and not np.isnan(y).any(), k
+        scale = max(np.max(np.abs(x)), 1e-300)
+        assert np.max(np.abs(x - y)) <= rtol * scale, (k, np.max(np.abs(x - y)) / scale)
+
+
+@pytest.mark.parametrize("n,shapes", [(80, [("16", "5", "5"), ("32", "3", "5")]),
+                                      (88, [("16", "6", "6"), ("32", "3", "6")]),
+                                      (20, [("16", "2", "2")])])
+@pytest.mark.parametrize("mode", [M_INIT2, M_SCF2, M_FINAL])
+def test_k_dyson_shapes_address_sanitizer(builds, tmp_path, n, shapes, mode):
+    exes, _ = builds
+    count = 2
+    P = make_problem(tmp_path / "p.bin", n, count, mode, use_sigma=int(mode != M_INIT2))
+    outs = []
+    for sh in shapes:
+        run(exes[("k_dyson", "address")], tmp_path / "p.bin", tmp_path / "o.bin", sh)
+        outs.append(read_out(tmp_path / "o.bin", n, count))
+    if mode == M_INIT2:
+        ref = init2_reference(P, count)
+        for o in outs:
+            assert np.max(np.abs(o["x"] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # the same products in the same order: equal bits, except the block
+    # reductions of |F|^2 and |F - Fold|^2, whose order follows the thread count
+    for o in outs[1:]:
+        names = written(o, mode)
+        compare(outs[0], o, [k for k in names if k != "partials"], 0.0)
+        if "partials" in names:
+            compare(outs[0], o, ["partials"], 1e-13)
+
+
+@pytest.mark.parametrize("n", [16, 44, 80])
+@pytest.mark.parametrize("mode", [M_INIT2, M_SCF2, M_FINAL])
+def test_small_lds_kernel_address_sanitizer(builds, tmp_path, n, mode):
+    """k_dyson_w (8x8 tile at 16 orbitals, 16x16 above) against numpy and k_dyson."""
+    exes, _ = builds
+    count = 2
+    P = make_problem(tmp_path / "p.bin", n, count, mode, use_sigma=int(mode != M_INIT2), seed=n)
+    run(exes[("k_dyson_w", "address")], tmp_path / "p.bin", tmp_path / "w.bin", ("wave", "0", "0"))
+    w = read_out(tmp_path / "w.bin", n, count)
+    for k in written(w, mode):
+        assert not np.isnan(w[k]).any(), k
+    if n == 80:
+        run(exes[("k_dyson", "address")], tmp_path / "p.bin", tmp_path / "v.bin", ("32", "3", "5"))
+        compare(w, read_out(tmp_path / "v.bin", n, count), written(w, mode), 1e-12)
+    if mode == M_INIT2:
+        ref = init2_reference(P, count)
+        assert np.max(np.abs(w["x"] - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("kernel,n,shape", [("k_dyson", 80, ("16", "5", "5")), ("k_dyson", 80, ("32", "3", "5")),
+                                            ("k_dyson_w", 80, ("wave", "0", "0")),
+                                            ("k_dyson_w", 16, ("wave", "0", "0"))])
+@pytest.mark.parametrize("mode", [M_SCF2, M_FINAL])
+def test_no_unordered_lds_access_thread_sanitizer(builds, tmp_path, kernel, n, shape, mode):
+    exes, _ = builds
+    make_problem(tmp_path / "p.bin", n, 1, mode, use_sigma=1)
+    run(exes[(kernel, "thread")], tmp_path / "p.bin", tmp_path / "o.bin", shape)
+    o = read_out(tmp_path / "o.bin", n, 1)
+    for k in written(o, mode):
+        assert not np.isnan(o[k]).any(), k
