@@ -497,6 +497,31 @@ static void soc_block(Mat &H, double lamb, const double *r, const double *r1, co
     }
 }
 
+// A non-magnetic H is time-reversal invariant: every 2x2 spin block h0*1 + h.sigma
+// has h0 real and h purely imaginary.  Im h0 and Re h act like random magnetic
+// fields, so the report puts them beside the spin-orbit part Im h
+// (src/hamiltonian.f90:477-504).
+static void report_time_reversal(Mat &H, int N_sites) {
+    const zd I(0, 1);
+    double odd0 = 0, oddh = 0, soc = 0;
+    for (int j = 1; j <= N_sites; j++)
+        for (int i = 1; i <= N_sites; i++) {
+            const zd b11 = H1(H, 2 * i - 1, 2 * j - 1), b12 = H1(H, 2 * i - 1, 2 * j);
+            const zd b21 = H1(H, 2 * i, 2 * j - 1), b22 = H1(H, 2 * i, 2 * j);
+            const zd h0 = 0.5 * (b11 + b22), hz = 0.5 * (b11 - b22);
+            const zd hx = 0.5 * (b12 + b21), hy = 0.5 * I * (b12 - b21);
+            odd0 = std::max(odd0, std::abs(h0.imag()));
+            oddh = std::max({oddh, std::abs(hx.real()), std::abs(hy.real()), std::abs(hz.real())});
+            soc = std::max({soc, std::abs(hx.imag()), std::abs(hy.imag()), std::abs(hz.imag())});
+        }
+    printf(" Time reversal (all spin blocks h0*1 + h.sigma):\n");
+    printf("   max |Im h0| =%10.2E eV   (should be 0)\n", odd0);
+    printf("   max |Re h|  =%10.2E eV   (should be 0: Zeeman-like)\n", oddh);
+    printf("   max |Im h|  =%10.2E eV   (spin-orbit coupling)\n", soc);
+    if (std::max(odd0, oddh) > 1e-8)
+        printf(" WARNING: H breaks time reversal by more than 1e-8 eV; spin polarisation of that size is not physical\n");
+}
+
 static void chain_from_positions(Mat &H, const Params &P, const std::vector<std::array<double, 3>> &rs) {
     const int nat = (int)rs.size();
     for (int i = 1; i <= nat; i++) {
@@ -607,6 +632,18 @@ static Mat build_peptide(const Params &P) {
         for (auto &r : rs) fprintf(g, "C %14.8f%14.8f%14.8f\n", r[0], r[1], r[2]);
         fclose(g);
     }
+    // geometry summary and hopping table (src/hamiltonian.f90:260-270)
+    printf(" Peptide helix: twist%7.2f deg, rise%6.3f A, radius%6.3f A, hand%5.1f, SOC %s\n", P.pep_twist, P.pep_rise,
+           P.hel_radius, P.hand, P.soc_model.c_str());
+    printf(" Dipole ramp:%9.5f eV  (eps_1 =%9.5f, eps_N =%9.5f eV)\n", P.dipole_ramp, H1(H, 1, 1).real(),
+           H1(H, P.Norb, P.Norb).real());
+    printf("    j    l_j [A]    t_j/t_hop     (hop_decay =%6.3f A)\n", P.hop_decay);
+    for (int j = 1; j <= jmax; j++) {
+        double d[3] = {rs[j][0] - rs[0][0], rs[j][1] - rs[0][1], rs[j][2] - rs[0][2]};
+        const double l = norm3(d);
+        printf("%5d%11.4f%14.4E\n", j, l, std::exp(-(l - l1) / P.hop_decay));
+    }
+    report_time_reversal(H, nat);
     return H;
 }
 
@@ -648,6 +685,7 @@ static Mat build_dftb(const Params &P) {
         }
     printf(" DFTB Hamiltonian: %d blocks, %d atoms, %d orbitals\n", nblk, P.N_atoms, P.N_sites);
     printf(" max |H - H^dagger| in file = %10.2E eV (H symmetrised)\n", herm);
+    report_time_reversal(Hs, P.N_sites);
     return Hs;
 }
 
@@ -772,6 +810,9 @@ static Mat build_dftb_hs(const Params &P) {
     for (int i = 1; i <= n; i++)
         for (int j = 1; j <= n; j++) H1(H, i, j) = 0.5 * (H1(Ht, i, j) + std::conj(H1(Ht, j, i)));
     printf(" DFTB H0/S Hamiltonian: %d atoms, %d orbitals, SOC form %s\n", P.N_atoms, N, P.soc_form.c_str());
+    for (size_t e = 0; e < P.elem_name.size(); e++)
+        printf("   element %-2s: l_max = %d, xi(p) = %12.4E eV\n", P.elem_name[e].c_str(), P.elem_lmax[e], P.elem_xi[e]);
+    report_time_reversal(H, P.N_sites);
     return H;
 }
 

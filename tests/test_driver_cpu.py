@@ -117,3 +117,75 @@ def test_dftb_hs_hamiltonian_matches_reference(tmp_path):
     assert_close_table(new, ref)
     # integer index columns identical
     assert [r[:2] for r in new] == [r[:2] for r in ref]
+
+
+def _spin_block_report(H):
+    """max |Im h0|, max |Re h|, max |Im h| over the 2x2 spin blocks h0*1 + h.sigma of H."""
+    n = H.shape[0] // 2
+    B = H.reshape(n, 2, n, 2).transpose(0, 2, 1, 3)
+    h0 = 0.5 * (B[..., 0, 0] + B[..., 1, 1])
+    hz = 0.5 * (B[..., 0, 0] - B[..., 1, 1])
+    hx = 0.5 * (B[..., 0, 1] + B[..., 1, 0])
+    hy = 0.5j * (B[..., 0, 1] - B[..., 1, 0])
+    import numpy as np
+    return (np.max(np.abs(h0.imag)), max(np.max(np.abs(h.real)) for h in (hx, hy, hz)),
+            max(np.max(np.abs(h.imag)) for h in (hx, hy, hz)))
+
+
+def _report_values(stdout):
+    vals = {}
+    for line in stdout.splitlines():
+        for key in ("max |Im h0|", "max |Re h|", "max |Im h|"):
+            if line.strip().startswith(key):
+                vals[key] = float(line.split("=")[1].split()[0])
+    return vals
+
+
+def test_dftb_hs_time_reversal_report(tmp_path):
+    """The time-reversal report of src/hamiltonian.f90:477-504 after the dftb_hs
+    build: the three maxima against the spin blocks of the reference's own
+    hamiltonian_matrix.dat, printed to two decimals (es10.2)."""
+    import numpy as np
+    from qs_testlib import read_table
+    src = os.path.join(GOLDEN, "cases", "order2_dftb_hs")
+    for f in os.listdir(src):
+        p = os.path.join(src, f)
+        if os.path.isfile(p):
+            shutil.copy(p, tmp_path / f)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert " Time reversal (all spin blocks h0*1 + h.sigma):" in r.stdout
+    assert "   element C : l_max = 1, xi(p) =   8.4900E-03 eV" in r.stdout
+    ref = read_table(os.path.join(src, "reference", "hamiltonian_matrix.dat"))
+    n = int(max(max(row[0], row[1]) for row in ref))
+    H = np.zeros((n, n), complex)
+    for i, j, re, im in ref:
+        H[int(i) - 1, int(j) - 1] = re + 1j * im
+    low = np.tril(H, -1)
+    H = np.diag(np.diag(H)) + low + low.conj().T if not np.triu(H, 1).any() else H
+    odd0, oddh, soc = _spin_block_report(H)
+    got = _report_values(r.stdout)
+    assert soc > 1e-4 and abs(got["max |Im h|"] - soc) <= 0.006 * soc
+    # a non-magnetic H: both odd parts at round-off, and no warning
+    assert got["max |Im h0|"] < 1e-12 and got["max |Re h|"] < 1e-12 and odd0 < 1e-12 and oddh < 1e-12
+    assert "WARNING: H breaks time reversal" not in r.stdout
+
+
+def test_peptide_report(tmp_path):
+    """Peptide builder's summary (src/hamiltonian.f90:260-271): the hopping table's
+    bond lengths from the helix geometry, and the time-reversal report."""
+    import numpy as np
+    text = BASE.replace("N_ions = 4\n", "").replace("N_turns = 1\n", "") + (
+        "hamiltonian = peptide\nN_sites = 12\npep_twist = 100\npep_rise = 1.5\nhop_range = 3\nhop_decay = 0.7\n"
+        "soc_model = radial\ndipole_ramp = 0.2\n")
+    r = run_in(tmp_path, {"input.dat": text})
+    assert " Peptide helix: twist 100.00 deg, rise 1.500 A, radius 1.000 A, hand  1.0, SOC radial" in r.stdout, \
+        r.stdout + r.stderr
+    rows = [l.split() for l in r.stdout.splitlines() if len(l.split()) == 3 and l.split()[0].isdigit()]
+    assert [int(x[0]) for x in rows] == [1, 2, 3]
+    phi = np.arange(1, 13) * np.radians(100.0)
+    rs = np.stack([np.cos(phi), np.sin(phi), 1.5 * np.arange(12)], 1)
+    l = np.linalg.norm(rs[1:4] - rs[0], axis=1)
+    assert np.allclose([float(x[1]) for x in rows], l, atol=6e-5)
+    assert np.allclose([float(x[2]) for x in rows], np.exp(-(l - l[0]) / 0.7), rtol=6e-5)
+    got = _report_values(r.stdout)
+    assert got["max |Im h0|"] == 0.0 and got["max |Re h|"] == 0.0 and got["max |Im h|"] > 0.0
