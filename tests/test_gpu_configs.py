@@ -163,3 +163,70 @@ def test_helix80_full_size_solve_invariants():
     print("helix80 2^16: V=-1 %d steps (fresh), V=-0.9 %d steps (continued), J = %s, %s" %
           (r0.n_iter, r1.n_iter, J0, J1))
     g.close()
+
+
+def dftb240(n_omega, **kw):
+    """The DFTB-like system of the 240-orbital configuration (quichotte-and-sancho_amd/dftb_synth.py,
+    seed 0): 30 carbon atoms with s, p_y, p_z, p_x on a 3 A-radius, 3.5 A-pitch helix, dual SOC
+    xi_p = 0.00849 eV, U = 1.5 eV on every spatial orbital, Gamma = 0.2 eV on the first and last
+    atom's p_z, mu at the half-filling gap, on a reduced grid of n_omega energies over the spectrum."""
+    import importlib
+    ds = importlib.import_module("quichotte-and-sancho_amd.dftb_synth")
+    from qs_testlib import Problem
+    d = ds.problem(n_atoms=30, seed=0, n_omega=n_omega)
+    return Problem(d["H"], d["gammaL"], d["gammaR"], d["U_site"], 2, 300.0, d["mu"], 0, 0, d["delta"], [0.0],
+                   n_omega=n_omega, w_init=d["w_init"], nhist=16, single_history=True, **kw)
+
+
+def test_dftb240_system_seams_match_oracle():
+    """init2, residual2 and Sigma2 of the 240-orbital DFTB-like system at 64 energies, 1e-11 relative."""
+    prob = dftb240(64)
+    assert prob.norb == 240
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    V = 0.3
+    x = o.init2(V)
+    xg = g.init2(V)
+    rel = lambda a, b: np.abs(np.asarray(a) - b).max() / np.abs(b).max()
+    assert rel(xg, x) < 1e-11
+    F0, _ = o.residual2(V, x)
+    x1 = x + 0.5 * F0
+    Fo, abo = o.residual2(V, x1)
+    Fg, abg = g.residual2(V, x1)
+    assert abo == abg and rel(Fg, Fo) < 1e-11
+    sro, slo = o.sigma2()
+    srg, slg, _ = g.sigma()
+    assert rel(srg, sro) < 1e-11 and rel(slg, slo) < 1e-11
+
+
+def test_dftb240_system_runaway_matches_oracle():
+    """The system as specified couples the leads to the end atoms' p_z only, which
+    leaves resonances down to 1e-8 eV wide (widths of H - i(GamL + GamR)/2): no
+    grid that fits one GPU resolves them, the first residual's occupations leave
+    [-1, 2], and the solve aborts as a runaway (check_diag_sanity,
+    common/scf_checks.f90:59-91; scf_residual2, src/scf.f90:180-186).  At 128
+    energies both sides abort at the first residual with the same worst state,
+    its occupation and spectral weight at 1e-10 relative."""
+    import ctypes
+    from qs_testlib import oracle_lib, ptr
+    prob = dftb240(128)
+    o = Oracle(prob)
+    g = gpu_engine(prob)
+    V = 0.3
+    Fo, abo = o.residual2(V, o.init2(V))
+    assert abo and not Fo.any()
+    r, l = o.diag()
+    out = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()]
+    labels = np.arange(1, o.nA + 1, dtype=np.int32)
+    oracle_lib().orc_check_diag_sanity(ptr(r), ptr(l), o.nA, prob.n_omega, prob.delta,
+                                       *[ctypes.byref(v) for v in out], ptr(labels))
+    ra, un, io, no, wo = [v.value for v in out]
+    assert ra and un
+    ro = o.solve(V, True)
+    rg = g.solve_scf(V, True)
+    assert not ro["converged"] and ro["n_iter"] == 1
+    assert rg.runaway and not rg.converged and rg.n_iter == 1
+    print("dftb240 runaway: gpu state %d n %.12g w %.12g, oracle state %d n %.12g w %.12g"
+          % (rg.i_worst, rg.n_worst, rg.w_worst, io, no, wo))
+    assert rg.i_worst == io
+    assert abs(rg.n_worst - no) <= 1e-10 * abs(no) and abs(rg.w_worst - wo) <= 1e-10 * abs(wo)
