@@ -382,12 +382,17 @@ int setup_device(qs_handle *h) {
         h->dev_bytes += (int64_t)bytes;
     }
     if ((rc = dalloc(h, &h->d_red, 64))) return rc;
-    // column-sum scratch: 64 row-chunks x the widest column set
+    // column-sum scratch: the row chunks of the widest column set over the grid, and
+    // behind the Broyden partials the chunks of their own column sums
     int widest = std::max({8 * h->nsites, n, 2 * std::max(nA, 1), 2 * std::max(h->atom_tot, 1), 2});
-    size_t need = (size_t)64 * widest;
-    size_t hist_part = (size_t)qs_vec_blocks(std::max<int64_t>(h->n_vec, 1)) * 2 * MAXH_DEV;
+    const int hist_blocks = qs_vec_blocks(std::max<int64_t>(h->n_vec, 1));
+    size_t need = (size_t)qs_colsum_chunks(widest, nw, nullptr) * widest;
+    for (int c : {8 * h->nsites, n, std::max(nA, 1), 2 * std::max(h->atom_tot, 1), 2})
+        need = std::max(need, (size_t)qs_colsum_chunks(c, nw, nullptr) * c);
+    size_t hist_part = (size_t)hist_blocks * 2 * MAXH_DEV;
+    size_t hist_sums = (size_t)qs_colsum_chunks(2 * MAXH_DEV, hist_blocks, nullptr) * 2 * MAXH_DEV;
     size_t et_part = (size_t)256 * 4 * std::max(h->nAs, 1);
-    h->scratch_doubles = std::max({need, hist_part + 64 * 2 * MAXH_DEV, et_part, (size_t)1024});
+    h->scratch_doubles = std::max({need, hist_part + hist_sums, et_part, (size_t)1024});
     if ((rc = dalloc(h, &h->d_scratch, h->scratch_doubles))) return rc;
     if ((rc = dalloc(h, &h->d_sums, (size_t)std::max(widest, 64)))) return rc;
 

@@ -151,7 +151,10 @@ hipError_t qs_launch_sigma2(const Sigma2Plan &p, const double2 *Gt, double2 *Z, 
 hipError_t qs_launch_equal_time_sigma1(const double *x, const int *srcidx, const int *act_site,
                                        const double *U_site_act, double2 *gt0, double2 *sig1, int nA, int nAs, int nw,
                                        double pp, double *part, hipStream_t s);
-// out[c] = sum_r in[r][c]; scratch >= 64 * ncols doubles
+// out[c] = sum_r in[r][c] (deterministic order); scratch >= qs_colsum_chunks(ncols, nrows) * ncols
+// doubles, at most QS_COLSUM_CHUNKS * ncols
+constexpr int QS_COLSUM_CHUNKS = 1024;
+int qs_colsum_chunks(int ncols, int nrows, int *rows_per);
 hipError_t qs_launch_reduce_cols(const double *in, int ncols, int nrows, double *out, double *scratch,
                                  hipStream_t s);
 hipError_t qs_launch_reduce_partials(const double *partials, int nblocks, double *out, hipStream_t s);

@@ -18,7 +18,6 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int MAXH = 32;     // largest Broyden history handled on the device
-constexpr int RCHUNKS = 64;  // row chunks of the column sums
 
 // ---- equal-time lesser + Sigma1 ------------------------------------------------
 // srcidx[4 ia + q]: offsets in an energy block of Im G<(i,i), Im G<(i+1,i+1),
@@ -44,25 +43,21 @@ __global__ __launch_bounds__(NT) void k_et_partial(const double *__restrict__ x,
     part[(size_t)chunk * nq + q] = (s0 + s1) + (s2 + s3);
 }
 
-__global__ void k_et_final(const double *__restrict__ part, int nq, int nch, const int *__restrict__ act_site,
-                           const double *__restrict__ U_site_act, double pp, double2 *__restrict__ gt0,
-                           double2 *__restrict__ sig1) {
-    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ia >= nq / 4) return;
+// one block per active site: its NT threads add the chunks' partials of the
+// site's four sums, then a block reduction (fixed order)
+__global__ __launch_bounds__(NT) void k_et_final(const double *__restrict__ part, int nq, int nch,
+                                                 const int *__restrict__ act_site,
+                                                 const double *__restrict__ U_site_act, double pp,
+                                                 double2 *__restrict__ gt0, double2 *__restrict__ sig1) {
+    __shared__ double scratch[16];
+    const int ia = blockIdx.x;
     double S[4];
     for (int q = 0; q < 4; q++) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int c = 0;
-        for (; c + 3 < nch; c += 4) {
-            const double *pp = part + (size_t)c * nq + 4 * ia + q;
-            s0 += pp[0];
-            s1 += pp[nq];
-            s2 += pp[2 * (size_t)nq];
-            s3 += pp[3 * (size_t)nq];
-        }
-        for (; c < nch; c++) s0 += part[(size_t)c * nq + 4 * ia + q];
-        S[q] = (s0 + s1) + (s2 + s3);
+        double v = 0.0;
+        for (int c = threadIdx.x; c < nch; c += NT) v += part[(size_t)c * nq + 4 * ia + q];
+        S[q] = block_sum(v, scratch);
     }
+    if (threadIdx.x != 0) return;
     // G<(t=0) block: g00 = i pp S0, g11 = i pp S1, g01 = pp (S2 + i S3), g10 = -conj(g01)
     double2 g[4];
     g[0] = make_double2(0.0, pp * S[0]);
@@ -83,22 +78,62 @@ __global__ void k_et_final(const double *__restrict__ part, int nq, int nch, con
 }
 
 // ---- column sums ---------------------------------------------------------------
+// Column sums of a [nrows][ncols] array in two launches.  Partial: block `chunk`
+// sums rows [chunk * rows_per, ...) of every column; with ncols <= NT its
+// threads are Q = NT / ncols row lanes of one column each (t = q * ncols + c),
+// so a wavefront's loads are contiguous rows, and four interleaved sums keep
+// four loads in flight.  Final: one block per column adds the chunks' partials.
+// The chunking depends only on the sizes, so the sums are deterministic.
 __global__ __launch_bounds__(NT) void k_colsum_partial(const double *__restrict__ in, int ncols, int nrows,
-                                                       double *__restrict__ part) {
-    const int c = blockIdx.x * NT + threadIdx.x, chunk = blockIdx.y;
-    if (c >= ncols) return;
-    const int per = (nrows + RCHUNKS - 1) / RCHUNKS, r0 = chunk * per, r1 = min(nrows, r0 + per);
-    double s = 0.0;
-    for (int r = r0; r < r1; r++) s += in[(size_t)r * ncols + c];
-    part[(size_t)chunk * ncols + c] = s;
+                                                       int rows_per, double *__restrict__ part) {
+    __shared__ double red[NT];
+    const int chunk = blockIdx.x, t = threadIdx.x;
+    const int r0 = chunk * rows_per, r1 = min(nrows, r0 + rows_per);
+    if (ncols <= NT) {
+        const int Q = NT / ncols, c = t % ncols, q = t / ncols;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (q < Q) {
+            int r = r0 + q;
+            for (; r + 3 * Q < r1; r += 4 * Q) {
+                const double *p = in + (size_t)r * ncols + c;
+                s0 += p[0];
+                s1 += p[(size_t)Q * ncols];
+                s2 += p[(size_t)2 * Q * ncols];
+                s3 += p[(size_t)3 * Q * ncols];
+            }
+            for (; r < r1; r += Q) s0 += in[(size_t)r * ncols + c];
+        }
+        red[t] = (s0 + s1) + (s2 + s3);
+        __syncthreads();
+        if (t < ncols) {
+            double acc = 0.0;
+            for (int k = 0; k < Q; k++) acc += red[k * ncols + t];
+            part[(size_t)chunk * ncols + t] = acc;
+        }
+    } else {
+        for (int c = t; c < ncols; c += NT) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int r = r0;
+            for (; r + 3 < r1; r += 4) {
+                const double *p = in + (size_t)r * ncols + c;
+                s0 += p[0];
+                s1 += p[ncols];
+                s2 += p[2 * (size_t)ncols];
+                s3 += p[3 * (size_t)ncols];
+            }
+            for (; r < r1; r++) s0 += in[(size_t)r * ncols + c];
+            part[(size_t)chunk * ncols + c] = (s0 + s1) + (s2 + s3);
+        }
+    }
 }
-__global__ __launch_bounds__(NT) void k_colsum_final(const double *__restrict__ part, int ncols,
+__global__ __launch_bounds__(NT) void k_colsum_final(const double *__restrict__ part, int ncols, int nchunks,
                                                      double *__restrict__ out) {
-    const int c = blockIdx.x * NT + threadIdx.x;
-    if (c >= ncols) return;
+    __shared__ double scratch[16];
+    const int c = blockIdx.x;
     double s = 0.0;
-    for (int k = 0; k < RCHUNKS; k++) s += part[(size_t)k * ncols + c];
-    out[c] = s;
+    for (int k = threadIdx.x; k < nchunks; k += NT) s += part[(size_t)k * ncols + c];
+    s = block_sum(s, scratch);
+    if (threadIdx.x == 0) out[c] = s;
 }
 
 // partials [nblocks][4] = (max, sum, sum, -) -> out[3]
@@ -348,19 +383,29 @@ int qs_vec_blocks(int64_t n) {
 hipError_t qs_launch_equal_time_sigma1(const double *x, const int *srcidx, const int *act_site,
                                        const double *U_site_act, double2 *gt0, double2 *sig1, int nA, int nAs, int nw,
                                        double pp, double *part, hipStream_t s) {
+    if (nAs <= 0) return hipSuccess;
     const int nq = 4 * nAs, nch = 256;
     const size_t nblk = 3 * (size_t)nA * nA;
     hipLaunchKernelGGL(k_et_partial, dim3((nq + NT - 1) / NT, nch), dim3(NT), 0, s, x, srcidx, nq, nw, nblk, part);
-    hipLaunchKernelGGL(k_et_final, dim3((nAs + 63) / 64), dim3(64), 0, s, part, nq, nch, act_site, U_site_act, pp, gt0,
-                       sig1);
+    hipLaunchKernelGGL(k_et_final, dim3(nAs), dim3(NT), 0, s, part, nq, nch, act_site, U_site_act, pp, gt0, sig1);
     return hipGetLastError();
+}
+
+int qs_colsum_chunks(int ncols, int nrows, int *rows_per) {
+    // at least four rows per thread; at most QS_COLSUM_CHUNKS chunks
+    const int Q = ncols <= NT ? NT / ncols : 1;
+    int rp = std::max(4 * Q, (nrows + QS_COLSUM_CHUNKS - 1) / QS_COLSUM_CHUNKS);
+    if (rows_per) *rows_per = rp;
+    return std::max(1, (nrows + rp - 1) / rp);
 }
 
 hipError_t qs_launch_reduce_cols(const double *in, int ncols, int nrows, double *out, double *scratch,
                                  hipStream_t s) {
-    hipLaunchKernelGGL(k_colsum_partial, dim3((ncols + NT - 1) / NT, RCHUNKS), dim3(NT), 0, s, in, ncols, nrows,
-                       scratch);
-    hipLaunchKernelGGL(k_colsum_final, dim3((ncols + NT - 1) / NT), dim3(NT), 0, s, scratch, ncols, out);
+    if (ncols <= 0) return hipSuccess;
+    int rows_per = 0;
+    const int nch = qs_colsum_chunks(ncols, nrows, &rows_per);
+    hipLaunchKernelGGL(k_colsum_partial, dim3(nch), dim3(NT), 0, s, in, ncols, nrows, rows_per, scratch);
+    hipLaunchKernelGGL(k_colsum_final, dim3(ncols), dim3(NT), 0, s, scratch, ncols, nch, out);
     return hipGetLastError();
 }
 
