@@ -161,13 +161,15 @@ int qs_profile(qs_handle *h, int32_t enable);
 int qs_profile_read(qs_handle *h, double *ms, int64_t *count, int32_t nstage);
 const char *qs_stage_name(int32_t stage);
 /* launch shape of the Dyson kernel picked for this size at qs_create (0: 256
- * threads, 1: 512 threads per energy, 2: one 64-lane wavefront per energy) and
- * the timed ms of each on a slice of the grid (0 for a shape that does not
- * apply to this size); returns the number of shapes */
+ * threads, 1: 512 threads per energy with the matrix in registers and LDS, 2:
+ * 16x16 or 8x8 threads with a per-energy scratch slab, 3: the large-Norb chunked
+ * path on MFMA, above 96 orbitals) and the timed ms of each on a slice of the
+ * grid (0 for a shape that does not apply to this size); returns the number of
+ * shapes */
 int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax);
 /* a description of the launch shape the handle uses; *uses_mfma (may be NULL)
  * is 1 when its dense contractions issue MFMA instructions, 0 when they are
- * vector FMAs or library calls */
+ * FP64 vector FMAs */
 const char *qs_dyson_shape(const qs_handle *h, int32_t *uses_mfma);
 
 /* ---- SANCHO: interaction self-energies of orders 2 and 3 ----------------

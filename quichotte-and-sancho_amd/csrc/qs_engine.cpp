@@ -378,7 +378,7 @@ int setup_device(qs_handle *h) {
     if (n > qs_dyson_wave_max_norb() || (fv && atoi(fv) == QS_DYSON_BIG)) {
         size_t bytes = 0;
         h->big = qs_big_create(n, nw, kBigBudget, &bytes);
-        if (!h->big) return fail(h, QS_ERR_HIP, "large-Norb Dyson workspace (rocBLAS handle, chunk buffers)");
+        if (!h->big) return fail(h, QS_ERR_HIP, "large-Norb Dyson workspace (chunk buffers)");
         h->dev_bytes += (int64_t)bytes;
     }
     if ((rc = dalloc(h, &h->d_red, 64))) return rc;
