@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define QS_WAVE 64
 
 struct cplx { double x, y; };
@@ -73,6 +75,28 @@ __device__ __forceinline__ double block_max(double v, double *scratch) {
     double s = scratch[0];
     for (int i = 1; i < nw; i++) s = fmax(s, scratch[i]);
     return s;
+}
+
+// Largest 64-bit key of the wavefront, on every lane: DPP shifts within rows of
+// 16 lanes, then row broadcasts 15 and 31 (gfx9 DPP), then lane 63.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t k) {
+    auto step = [&](auto ctrl_c, auto rmask_c) {
+        constexpr int ctrl = decltype(ctrl_c)::value, rmask = decltype(rmask_c)::value;
+        const int lo = (int)(uint32_t)k, hi = (int)(uint32_t)(k >> 32);
+        const uint32_t olo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, ctrl, rmask, 0xf, false);
+        const uint32_t ohi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, ctrl, rmask, 0xf, false);
+        const uint64_t o = ((uint64_t)ohi << 32) | olo;
+        k = o > k ? o : k;
+    };
+    step(std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xf>{});   // row_shr:1
+    step(std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xf>{});   // row_shr:2
+    step(std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xf>{});   // row_shr:4
+    step(std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xf>{});   // row_shr:8
+    step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xa>{});   // row_bcast:15
+    step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xc>{});   // row_bcast:31
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Fermi-Dirac as the reference evaluates it (src/green_functions.f90:74-84):

@@ -45,32 +45,6 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 template <int RA, int RB> constexpr int eb_gr() { return RA * RB >= 25 ? 13 : 8; }
 template <int RA, int RB> constexpr int eb_gl() { return RA * RB >= 25 ? 5 : 4; }
 
-// pivot candidates (value, row; row < 0: none): keep the larger value, then the
-// smaller row.  Commutative, so both lanes of a pair agree on the result.
-__device__ __forceinline__ void cand_merge(double &bv, int &bi, double v2, int i2) {
-    const bool take = i2 >= 0 && (bi < 0 || v2 > bv || (v2 == bv && i2 < bi));
-    bv = take ? v2 : bv;
-    bi = take ? i2 : bi;
-}
-// merge with lane l ^ O (O = 16 or 32) through the gfx950 lane-swap instructions:
-// permlane{16,32}_swap(x, x) returns x at lane l and at lane l ^ O, in an order
-// that depends on the lane, which the commutative merge does not care about
-template <int O>
-__device__ __forceinline__ void lane_merge(double &bv, int &bi) {
-    const unsigned lo = (unsigned)__double2loint(bv), hi = (unsigned)__double2hiint(bv), ix = (unsigned)bi;
-    const auto l2 = O == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
-                            : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-    const auto h2 = O == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
-                            : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-    const auto i2 = O == 16 ? __builtin_amdgcn_permlane16_swap(ix, ix, false, false)
-                            : __builtin_amdgcn_permlane32_swap(ix, ix, false, false);
-    double v = __hiloint2double((int)h2[0], (int)l2[0]);
-    int i = (int)i2[0];
-    cand_merge(v, i, __hiloint2double((int)h2[1], (int)l2[1]), (int)i2[1]);
-    bv = v;
-    bi = i;
-}
-
 // packed G^< positions (common/linear_algebra.f90:142-191): Re part over a < b
 // row-wise, Im part over a <= b row-wise, after the 2 nA^2 G^r values
 __device__ __forceinline__ int su_index(int a, int b, int nA) { return a * (nA - 1) - (a * (a - 1)) / 2 + (b - a - 1); }
@@ -115,26 +89,64 @@ __device__ __forceinline__ WLayout wlayout(char *base, int n, int nCL, int RA, i
     return L;
 }
 
-template <int RA, int RB, int TI, int TJ>
-__global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
+// The Gauss-Jordan part alone (PART 1 below) needs only the pivot-search buffers
+// and the maps: crow [2][NW][TJ RB] | pcol [2][TI RA] | cand [2 NW] | red [3 NW] |
+// 5 int maps [TI RA] (14.6 KB at 80 orbitals, against 58 KB for the whole kernel).
+__host__ __device__ inline size_t wlayout_gj_bytes(int RA, int RB, int TI, int TJ) {
+    const int NW = TI * TJ / 64;
+    return (size_t)16 * (2 * NW * TJ * RB + 2 * TI * RA + 2 * NW) + (size_t)8 * 3 * NW + (size_t)4 * 5 * TI * RA;
+}
+__device__ __forceinline__ WLayout wlayout_gj(char *base, int RA, int RB, int TI, int TJ) {
+    const int NW = TI * TJ / 64;
+    WLayout L;
+    L.stage = L.srow = L.grcl = L.glcl = L.chunk = nullptr;
+    L.crow = (double2 *)base;
+    L.pcol = L.crow + 2 * NW * TJ * RB;
+    L.cand = L.pcol + 2 * TI * RA;
+    L.red = (double *)(L.cand + 2 * NW);
+    L.piv = (int *)(L.red + 3 * NW);
+    L.qinv = L.piv + TI * RA;
+    L.spc = L.qinv + TI * RA;
+    L.actl = L.spc + TI * RA;
+    L.posl = L.actl + TI * RA;
+    return L;
+}
+// per energy, the slab holds M (n^2 complex, column-major) and then the pivot
+// maps piv, qinv (2 n ints) that pass from PART 1 to PART 2
+__host__ __device__ inline size_t wave_slab_stride(int n) { return (size_t)n * n + (2 * (size_t)n * 4 + 15) / 16; }
+// waves per SIMD asked of the Gauss-Jordan part: three energies per CU where the
+// register tile allows it without spilling inside the pivot loop
+template <int RA> constexpr int gj_waves_per_simd() { return RA <= 5 ? 3 : 1; }
+
+// PART 0: the whole per-energy solve in one kernel.  PART 1: the build and the
+// Gauss-Jordan inverse, which write M and the pivot maps to the slab; PART 2: the
+// Keldysh products and the epilogues, from the slab.  The split lets the
+// Gauss-Jordan part, whose steps are bound by the latency of their dependency
+// chain, run three energies per CU (its registers and LDS allow it) instead of two.
+template <int RA, int RB, int TI, int TJ, int PART>
+__global__ __launch_bounds__(TI * TJ, PART == 1 ? gj_waves_per_simd<RA>() : 1) void k_dyson_w(DysonArgs A) {
     constexpr int NT = TI * TJ, NW = NT / 64, EB = eb_gr<RA, RB>(), EBL = eb_gl<RA, RB>();
     extern __shared__ __align__(16) char lds_raw[];
     const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL;
     const int tid = threadIdx.x, li = tid / TJ, lj = tid % TJ, lane = tid & 63, wv = tid >> 6;
     const int iw = A.iw0 + blockIdx.x;
-    const WLayout L = wlayout(lds_raw, n, nCL, RA, RB, TI, TJ);
+    const WLayout L = PART == 1 ? wlayout_gj(lds_raw, RA, RB, TI, TJ) : wlayout(lds_raw, n, nCL, RA, RB, TI, TJ);
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
-    double2 *Ms = A.mscr + (size_t)blockIdx.x * n * n;  // [c][r]: M column-major
+    double2 *Ms = A.mscr + (size_t)blockIdx.x * wave_slab_stride(n);  // [c][r]: M column-major
+    int *maps = (int *)(Ms + (size_t)n * n);                             // piv [n], qinv [n]
     // G^r[q_r][p_c] = M[r][c]: G^r(i, j) = M(piv[i], qinv[j]), read from Ms
     auto GR = [&](int i, int j) { return Ms[L.qinv[j] * n + L.piv[i]]; };
     const int colbase = lj * n + li;                         // (r, c) = (li + TI a, lj + TJ b) at colbase + TJ b n + TI a
 
-    for (int i = tid; i < n; i += NT) L.posl[i] = A.pos[i];
-    for (int i = tid; i < nA; i += NT) L.actl[i] = A.act[i];
+    if (PART != 1) {
+        for (int i = tid; i < n; i += NT) L.posl[i] = A.pos[i];
+        for (int i = tid; i < nA; i += NT) L.actl[i] = A.act[i];
+    }
 
+    if constexpr (PART != 2) {
     // ---- 1. A = w + Hemb - Sigma1 - Sigma2^r ------------------------------------
     double2 M[RA][RB];
     // row-clamped base per a, columns past n read the allocation's zero pad
@@ -180,10 +192,8 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
             const int i = li + TI * a, j = lj + TJ * b;
             if (i >= n || j >= n) M[a][b] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
-    unsigned used = 0;  // bit a: row li + TI a already pivoted (or padding)
-#pragma unroll
-    for (int a = 0; a < RA; a++)
-        if (li + TI * a >= n) used |= 1u << a;
+    // rows already pivoted, as a wavefront-uniform mask (n <= 128: two words)
+    uint64_t used_lo = 0, used_hi = 0;
 
     // ---- 2. Gauss-Jordan with partial pivoting ----------------------------------
     // step k = -1 only posts the candidates of column 0.
@@ -220,6 +230,8 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
             p = max(p, 0);  // p < 0 is unreachable: n - k unused rows remain at step k
             p = __builtin_amdgcn_readfirstlane(p);
             pw = __builtin_amdgcn_readfirstlane(pw);
+            if (p < 64) used_lo |= 1ull << p;
+            else used_hi |= 1ull << (p - 64);
             if (tid == 0) { L.piv[k] = p; L.qinv[p] = k; }
             const double2 *prow = L.crow + (buf * NW + pw) * TJ * RB;
             const int tp = p % TI, ap = p / TI;
@@ -248,7 +260,6 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
 #pragma unroll
                 for (int a = 0; a < RA; a++)
                     if (a == ap) {
-                        used |= 1u << a;
 #pragma unroll
                         for (int b = 0; b < RB; b++) M[a][b] = sp[b];
                     }
@@ -256,34 +267,46 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
         }
         if (k + 1 < n) {
             const int kk = k + 1, buf = kk & 1, bkk = kk / TJ, tkk = kk % TJ;
-            double best = -1.0;
-            int bi = -1;
+            double2 *pcn = L.pcol + buf * TI * RA;
+            // the owners of column kk post it: the next step's multipliers, and the
+            // values this wavefront's candidate search reads back
             if (lj == tkk) {
 #pragma unroll
                 for (int b = 0; b < RB; b++)
                     if (b == bkk)
 #pragma unroll
-                        for (int a = 0; a < RA; a++) {
-                            L.pcol[buf * TI * RA + li + TI * a] = M[a][b];
-                            if (!(used & (1u << a))) {
-                                double v = fabs(M[a][b].x) + fabs(M[a][b].y);
-                                if (v != v) v = -0.5;  // NaN never wins over a number
-                                if (bi < 0 || v > best) { best = v; bi = li + TI * a; }
-                            }
-                        }
+                        for (int a = 0; a < RA; a++) pcn[li + TI * a] = M[a][b];
             }
-            // the wavefront's owners of column kk: lanes tkk + TJ j
-            if (TJ == 8) {
-                const double vo = __shfl_xor(best, 8, 64);
-                const int io = __shfl_xor(bi, 8, 64);
-                cand_merge(best, bi, vo, io);
+            // a wavefront's LDS operations complete in order: its own posts are read
+            // back without a barrier (the fence only keeps the compiler from moving them)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // this wavefront's rows li in [LW wv, LW wv + LW), all a: lane i takes row
+            // LW wv + i % LW + TI (i / LW), so lane order is row order.  Key: |Re| + |Im|
+            // as ordered bits + 2; a NaN ranks 1, below every number; used rows and
+            // padding 0.  The largest key, on the first lane that holds it, is the
+            // zgetf2 candidate: largest value, then smallest row.
+            constexpr int LW = 64 / TJ;
+            uint64_t key = 0;
+            if (lane < LW * RA) {
+                const int r = LW * wv + lane % LW + TI * (lane / LW);
+                const uint64_t um = r < 64 ? used_lo >> (r & 63) : used_hi >> (r & 63);
+                if (r < n && !(um & 1)) {
+                    const double2 c = pcn[r];
+                    const double v = fabs(c.x) + fabs(c.y);
+                    key = v != v ? 1 : (uint64_t)__double_as_longlong(v) + 2;
+                }
             }
-            lane_merge<16>(best, bi);
-            lane_merge<32>(best, bi);
-            const int rw = __builtin_amdgcn_readlane(bi, tkk);
-            if (lane == tkk) {
-                L.cand[buf * NW + wv] = make_double2(best, (double)bi);
+            const uint64_t wmax = wave_max_u64(key);
+            int rw = -1;
+            double best = -1.0;
+            if (wmax != 0) {
+                const uint64_t hit = __ballot(key == wmax);
+                const int wl = __ffsll((unsigned long long)hit) - 1;
+                rw = LW * wv + wl % LW + TI * (wl / LW);
+                best = wmax == 1 ? -0.5 : __longlong_as_double((long long)(wmax - 2));
             }
+            rw = __builtin_amdgcn_readfirstlane(rw);
+            if (lane == 0) L.cand[buf * NW + wv] = make_double2(best, (double)rw);
             if (rw >= 0 && li == rw % TI) {
                 const int aw = rw / TI;
                 double2 *dst = L.crow + (buf * NW + wv) * TJ * RB;
@@ -299,7 +322,6 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
 
     __builtin_amdgcn_s_setprio(0);
     // ---- 3. scratch copy of M: Ms (column-major) ------------------------------------
-    for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
 #pragma unroll
     for (int a = 0; a < RA; a++)
 #pragma unroll
@@ -309,6 +331,23 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
                 Ms[colbase + TJ * b * n + TI * a] = M[a][b];
             }
         }
+    if constexpr (PART == 1) {
+        // the maps were complete at the last step's barrier
+        for (int c = tid; c < n; c += NT) {
+            maps[c] = L.piv[c];
+            maps[n + c] = L.qinv[c];
+        }
+        return;
+    }
+    } else {
+        // the first part's M and maps, from the slab (ordered by the kernel boundary)
+        for (int c = tid; c < n; c += NT) {
+            L.piv[c] = maps[c];
+            L.qinv[c] = maps[n + c];
+        }
+        __syncthreads();
+    }
+    for (int c = tid; c < n; c += NT) L.spc[c] = A.suppos[L.piv[c]];
     __threadfence_block();
     __syncthreads();
 
@@ -590,16 +629,25 @@ __global__ __launch_bounds__(TI * TJ) void k_dyson_w(DysonArgs A) {
     }
 }
 
+// split = false: one fused kernel per chunk; split = true: the Gauss-Jordan part,
+// then the Keldysh part, per chunk
 template <int RA, int RB, int TI, int TJ>
-hipError_t launch_w(const DysonArgs &a0, int count, hipStream_t s) {
+hipError_t launch_w(const DysonArgs &a0, int count, bool split, hipStream_t s) {
     const size_t lds = wlayout_bytes(a0.norb, a0.nCL, RA, RB, TI, TJ);
-    if (hipError_t e = qs_lds_attr((const void *)k_dyson_w<RA, RB, TI, TJ>)) return e;
+    const size_t lds1 = wlayout_gj_bytes(RA, RB, TI, TJ);
+    if (hipError_t e = qs_lds_attr((const void *)k_dyson_w<RA, RB, TI, TJ, 0>)) return e;
+    if (hipError_t e = qs_lds_attr((const void *)k_dyson_w<RA, RB, TI, TJ, 2>)) return e;
     // the scratch slab holds mscr_energies energies: launch in chunks of that many
     for (int c0 = 0; c0 < count; c0 += a0.mscr_energies) {
         DysonArgs a = a0;
         a.iw0 = a0.iw0 + c0;
         const int cnt = count - c0 < a0.mscr_energies ? count - c0 : a0.mscr_energies;
-        hipLaunchKernelGGL((k_dyson_w<RA, RB, TI, TJ>), dim3(cnt), dim3(TI * TJ), lds, s, a);
+        if (!split) {
+            hipLaunchKernelGGL((k_dyson_w<RA, RB, TI, TJ, 0>), dim3(cnt), dim3(TI * TJ), lds, s, a);
+        } else {
+            hipLaunchKernelGGL((k_dyson_w<RA, RB, TI, TJ, 1>), dim3(cnt), dim3(TI * TJ), lds1, s, a);
+            hipLaunchKernelGGL((k_dyson_w<RA, RB, TI, TJ, 2>), dim3(cnt), dim3(TI * TJ), lds, s, a);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -614,23 +662,24 @@ size_t qs_dyson_wave_lds_bytes(int n, int nCL) {
     return wlayout_bytes(n, nCL, R, R, T, T);
 }
 
-size_t qs_dyson_wave_scratch_bytes(int n) { return (size_t)n * n * sizeof(double2); }
+size_t qs_dyson_wave_scratch_bytes(int n) { return wave_slab_stride(n) * sizeof(double2); }
 
 int qs_dyson_wave_max_norb() { return 96; }
 
-hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s) {
+hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, bool split, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     if (!a.mscr || a.mscr_energies <= 0 || a.norb > qs_dyson_wave_max_norb()) return hipErrorInvalidValue;
     if (qs_dyson_wave_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
     const int n = a.norb;
-    if (n <= 8) return launch_w<1, 1, 8, 8>(a, count, s);
-    if (n <= 16) return launch_w<2, 2, 8, 8>(a, count, s);
+    if (split && n <= 16) return hipErrorInvalidValue;  // one wavefront per energy: nothing to gain
+    if (n <= 8) return launch_w<1, 1, 8, 8>(a, count, split, s);
+    if (n <= 16) return launch_w<2, 2, 8, 8>(a, count, split, s);
     switch ((n + 15) / 16) {
-        case 2: return launch_w<2, 2, 16, 16>(a, count, s);
-        case 3: return launch_w<3, 3, 16, 16>(a, count, s);
-        case 4: return launch_w<4, 4, 16, 16>(a, count, s);
-        case 5: return launch_w<5, 5, 16, 16>(a, count, s);
-        case 6: return launch_w<6, 6, 16, 16>(a, count, s);
+        case 2: return launch_w<2, 2, 16, 16>(a, count, split, s);
+        case 3: return launch_w<3, 3, 16, 16>(a, count, split, s);
+        case 4: return launch_w<4, 4, 16, 16>(a, count, split, s);
+        case 5: return launch_w<5, 5, 16, 16>(a, count, split, s);
+        case 6: return launch_w<6, 6, 16, 16>(a, count, split, s);
         default: return hipErrorInvalidValue;
     }
 }

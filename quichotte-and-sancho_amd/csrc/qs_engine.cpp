@@ -126,7 +126,7 @@ struct qs_handle {
     bool final_valid = false;
     Prof prof;
     int dyson_variant = 0;
-    double dyson_tune_ms[4] = {0, 0, 0, 0};
+    double dyson_tune_ms[QS_DYSON_NSHAPES] = {};
     // precision of the order-2 SCF sweeps: 0 = FP64; 1 = complex-FP32 Dyson/Keldysh
     // (MFMA) while max|F| >= mixed_switch, then FP64 to the end (and the final sweep)
     int precision = 0;
@@ -606,7 +606,7 @@ Sanity diag_sanity(const double *sumImGL, const double *sumImGR, int nb, double 
 // length): a handle created while other work runs (a serving loop) then takes
 // the shape an earlier handle of that size measured, and gives the same
 // results bit for bit.
-struct TuneRecord { int variant; double ms[4]; };
+struct TuneRecord { int variant; double ms[QS_DYSON_NSHAPES]; };
 std::mutex g_tune_mu;
 std::mutex g_capture_mu;   // graph captures of SCF steps, one at a time in the process
 std::map<std::tuple<int, int, int>, TuneRecord> g_tune;
@@ -631,7 +631,7 @@ int tune_dyson(qs_handle *h) {
         const auto it = g_tune.find(key);
         if (it != g_tune.end()) {
             h->dyson_variant = it->second.variant;
-            for (int i = 0; i < 4; i++) h->dyson_tune_ms[i] = it->second.ms[i];
+            for (int i = 0; i < QS_DYSON_NSHAPES; i++) h->dyson_tune_ms[i] = it->second.ms[i];
             return QS_OK;
         }
     }
@@ -651,13 +651,13 @@ int tune_dyson(qs_handle *h) {
         HIPCHK(h, hipEventSynchronize(e1));
         float ms = 0.f;
         HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-        if (v < 4) h->dyson_tune_ms[v] = ms / 2;
+        if (v < QS_DYSON_NSHAPES) h->dyson_tune_ms[v] = ms / 2;
         if (ms < best) { best = ms; h->dyson_variant = v; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     TuneRecord r{h->dyson_variant, {}};
-    for (int i = 0; i < 4; i++) r.ms[i] = h->dyson_tune_ms[i];
+    for (int i = 0; i < QS_DYSON_NSHAPES; i++) r.ms[i] = h->dyson_tune_ms[i];
     g_tune.emplace(key, r);
     return QS_OK;
 }
@@ -1761,7 +1761,7 @@ const char *qs_dyson_shape(const qs_handle *h, int32_t *uses_mfma) {
 int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant, int32_t nmax) {
     if (!h) return QS_ERR_ARG;
     if (variant) *variant = h->dyson_variant;
-    for (int i = 0; i < nmax && i < 4; i++)
+    for (int i = 0; i < nmax && i < QS_DYSON_NSHAPES; i++)
         if (ms_per_variant) ms_per_variant[i] = h->dyson_tune_ms[i];
     return qs_dyson_variants(h->norb);
 }

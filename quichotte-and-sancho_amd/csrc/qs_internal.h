@@ -67,7 +67,9 @@ size_t qs_dyson_lds_bytes(int norb, int nCL);
 int qs_dyson_max_norb();
 // launch shapes: 0 = 256-thread workgroup, 1 = 512-thread workgroup (norb > 16),
 // 2 = one wavefront per energy (norb <= qs_dyson_wave_max_norb(), needs mscr)
-enum { QS_DYSON_WG256 = 0, QS_DYSON_WG512 = 1, QS_DYSON_WAVE = 2, QS_DYSON_BIG = 3, QS_DYSON_NSHAPES = 4 };
+// 4 = shape 2 split into a Gauss-Jordan kernel (three energies per CU) and a Keldysh kernel
+enum { QS_DYSON_WG256 = 0, QS_DYSON_WG512 = 1, QS_DYSON_WAVE = 2, QS_DYSON_BIG = 3, QS_DYSON_SPLIT = 4,
+       QS_DYSON_NSHAPES = 5 };
 // large-Norb path (qs_dyson_big.hip): energies in chunks through HBM, batched
 // library factorisation and products, hand-written build and packing kernels
 struct QsBigWork;
@@ -88,8 +90,9 @@ hipError_t qs_launch_gj_inv(int np, int count, void *M, void *G, int *status, bo
 // C[b] = A[b] op(B[b]), op = identity or conjugate transpose
 hipError_t qs_launch_cgemm(int np, int count, const void *A, const void *B, void *C, bool conjB, bool fp32,
                            hipStream_t s);
-// one wavefront per energy (qs_dyson_wave.hip)
-hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, hipStream_t s);
+// small-LDS kernels (qs_dyson_wave.hip): one fused kernel, or (split) the Gauss-Jordan
+// part and the Keldysh part as two kernels through the slab (17..96 orbitals)
+hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, bool split, hipStream_t s);
 size_t qs_dyson_wave_lds_bytes(int norb, int nCL);
 size_t qs_dyson_wave_scratch_bytes(int norb);
 int qs_dyson_wave_max_norb();

@@ -323,8 +323,8 @@ class ScfEngine:
 
     def dyson_choice(self):
         v = ctypes.c_int32()
-        ms = np.zeros(4)
-        n = self.L.qs_dyson_choice(self.h, ctypes.byref(v), _ptr(ms), 4)
+        ms = np.zeros(8)
+        n = self.L.qs_dyson_choice(self.h, ctypes.byref(v), _ptr(ms), 8)
         return v.value, ms[:n].tolist()
 
     def set_precision(self, mode, switch_fmax=1e-5):

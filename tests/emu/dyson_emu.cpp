@@ -7,6 +7,8 @@
 // usage: dyson_emu <problem.bin> <out.bin> <TI> <RA> <RB>   (k_dyson<TI, RA, RB>)
 //        dyson_emu <problem.bin> <out.bin> wave 0 0      (built with QS_EMU_WAVE and
 //        qs_dyson_wave.hip as the source: the small-LDS kernel's own launch)
+//        dyson_emu <problem.bin> <out.bin> split 0 0     (the same kernel as its
+//        Gauss-Jordan and Keldysh parts, two launches through the slab)
 // problem.bin: int32 n, count, mode, use_sigma, nCL; float64 w_init, delta,
 // mu, beta, V; complex128 Hemb (n*n), GamL (n*n), GamR (n*n), GamL_cl
 // (nCL*nCL); int32 cl (nCL); complex128 sig1 (n/2*4); complex128 sig
@@ -18,7 +20,7 @@
 #include QS_DYSON_SOURCE
 
 #ifndef QS_EMU_WAVE
-hipError_t qs_launch_dyson_wave(const DysonArgs &, int, hipStream_t) { return hipErrorInvalidValue; }
+hipError_t qs_launch_dyson_wave(const DysonArgs &, int, bool, hipStream_t) { return hipErrorInvalidValue; }
 int qs_dyson_wave_max_norb() { return 0; }
 #endif
 hipError_t qs_launch_dyson_big(const DysonArgs &, int, hipStream_t) { return hipErrorInvalidValue; }
@@ -102,7 +104,8 @@ int main(int argc, char **argv) {
     std::vector<double2> mscr(qs_dyson_wave_scratch_bytes(n) / sizeof(double2) * count);
     a.mscr = mscr.data();
     a.mscr_energies = count;
-    if (shape == "wave,0,0") rc = qs_launch_dyson_wave(a, count, nullptr);
+    if (shape == "wave,0,0") rc = qs_launch_dyson_wave(a, count, false, nullptr);
+    else if (shape == "split,0,0") rc = qs_launch_dyson_wave(a, count, true, nullptr);
 #else
     if (shape == "16,5,5") rc = launch_v<16, 5, 5>(a, count, nullptr);
     else if (shape == "32,3,5") rc = launch_v<32, 3, 5>(a, count, nullptr);

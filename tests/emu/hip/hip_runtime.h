@@ -118,6 +118,53 @@ T exchange(T v, int src_of_lane_xor, bool first) {
     return out;
 }
 
+// every lane's value of v, in lane order
+template <class T>
+void gather64(T v, T (&out)[64]) {
+    static_assert(sizeof(T) <= 8, "wave exchange of at most 8 bytes");
+    const int t = (int)emu_tid.x, lane = t & 63;
+    Wave &w = cur->waves[t >> 6];
+    uint64_t b = 0;
+    memcpy(&b, &v, sizeof(T));
+    w.buf[lane] = b;
+    w.bar.wait();
+    for (int i = 0; i < 64; i++) memcpy(&out[i], &w.buf[i], sizeof(T));
+    w.bar.wait();
+}
+
+// DPP move (the controls the kernels use): row_shr:1..15 within rows of 16 lanes,
+// row_bcast:15 (lane 15 of each row to the next row) and row_bcast:31 (lane 31 to
+// rows 2 and 3); a lane whose row or bank is masked off, or whose source is
+// outside the row with bound_ctrl off, keeps `old`
+inline int update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
+    int all[64];
+    gather64(src, all);
+    const int l = (int)emu_tid.x & 63, row = l >> 4, l16 = l & 15;
+    if (!((row_mask >> row) & 1) || !((bank_mask >> (l16 >> 2)) & 1)) return old;
+    int from = -1;
+    if (ctrl >= 0x111 && ctrl <= 0x11f) {
+        if (l16 >= ctrl - 0x110) from = l - (ctrl - 0x110);
+    } else if (ctrl == 0x142) {
+        if (row >= 1) from = 16 * row - 1;
+    } else if (ctrl == 0x143) {
+        if (row >= 2) from = 31;
+    } else {
+        fprintf(stderr, "emu: DPP control 0x%x not emulated\n", ctrl);
+        abort();
+    }
+    if (from < 0) return bound_ctrl ? 0 : old;
+    return all[from];
+}
+
+inline unsigned long long ballot(bool pred) {
+    int all[64];
+    gather64((int)pred, all);
+    unsigned long long m = 0;
+    for (int i = 0; i < 64; i++)
+        if (all[i]) m |= 1ull << i;
+    return m;
+}
+
 template <class T>
 T exchange_from(T v, int src) {
     const int t = (int)emu_tid.x;
@@ -167,6 +214,31 @@ inline double __hiloint2double(int hi, int lo) {
     return d;
 }
 #define __builtin_amdgcn_s_setprio(p) ((void)0)
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, rmask, bmask, bc) emu::update_dpp((old), (src), (ctrl), (rmask), (bmask), (bc))
+#define __ballot(p) emu::ballot(p)
+// a wavefront-scope fence: the lanes of a wavefront meet, which orders their
+// LDS accesses as the hardware's in-order LDS queue does
+#define __builtin_amdgcn_fence(order, scope) emu::wave_fence(scope)
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+inline long long __double_as_longlong(double d) {
+    long long r;
+    memcpy(&r, &d, 8);
+    return r;
+}
+inline double __longlong_as_double(long long v) {
+    double r;
+    memcpy(&r, &v, 8);
+    return r;
+}
+namespace emu {
+inline void wave_fence(const char *scope) {
+    if (strcmp(scope, "wavefront") != 0) {
+        fprintf(stderr, "emu: fence scope %s not emulated\n", scope);
+        abort();
+    }
+    cur->waves[(int)emu_tid.x >> 6].bar.wait();
+}
+}  // namespace emu
 // the barrier that follows orders global memory as well (host threads share it)
 inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline int atomicOr(int *p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }

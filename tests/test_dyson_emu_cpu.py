@@ -173,7 +173,8 @@ def test_k_dyson_shapes_address_sanitizer(builds, tmp_path, n, shapes, mode):
 @pytest.mark.parametrize("n", [16, 44, 80])
 @pytest.mark.parametrize("mode", [M_INIT2, M_SCF2, M_FINAL])
 def test_small_lds_kernel_address_sanitizer(builds, tmp_path, n, mode):
-    """k_dyson_w (8x8 tile at 16 orbitals, 16x16 above) against numpy and k_dyson."""
+    """k_dyson_w (8x8 tile at 16 orbitals, 16x16 above) against numpy and k_dyson, and
+    its split form (Gauss-Jordan kernel, then Keldysh kernel) against itself."""
     exes, _ = builds
     count = 2
     P = make_problem(tmp_path / "p.bin", n, count, mode, use_sigma=int(mode != M_INIT2), seed=n)
@@ -181,6 +182,10 @@ def test_small_lds_kernel_address_sanitizer(builds, tmp_path, n, mode):
     w = read_out(tmp_path / "w.bin", n, count)
     for k in written(w, mode):
         assert not np.isnan(w[k]).any(), k
+    if n > 16:
+        # split into the Gauss-Jordan and Keldysh kernels: the same operations, the same bits
+        run(exes[("k_dyson_w", "address")], tmp_path / "p.bin", tmp_path / "s.bin", ("split", "0", "0"))
+        compare(w, read_out(tmp_path / "s.bin", n, count), written(w, mode), 0.0)
     if n == 80:
         run(exes[("k_dyson", "address")], tmp_path / "p.bin", tmp_path / "v.bin", ("32", "3", "5"))
         compare(w, read_out(tmp_path / "v.bin", n, count), written(w, mode), 1e-12)
@@ -191,6 +196,7 @@ def test_small_lds_kernel_address_sanitizer(builds, tmp_path, n, mode):
 
 @pytest.mark.parametrize("kernel,n,shape", [("k_dyson", 80, ("16", "5", "5")), ("k_dyson", 80, ("32", "3", "5")),
                                             ("k_dyson_w", 80, ("wave", "0", "0")),
+                                            ("k_dyson_w", 80, ("split", "0", "0")),
                                             ("k_dyson_w", 16, ("wave", "0", "0"))])
 @pytest.mark.parametrize("mode", [M_SCF2, M_FINAL])
 def test_no_unordered_lds_access_thread_sanitizer(builds, tmp_path, kernel, n, shape, mode):

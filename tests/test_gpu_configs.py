@@ -9,11 +9,12 @@ helix80 -- the same helix with N_ions 8, N_turns 5 (80 spin-orbitals), U = 2,
 2^16 energies on (-12, 12], V = -1 ... 1.
 
 Tolerances.  Both sides stop at the first step with max|F| < tolf, and a
-rounding-level difference in the Broyden path can move that step by one (the
-oracle's own step count at V = 0.1 moves from 26 to 25 with its reduction
-order), so at tolf = 1e-8 the converged observables are fixed only to O(tolf):
-the sweep is compared at 1e-8 relative (the precision target of the I(V)
-curve) with step counts within one.  The same system solved to tolf = 1e-12,
+rounding-level difference in the Broyden path can move that step (the oracle's
+own step counts move with its OpenMP reduction order: at V = 0.1 from 26 to 25,
+at V = 0.2 from 23 to 20 between hosts of different core counts), so at tolf =
+1e-8 the converged observables are fixed only to O(tolf): the sweep is compared
+at 1e-8 relative (the precision target of the I(V) curve), with the oracle on a
+fixed 16 threads and step counts within three.  The same system solved to tolf = 1e-12,
 far below that level, is compared at 1e-10 relative on J, the populations and
 the spin texture (whatever the two step counts).  An absolute floor covers J(V = 0), which is round-off.
 """
@@ -57,7 +58,7 @@ def test_helix16_grid_is_the_benchmark_grid():
 def test_helix16_sweep_matches_oracle():
     """The 11-point sweep with continuation, both sides at tolf = 1e-8."""
     prob = helix16()
-    o = Oracle(prob)
+    o = Oracle(prob, nthreads=16)
     g = gpu_engine(prob)
     fresh_o = fresh_g = True
     report = []
@@ -66,7 +67,7 @@ def test_helix16_sweep_matches_oracle():
         rg = g.solve_scf(V, fresh_g)
         fresh_o, fresh_g = not ro["converged"], not rg.converged
         assert ro["converged"] and rg.converged
-        assert abs(ro["n_iter"] - rg.n_iter) <= 1, (V, ro["n_iter"], rg.n_iter)
+        assert abs(ro["n_iter"] - rg.n_iter) <= 3, (V, ro["n_iter"], rg.n_iter)
         Jo = np.array(o.current())
         Jg, rows_g = observables(g, V)
         rows_o = np.array(spin_texture_rows(V, o.g_less_obs()))[:, 2:]
