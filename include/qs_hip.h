@@ -136,6 +136,45 @@ int qs_precision_steps(const qs_handle *h, int32_t *fp32_steps, int32_t *fp64_st
 int qs_set_step_mode(qs_handle *h, int32_t mode);
 int qs_step_info(const qs_handle *h, int32_t *device_steps_active, int32_t *graph_steps);
 
+/* --- groups: several solves stepped together ---------------------------------
+ * No counterpart in the reference, whose voltage loop solves one bias point at a
+ * time (src/quichotte.f90:125-143).  For callers with many independent bias
+ * points (a sweep split over several handles, a serving loop): a group runs one
+ * SCF step of every member per qs_group_step, on one stream.  Each member's
+ * step is its own solve_scf step (src/scf.f90:78-151; residual src/scf.f90:158-188,
+ * broyden_solve common/broyden_mixing.f90:51-184) with device-controlled scalars
+ * (step mode 1), except that the Dyson/Keldysh sweeps of all members that share a
+ * kernel instance (same register tile and launch shape: orbital counts may differ
+ * within a tile class, grid lengths may differ) run as one launch.  A member's
+ * results equal those of qs_solve_scf on its handle bit for bit.  Solves join
+ * (qs_group_add) and leave (when they end) between steps, so a loop can keep the
+ * group full as requests arrive.  One thread drives a group; while a handle is a
+ * member, no other call may use it (qs_destroy removes it first).  At most 64
+ * members. */
+typedef struct qs_group qs_group;
+int qs_group_create(int32_t device, qs_group **out);
+void qs_group_destroy(qs_group *g);
+const char *qs_group_last_error(const qs_group *g);
+/* start a solve at voltage V on h and make it a member: fresh = 1 starts from
+ * the non-interacting solution, 0 continues from the handle's last input, as
+ * qs_solve_scf.  Needs a handle on which qs_solve_scf takes device-controlled
+ * steps (qs_step_info): order 2, FP64 sweeps, FP32 history of at most 16 columns,
+ * and one energy's packed slice within the fused update's LDS (at most 82 active
+ * spin-orbitals); QS_ERR_UNSUPPORTED otherwise. */
+int qs_group_add(qs_group *g, qs_handle *h, double V, int32_t fresh);
+int qs_group_members(const qs_group *g);
+/* one SCF step of every member.  Members whose solve ended (converged, aborted
+ * as a runaway, maxscf reached, or an error) get their final sweep and leave:
+ * done[k], res[k] (as qs_solve_scf's) and status[k] (QS_OK, or the error; the
+ * message in qs_last_error(done[k])) for k < *n_done.  The arrays must have room
+ * for every member (max_done >= qs_group_members); the observables of a finished
+ * handle are read as after qs_solve_scf. */
+int qs_group_step(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
+                  int32_t *n_done);
+/* measurement: Dyson launches of the last step and how many members ran in
+ * grouped launches; HIP-event time of all steps' Dyson sweeps and the step count */
+int qs_group_info(const qs_group *g, int32_t *launches, int32_t *fused_members, double *dyson_ms, int64_t *steps);
+
 /* --- seams inside the SCF step (order 2) ------------------------------------ */
 /* sweep(M_INIT2) + pack_gf (src/scf.f90:116-122): the non-interacting active
  * G^r, G^< packed into the handle's x; copied to x_out (n_vec) if not NULL */

@@ -123,19 +123,21 @@ template <int RA> constexpr int gj_waves_per_simd() { return RA <= 5 ? 3 : 1; }
 // Keldysh products and the epilogues, from the slab.  The split lets the
 // Gauss-Jordan part, whose steps are bound by the latency of their dependency
 // chain, run three energies per CU (its registers and LDS allow it) instead of two.
+//
+// blk is the workgroup's energy within A's range: energy iw0 + blk, slab entry blk.
 template <int RA, int RB, int TI, int TJ, int PART>
-__global__ __launch_bounds__(TI * TJ, PART == 1 ? gj_waves_per_simd<RA>() : 1) void k_dyson_w(DysonArgs A) {
+__device__ __forceinline__ void dyson_w_body(const DysonArgs &A, const int blk) {
     constexpr int NT = TI * TJ, NW = NT / 64, EB = eb_gr<RA, RB>(), EBL = eb_gl<RA, RB>();
     extern __shared__ __align__(16) char lds_raw[];
     const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL;
     const int tid = threadIdx.x, li = tid / TJ, lj = tid % TJ, lane = tid & 63, wv = tid >> 6;
-    const int iw = A.iw0 + blockIdx.x;
+    const int iw = A.iw0 + blk;
     const WLayout L = PART == 1 ? wlayout_gj(lds_raw, RA, RB, TI, TJ) : wlayout(lds_raw, n, nCL, RA, RB, TI, TJ);
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta), fR = qs_fermi(w, 0.0, A.mu, A.beta);
     const double2 *sigw = (A.order == 2 && A.use_sigma && A.sig) ? A.sig + (size_t)iw * A.sig_ld : nullptr;
     // 32-bit offsets from uniform bases throughout (3 n^2 < 2^31)
-    double2 *Ms = A.mscr + (size_t)blockIdx.x * wave_slab_stride(n);  // [c][r]: M column-major
+    double2 *Ms = A.mscr + (size_t)blk * wave_slab_stride(n);  // [c][r]: M column-major
     int *maps = (int *)(Ms + (size_t)n * n);                             // piv [n], qinv [n]
     // G^r[q_r][p_c] = M[r][c]: G^r(i, j) = M(piv[i], qinv[j]), read from Ms
     auto GR = [&](int i, int j) { return Ms[L.qinv[j] * n + L.piv[i]]; };
@@ -629,6 +631,25 @@ __global__ __launch_bounds__(TI * TJ, PART == 1 ? gj_waves_per_simd<RA>() : 1) v
     }
 }
 
+template <int RA, int RB, int TI, int TJ, int PART>
+__global__ __launch_bounds__(TI * TJ, PART == 1 ? gj_waves_per_simd<RA>() : 1) void k_dyson_w(DysonArgs A) {
+    dyson_w_body<RA, RB, TI, TJ, PART>(A, blockIdx.x);
+}
+
+// One launch over the sweeps of several solves (a group step): member m owns
+// workgroups [first[m], first[m + 1]) and its arguments are tab[m].  The member
+// index is workgroup-uniform, so the arguments come in through scalar loads as
+// a single solve's kernel arguments do.
+template <int RA, int RB, int TI, int TJ, int PART>
+__global__ __launch_bounds__(TI * TJ, PART == 1 ? gj_waves_per_simd<RA>() : 1) void k_dyson_wg(
+    const DysonArgs *__restrict__ tab, DysonGroupIdx g) {
+    const int b = blockIdx.x;
+    int m = 0;
+    for (int k = 1; k < g.nmem; k++) m += b >= g.first[k] ? 1 : 0;
+    const DysonArgs A = tab[m];
+    dyson_w_body<RA, RB, TI, TJ, PART>(A, b - g.first[m]);
+}
+
 // split = false: one fused kernel per chunk; split = true: the Gauss-Jordan part,
 // then the Keldysh part, per chunk
 template <int RA, int RB, int TI, int TJ>
@@ -654,7 +675,67 @@ hipError_t launch_w(const DysonArgs &a0, int count, bool split, hipStream_t s) {
     return hipSuccess;
 }
 
+// the members' whole sweeps (iw0 = 0, nw energies each) in one launch per part
+template <int RA, int RB, int TI, int TJ>
+hipError_t launch_wg(const DysonArgs *ht, const DysonArgs *dt, int nmem, bool split, hipStream_t s) {
+    DysonGroupIdx g{};
+    g.nmem = nmem;
+    size_t lds = 0;
+    int tot = 0;
+    for (int m = 0; m < nmem; m++) {
+        g.first[m] = tot;
+        tot += ht[m].nw;
+        const size_t lm = wlayout_bytes(ht[m].norb, ht[m].nCL, RA, RB, TI, TJ);
+        lds = lm > lds ? lm : lds;
+    }
+    g.first[nmem] = tot;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (hipError_t e = qs_lds_attr((const void *)k_dyson_wg<RA, RB, TI, TJ, 0>)) return e;
+    if (hipError_t e = qs_lds_attr((const void *)k_dyson_wg<RA, RB, TI, TJ, 2>)) return e;
+    if (!split) {
+        hipLaunchKernelGGL((k_dyson_wg<RA, RB, TI, TJ, 0>), dim3(tot), dim3(TI * TJ), lds, s, dt, g);
+    } else {
+        hipLaunchKernelGGL((k_dyson_wg<RA, RB, TI, TJ, 1>), dim3(tot), dim3(TI * TJ),
+                           wlayout_gj_bytes(RA, RB, TI, TJ), s, dt, g);
+        hipLaunchKernelGGL((k_dyson_wg<RA, RB, TI, TJ, 2>), dim3(tot), dim3(TI * TJ), lds, s, dt, g);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
+
+// the register tile of a size: 8 x 8 threads (one wavefront) up to 16 orbitals, 16 x 16 above
+int qs_dyson_group_key(int n, int variant) {
+    if (variant != QS_DYSON_WAVE && variant != QS_DYSON_SPLIT) return -1;
+    if (n < 1 || n > qs_dyson_wave_max_norb()) return -1;
+    if (variant == QS_DYSON_SPLIT && n <= 16) return -1;
+    const int T = n <= 16 ? 8 : 16, R = (n + T - 1) / T;
+    return (variant == QS_DYSON_SPLIT ? 1000 : 0) + 10 * T + R;
+}
+
+hipError_t qs_launch_dyson_group(const DysonArgs *ht, const DysonArgs *dt, int nmem, int variant, hipStream_t s) {
+    if (nmem < 1 || nmem > QS_GROUP_MAX || !ht || !dt) return hipErrorInvalidValue;
+    const int key = qs_dyson_group_key(ht[0].norb, variant);
+    if (key < 0) return hipErrorInvalidValue;
+    for (int m = 0; m < nmem; m++) {
+        const DysonArgs &a = ht[m];
+        if (qs_dyson_group_key(a.norb, variant) != key || !a.mscr || a.iw0 != 0 || a.nw < 1 ||
+            a.mscr_energies < a.nw)
+            return hipErrorInvalidValue;
+    }
+    const bool split = variant == QS_DYSON_SPLIT;
+    const int n = ht[0].norb;
+    if (n <= 8) return launch_wg<1, 1, 8, 8>(ht, dt, nmem, split, s);
+    if (n <= 16) return launch_wg<2, 2, 8, 8>(ht, dt, nmem, split, s);
+    switch ((n + 15) / 16) {
+        case 2: return launch_wg<2, 2, 16, 16>(ht, dt, nmem, split, s);
+        case 3: return launch_wg<3, 3, 16, 16>(ht, dt, nmem, split, s);
+        case 4: return launch_wg<4, 4, 16, 16>(ht, dt, nmem, split, s);
+        case 5: return launch_wg<5, 5, 16, 16>(ht, dt, nmem, split, s);
+        case 6: return launch_wg<6, 6, 16, 16>(ht, dt, nmem, split, s);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 // 8 x 8 threads (one wavefront) up to 16 orbitals, 16 x 16 above
 size_t qs_dyson_wave_lds_bytes(int n, int nCL) {

@@ -72,6 +72,8 @@ struct Prof {
 
 }  // namespace
 
+struct qs_group;
+
 struct qs_handle {
     qs_problem p{};
     int norb = 0, nw = 0, nsites = 0, order = 0;
@@ -140,6 +142,32 @@ struct qs_handle {
     QsCtrl *h_ctl = nullptr;        // pinned copy of the control header
     double *d_csum = nullptr;       // column sums: 2 nA sanity + 2 MAXH history
     int graph_steps = 0;            // steps of the last solve replayed from a graph
+    qs_group *group = nullptr;      // the group stepping this handle's solve (qs_group_add)
+};
+
+// Several order-2 solves stepped together on one stream (qs_group_*): each step
+// runs every member's Sigma1/Sigma2 stage, then one Dyson sweep launch per kernel
+// instance for all members that share it, then every member's sanity check and
+// Broyden passes.  A member keeps its own handle state; its handle's stream is
+// set to the group's while it is a member.
+struct qs_group {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    struct Member {
+        qs_handle *h = nullptr;
+        double V = 0.0;
+        int its = 0;          // residuals of this solve so far
+        double fmax = 0.0;    // max|F| of the last residual that was not aborted
+        hipStream_t own = nullptr;
+    };
+    std::vector<Member> mem;
+    DysonArgs *h_tab = nullptr;  // pinned: the members' Dyson arguments of a step
+    DysonArgs *d_tab = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around the step's Dyson sweeps
+    int last_launches = 0, last_fused = 0;  // Dyson launches of the last step; members in grouped launches
+    double dyson_ms = 0.0;                  // the Dyson sweeps of every step so far
+    int64_t steps = 0;
 };
 
 namespace {
@@ -980,14 +1008,15 @@ bool ctl_steps_ok(const qs_handle *h) {
            h->d_dx && h->d_ctl && qs_update_gt_fits(h->nA, h->ncolS, h->n_vec, 16);
 }
 
-// one order-2 SCF step with its scalar logic on the device: the residual
+// One order-2 SCF step with its scalar logic on the device: the residual
 // (scf_residual2, src/scf.f90:158-188), the sanity check, the history pass, the
 // Gram solve and the update (broyden_solve, common/broyden_mixing.f90:51-184),
 // then a copy of the control header to pinned host memory.  No host decision
-// inside, so the same sequence can be captured as a graph.
-int enqueue_ctl_step(qs_handle *h, double V) {
+// inside, so the same sequence can be captured as a graph.  A group step
+// (qs_group_step) runs the part before the Dyson sweep and the part after it for
+// each member around one sweep launch for all of them.
+int enqueue_ctl_pre(qs_handle *h) {
     const int nA = h->nA;
-    const int64_t n = h->n_vec;
     const double pp = h->p.delta / (2.0 * PI);
     hipEvent_t t0 = prof_begin(h);
     HIPCHK(h, qs_launch_equal_time_sigma1(h->d_x, h->d_et_src, h->d_act_site, h->d_U_act, h->d_gt0, h->d_sig1, nA,
@@ -1003,12 +1032,14 @@ int enqueue_ctl_step(qs_handle *h, double V) {
         prof_end(h, ST_FFT_F1 + part, t0);
     }
     h->sigma_valid = true;
-    DysonArgs a = dyson_args(h, QS_M_SCF2, true, V);
     HIPCHK(h, qs_launch_ctl_status(h->d_status, h->stream));
-    t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream));
-    prof_end(h, ST_DYSON, t0);
-    t0 = prof_begin(h);
+    return QS_OK;
+}
+
+int enqueue_ctl_post(qs_handle *h) {
+    const int nA = h->nA;
+    const int64_t n = h->n_vec;
+    hipEvent_t t0 = prof_begin(h);
     HIPCHK(h, qs_launch_reduce_partials(h->d_partials, h->nw, &h->d_ctl->fmax, h->stream));
     HIPCHK(h, qs_launch_reduce_cols(h->d_diagR, nA, h->nw, h->d_csum, h->d_scratch, h->stream));
     HIPCHK(h, qs_launch_reduce_cols(h->d_diagL, nA, h->nw, h->d_csum + nA, h->d_scratch, h->stream));
@@ -1030,6 +1061,16 @@ int enqueue_ctl_step(qs_handle *h, double V) {
     prof_end(h, ST_UPDATE, t0);
     HIPCHK(h, hipMemcpyAsync(h->h_ctl, h->d_ctl, offsetof(QsCtrl, gamma), hipMemcpyDeviceToHost, h->stream));
     return QS_OK;
+}
+
+int enqueue_ctl_step(qs_handle *h, double V) {
+    int rc;
+    if ((rc = enqueue_ctl_pre(h))) return rc;
+    DysonArgs a = dyson_args(h, QS_M_SCF2, true, V);
+    hipEvent_t t0 = prof_begin(h);
+    HIPCHK(h, qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream));
+    prof_end(h, ST_DYSON, t0);
+    return enqueue_ctl_post(h);
 }
 
 // broyden_solve with the scalar logic on the device.  Steps 1 and 2 run as plain
@@ -1132,6 +1173,24 @@ int broyden_device_ctl(qs_handle *h, double V, int *converged, int *n_iter, doub
     release();
     *fmax_out = fmax;
     return QS_OK;
+}
+
+int gfail(qs_group *g, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g->err = buf;
+    return code;
+}
+
+// the member leaves: its handle gets its own stream back
+void group_release(qs_group *g, size_t i) {
+    qs_group::Member &m = g->mem[i];
+    m.h->stream = m.own;
+    m.h->group = nullptr;
+    g->mem.erase(g->mem.begin() + (long)i);
 }
 
 }  // namespace
@@ -1353,6 +1412,15 @@ int qs_create(const qs_problem *p, qs_handle **out) {
 
 void qs_destroy(qs_handle *h) {
     if (!h) return;
+    if (h->group) {
+        qs_group *g = h->group;
+        (void)hipStreamSynchronize(g->stream);
+        for (size_t i = 0; i < g->mem.size(); i++)
+            if (g->mem[i].h == h) {
+                group_release(g, i);
+                break;
+            }
+    }
     if (h->p.device >= 0) (void)hipSetDevice(h->p.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
@@ -1764,6 +1832,226 @@ int qs_dyson_choice(const qs_handle *h, int32_t *variant, double *ms_per_variant
     for (int i = 0; i < nmax && i < QS_DYSON_NSHAPES; i++)
         if (ms_per_variant) ms_per_variant[i] = h->dyson_tune_ms[i];
     return qs_dyson_variants(h->norb);
+}
+
+}  // extern "C"
+
+namespace {
+
+// the end of a member's solve as qs_solve_scf ends it: the final sweep with the
+// converged (or last) Sigma, then final_sanity (src/scf.f90:138-150,260-275)
+int group_finish(qs_group::Member &m, int converged, int runaway, qs_scf_result *res) {
+    qs_handle *h = m.h;
+    qs_scf_result r{};
+    r.converged = converged;
+    r.n_iter = m.its;
+    r.fmax = m.fmax;
+    r.runaway = runaway;
+    int rc = final_sweep(h, m.V);
+    if (rc == QS_OK) {
+        if (r.converged && h->unphys_final) r.converged = 0;
+        r.unphysical = h->unphys_final;
+        r.i_worst = h->iworst_final;
+        r.n_worst = h->nworst_final;
+        r.w_worst = h->wworst_final;
+    }
+    if (res) *res = r;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qs_group_create(int32_t device, qs_group **out) {
+    if (!out) return QS_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return QS_ERR_HIP;
+    qs_group *g = new qs_group;
+    g->device = device;
+    bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipHostMalloc((void **)&g->h_tab, sizeof(DysonArgs) * QS_GROUP_MAX, 0) == hipSuccess &&
+              hipMalloc((void **)&g->d_tab, sizeof(DysonArgs) * QS_GROUP_MAX) == hipSuccess &&
+              hipEventCreate(&g->ev[0]) == hipSuccess && hipEventCreate(&g->ev[1]) == hipSuccess;
+    if (!ok) {
+        qs_group_destroy(g);
+        return QS_ERR_HIP;
+    }
+    *out = g;
+    return QS_OK;
+}
+
+void qs_group_destroy(qs_group *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    while (!g->mem.empty()) group_release(g, g->mem.size() - 1);
+    for (hipEvent_t e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (g->d_tab) (void)hipFree(g->d_tab);
+    if (g->h_tab) (void)hipHostFree(g->h_tab);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+const char *qs_group_last_error(const qs_group *g) { return g ? g->err.c_str() : "null group"; }
+
+int qs_group_members(const qs_group *g) { return g ? (int)g->mem.size() : QS_ERR_ARG; }
+
+int qs_group_add(qs_group *g, qs_handle *h, double V, int32_t fresh) {
+    if (!g || !h) return QS_ERR_ARG;
+    if (h->group) return gfail(g, QS_ERR_STATE, "the handle is already a member of a group");
+    if ((int)g->mem.size() >= QS_GROUP_MAX) return gfail(g, QS_ERR_UNSUPPORTED, "a group holds at most %d solves", QS_GROUP_MAX);
+    if (h->p.device != g->device) return gfail(g, QS_ERR_ARG, "the handle is on device %d, the group on %d", h->p.device, g->device);
+    if (!(h->step_mode > 0 && ctl_steps_ok(h)))
+        return gfail(g, QS_ERR_UNSUPPORTED, "group steps need an order-2 FP64 solve with device-controlled steps "
+                                            "(FP32 history of at most 16 columns)");
+    HIPCHK(h, hipSetDevice(h->p.device));
+    qs_group::Member m;
+    m.h = h;
+    m.V = V;
+    m.own = h->stream;
+    h->stream = g->stream;
+    h->group = g;
+    h->V_scf = V;
+    h->final_valid = false;
+    h->graph_steps = 0;
+    h->steps32 = 0;
+    int rc = QS_OK;
+    // the fresh start (src/scf.f90:114-120), as qs_solve_scf
+    if (fresh || !h->x_valid) {
+        if ((rc = run_dyson(h, QS_M_INIT2, false, V)) == QS_OK && (rc = check_status(h)) == QS_OK) h->x_valid = true;
+    }
+    hipError_t e = hipSuccess;
+    if (rc == QS_OK && (e = qs_launch_ctl_reset(h->d_ctl, h->d_status, h->stream)) == hipSuccess) {
+        hipEvent_t t0 = prof_begin(h);
+        e = qs_launch_permute_in(h->d_x, h->d_Gt, h->d_blk_src, h->nA, h->ncolS, h->nw, h->stream);
+        prof_end(h, ST_PERMUTE, t0);
+    }
+    if (rc == QS_OK && e != hipSuccess) rc = fail(h, QS_ERR_HIP, "group start: %s", hipGetErrorString(e));
+    if (rc != QS_OK) {
+        (void)hipStreamSynchronize(h->stream);
+        h->stream = m.own;
+        h->group = nullptr;
+        return gfail(g, rc, "%s", h->err.c_str());
+    }
+    g->mem.push_back(m);
+    return QS_OK;
+}
+
+int qs_group_step(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
+                  int32_t *n_done) {
+    if (!g || !n_done) return QS_ERR_ARG;
+    *n_done = 0;
+    const int nm = (int)g->mem.size();
+    if (nm == 0) return QS_OK;
+    if (max_done < nm || !done || !res || !status)
+        return gfail(g, QS_ERR_ARG, "room for %d finished solves is needed", nm);
+    if (hipSetDevice(g->device) != hipSuccess) return gfail(g, QS_ERR_HIP, "hipSetDevice");
+    // a member whose launches fail leaves with the error; the others go on
+    std::vector<int> rcm(nm, QS_OK);
+    for (int i = 0; i < nm; i++) rcm[i] = enqueue_ctl_pre(g->mem[i].h);
+    // members that share a Dyson kernel instance sweep in one launch; the others
+    // (the large-Norb path, the shapes without a grouped form, a slab shorter than
+    // the grid) launch on their own
+    std::vector<std::pair<int, int>> order;  // (key, member)
+    for (int i = 0; i < nm; i++) {
+        if (rcm[i]) continue;
+        const qs_handle *h = g->mem[i].h;
+        int key = qs_dyson_group_key(h->norb, h->dyson_variant);
+        if (h->mscr_energies < h->nw) key = -1;
+        order.push_back({key, i});
+    }
+    std::stable_sort(order.begin(), order.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+        return a.first < b.first;
+    });
+    int ntab = 0;
+    for (auto &o : order)
+        if (o.first >= 0) g->h_tab[ntab++] = dyson_args(g->mem[o.second].h, QS_M_SCF2, true, g->mem[o.second].V);
+    hipError_t e = hipSuccess;
+    if (ntab) e = hipMemcpyAsync(g->d_tab, g->h_tab, sizeof(DysonArgs) * ntab, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) e = hipEventRecord(g->ev[0], g->stream);
+    g->last_launches = 0;
+    g->last_fused = ntab;
+    int t = 0;
+    for (size_t k = 0; k < order.size() && e == hipSuccess;) {
+        const int key = order[k].first;
+        size_t k2 = k;
+        while (k2 < order.size() && order[k2].first == key) k2++;
+        if (key >= 0) {
+            const int cnt = (int)(k2 - k);
+            e = qs_launch_dyson_group(g->h_tab + t, g->d_tab + t, cnt, g->mem[order[k].second].h->dyson_variant,
+                                      g->stream);
+            t += cnt;
+            g->last_launches++;
+        } else {
+            for (size_t j = k; j < k2 && e == hipSuccess; j++) {
+                qs_handle *h = g->mem[order[j].second].h;
+                e = qs_launch_dyson_variant(dyson_args(h, QS_M_SCF2, true, g->mem[order[j].second].V), h->nw,
+                                            h->dyson_variant, g->stream);
+                g->last_launches++;
+            }
+        }
+        k = k2;
+    }
+    if (e == hipSuccess) e = hipEventRecord(g->ev[1], g->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(g->stream);
+        return gfail(g, QS_ERR_HIP, "group Dyson sweep: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < nm; i++)
+        if (!rcm[i]) rcm[i] = enqueue_ctl_post(g->mem[i].h);
+    e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) return gfail(g, QS_ERR_HIP, "group step: %s", hipGetErrorString(e));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g->ev[0], g->ev[1]) == hipSuccess) g->dyson_ms += ms;
+    g->steps++;
+    // each member's step outcome, as in broyden_device_ctl
+    std::vector<char> leave(nm, 0);
+    for (int i = 0; i < nm; i++) {
+        qs_group::Member &m = g->mem[i];
+        qs_handle *h = m.h;
+        if (h->prof.on) prof_collect(h);
+        int fin = 0, conv = 0, runaway = 0, rc = rcm[i];
+        if (rc == QS_OK) {
+            m.its++;
+            const QsCtrl &c = *h->h_ctl;
+            if (c.singular) {
+                rc = fail(h, QS_ERR_SINGULAR, "singular matrix in the Dyson inverse (zgetrf INFO != 0)");
+            } else if (c.abort) {
+                fin = 1;
+                runaway = 1;
+            } else {
+                m.fmax = c.fmax;
+                if (c.converged) fin = conv = 1;
+                else if (m.its >= h->p.maxscf) fin = 1;
+                else {
+                    std::swap(h->d_x, h->d_xold);  // x <- x_new, xold <- x
+                    std::swap(h->d_F, h->d_Fold);  // Fold <- F
+                }
+            }
+        }
+        if (rc == QS_OK && !fin) continue;
+        const int k = *n_done;
+        done[k] = h;
+        res[k] = qs_scf_result{};
+        if (rc == QS_OK) rc = group_finish(m, conv, runaway, &res[k]);
+        status[k] = rc;
+        (*n_done)++;
+        leave[i] = 1;
+    }
+    for (int i = nm - 1; i >= 0; i--)
+        if (leave[i]) group_release(g, (size_t)i);
+    return QS_OK;
+}
+
+int qs_group_info(const qs_group *g, int32_t *launches, int32_t *fused_members, double *dyson_ms, int64_t *steps) {
+    if (!g) return QS_ERR_ARG;
+    if (launches) *launches = g->last_launches;
+    if (fused_members) *fused_members = g->last_fused;
+    if (dyson_ms) *dyson_ms = g->dyson_ms;
+    if (steps) *steps = g->steps;
+    return QS_OK;
 }
 
 }  // extern "C"

@@ -94,6 +94,19 @@ hipError_t qs_launch_cgemm(int np, int count, const void *A, const void *B, void
 // part and the Keldysh part as two kernels through the slab (17..96 orbitals)
 hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, bool split, hipStream_t s);
 size_t qs_dyson_wave_lds_bytes(int norb, int nCL);
+// One launch over the whole sweeps of several solves (a group step).  Member m
+// owns workgroups [first[m], first[m + 1]) of each part's launch; its arguments
+// are tab[m] in device memory.
+constexpr int QS_GROUP_MAX = 64;
+struct DysonGroupIdx { int nmem; int first[QS_GROUP_MAX + 1]; };
+// the kernel instance a size runs under launch shape `variant` (2 or 4): members
+// with equal keys can share a launch, whatever their orbital counts and grids; -1
+// when the shape has no grouped form
+int qs_dyson_group_key(int norb, int variant);
+// ht: the members' arguments on the host (sizes, checks), dt: the same in device
+// memory.  Each member sweeps its nw energies (iw0 = 0) through its own slab,
+// which must hold them all (mscr_energies >= nw).
+hipError_t qs_launch_dyson_group(const DysonArgs *ht, const DysonArgs *dt, int nmem, int variant, hipStream_t s);
 size_t qs_dyson_wave_scratch_bytes(int norb);
 int qs_dyson_wave_max_norb();
 

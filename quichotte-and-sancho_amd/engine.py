@@ -106,6 +106,14 @@ def load_library(path=SO):
     L.qs_sancho_self_energies_static.argtypes = [I32, I32, I32, D, P, P, P, P, P, P, P, P, P, P]
     L.qs_sancho_profile.argtypes = [P, I32]
     L.qs_sancho_profile_read.argtypes = [P, P, P]
+    L.qs_group_create.argtypes = [I32, ctypes.POINTER(P)]
+    L.qs_group_destroy.argtypes = [P]
+    L.qs_group_last_error.argtypes = [P]
+    L.qs_group_last_error.restype = ctypes.c_char_p
+    L.qs_group_add.argtypes = [P, P, D, I32]
+    L.qs_group_members.argtypes = [P]
+    L.qs_group_step.argtypes = [P, I32, P, P, P, P]
+    L.qs_group_info.argtypes = [P, P, P, P, P]
     _lib = L
     return L
 
@@ -374,6 +382,71 @@ class ScfEngine:
 
 
 _hip = None
+
+
+class ScfGroup:
+    """qs_group_*: several order-2 solves stepped together on one stream, with the
+    Dyson sweeps of members that share a kernel instance in one launch.  Members
+    are ScfEngine objects; add() starts a solve, step() runs one SCF step of every
+    member and returns the solves that ended as (engine, ScfResult, status) with
+    status 0 or the library's error code.  A finished engine's observables are
+    read as after ScfEngine.solve_scf.  One thread drives a group."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        self.L = L
+        g = ctypes.c_void_p()
+        rc = L.qs_group_create(int(device), ctypes.byref(g))
+        if rc != 0:
+            raise QsError(rc, "qs_group_create failed")
+        self.g = g
+        self._members = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise QsError(rc, self.L.qs_group_last_error(self.g).decode())
+
+    def add(self, eng: "ScfEngine", V, fresh=True):
+        self._check(self.L.qs_group_add(self.g, eng.h, float(V), int(bool(fresh))))
+        self._members[eng.h.value] = eng
+
+    def __len__(self):
+        return self.L.qs_group_members(self.g)
+
+    def step(self):
+        nm = max(len(self), 1)
+        done = (ctypes.c_void_p * nm)()
+        res = (_Result * nm)()
+        st = (ctypes.c_int32 * nm)()
+        nd = ctypes.c_int32()
+        self._check(self.L.qs_group_step(self.g, nm, done, res, st, ctypes.byref(nd)))
+        out = []
+        for k in range(nd.value):
+            eng = self._members.pop(done[k])
+            r = res[k]
+            out.append((eng, ScfResult(bool(r.converged), r.n_iter, r.fmax, bool(r.runaway), bool(r.unphysical),
+                                       r.i_worst, r.n_worst, r.w_worst), int(st[k])))
+        return out
+
+    def info(self):
+        """(Dyson launches of the last step, members in grouped launches, ms of all
+        steps' Dyson sweeps, steps)"""
+        a, b, s = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        ms = ctypes.c_double()
+        self._check(self.L.qs_group_info(self.g, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms), ctypes.byref(s)))
+        return a.value, b.value, ms.value, s.value
+
+    def close(self):
+        if getattr(self, "g", None):
+            self.L.qs_group_destroy(self.g)
+            self.g = None
+            self._members = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def hip_runtime():

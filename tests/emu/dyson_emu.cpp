@@ -9,6 +9,8 @@
 //        qs_dyson_wave.hip as the source: the small-LDS kernel's own launch)
 //        dyson_emu <problem.bin> <out.bin> split 0 0     (the same kernel as its
 //        Gauss-Jordan and Keldysh parts, two launches through the slab)
+//        dyson_emu <p1.bin> <o1.bin> group|gsplit <p2.bin> <o2.bin>  (two problems
+//        as the members of one grouped launch, fused or split)
 // problem.bin: int32 n, count, mode, use_sigma, nCL; float64 w_init, delta,
 // mu, beta, V; complex128 Hemb (n*n), GamL (n*n), GamR (n*n), GamL_cl
 // (nCL*nCL); int32 cl (nCL); complex128 sig1 (n/2*4); complex128 sig
@@ -42,68 +44,117 @@ static void wr(FILE *f, const std::vector<T> &v) {
     fwrite(v.data(), sizeof(T), v.size(), f);
 }
 
-int main(int argc, char **argv) {
-    if (argc != 6) return 2;
-    FILE *f = fopen(argv[1], "rb");
-    if (!f) return 2;
+// one problem file: the arrays it holds, and the kernel arguments over them
+struct Problem {
+    int n = 0, count = 0, mode = 0;
+    std::vector<double2> Hemb, gl, gr, glcl, sig1, sig, onsL, mscr;
+    std::vector<int> cl, act, pos, suppos, tri, status;
+    std::vector<double> x, Fold, xout, partials, diagR, diagL, dosd, trUD;
+    DysonArgs a{};
+};
+
+static bool load(const char *path, Problem &P) {
+    FILE *f = fopen(path, "rb");
+    if (!f) return false;
     auto hi = rd<int>(f, 5);
     const int n = hi[0], count = hi[1], mode = hi[2], use_sigma = hi[3], nCL = hi[4];
+    P.n = n; P.count = count; P.mode = mode;
     auto hd = rd<double>(f, 5);
     // Hemb keeps the library's zero pad of 64 entries past n^2
-    auto Hemb = rd<double2>(f, (size_t)n * n);
-    Hemb.resize((size_t)n * n + 64, make_double2(0.0, 0.0));
-    auto gl = rd<double2>(f, (size_t)n * n), gr = rd<double2>(f, (size_t)n * n);
-    auto glcl = rd<double2>(f, (size_t)nCL * nCL);
-    auto cl = rd<int>(f, nCL);
+    P.Hemb = rd<double2>(f, (size_t)n * n);
+    P.Hemb.resize((size_t)n * n + 64, make_double2(0.0, 0.0));
+    P.gl = rd<double2>(f, (size_t)n * n);
+    P.gr = rd<double2>(f, (size_t)n * n);
+    P.glcl = rd<double2>(f, (size_t)nCL * nCL);
+    P.cl = rd<int>(f, nCL);
     const int ns = n / 2, nAs = ns, nA = n, nA2 = n * n;
-    auto sig1 = rd<double2>(f, (size_t)ns * 4);
+    P.sig1 = rd<double2>(f, (size_t)ns * 4);
     const int sig_ld = (12 * (nAs * (nAs + 1) / 2) + 7) / 8 * 8;
-    std::vector<double2> sig;
-    if (use_sigma) sig = rd<double2>(f, (size_t)count * sig_ld);
-    std::vector<double> x, Fold;
+    if (use_sigma) P.sig = rd<double2>(f, (size_t)count * sig_ld);
     if (mode == QS_M_SCF2) {
-        x = rd<double>(f, (size_t)count * 3 * nA2);
-        Fold = rd<double>(f, (size_t)count * 3 * nA2);
+        P.x = rd<double>(f, (size_t)count * 3 * nA2);
+        P.Fold = rd<double>(f, (size_t)count * 3 * nA2);
     }
     fclose(f);
-    std::vector<int> act(n), pos(n), suppos(n), tri(nA2);
-    for (int i = 0; i < n; i++) act[i] = pos[i] = suppos[i] = i;
+    P.act.resize(n); P.pos.resize(n); P.suppos.resize(n); P.tri.resize(nA2);
+    for (int i = 0; i < n; i++) P.act[i] = P.pos[i] = P.suppos[i] = i;
     int e = 0;
     for (int i = 0; i < nA; i++)
-        for (int j = i + 1; j < nA; j++) tri[e++] = (i << 16) | j;
+        for (int j = i + 1; j < nA; j++) P.tri[e++] = (i << 16) | j;
     for (int i = 0; i < nA; i++)
-        for (int j = i; j < nA; j++) tri[e++] = (i << 16) | j;
+        for (int j = i; j < nA; j++) P.tri[e++] = (i << 16) | j;
     const double qnan = nan("");
-    std::vector<double> xout((size_t)count * 3 * nA2, qnan), partials((size_t)count * 4, qnan);
-    std::vector<double> diagR((size_t)count * nA, qnan), diagL((size_t)count * nA, qnan);
-    std::vector<double2> onsL((size_t)count * 4 * ns, make_double2(qnan, qnan));
-    std::vector<double> dosd((size_t)count * n, qnan), trUD((size_t)count * 2, qnan);
-    std::vector<int> status(1, 0);
+    P.xout.assign((size_t)count * 3 * nA2, qnan);
+    P.partials.assign((size_t)count * 4, qnan);
+    P.diagR.assign((size_t)count * nA, qnan);
+    P.diagL.assign((size_t)count * nA, qnan);
+    P.onsL.assign((size_t)count * 4 * ns, make_double2(qnan, qnan));
+    P.dosd.assign((size_t)count * n, qnan);
+    P.trUD.assign((size_t)count * 2, qnan);
+    P.status.assign(1, 0);
 
-    DysonArgs a{};
+    DysonArgs &a = P.a;
     a.norb = n; a.nA = nA; a.nS = n; a.nCL = nCL; a.nsites = ns; a.nAs = nAs; a.n_atoms = 0; a.atom_tot = 0;
     a.nw = count; a.iw0 = 0;
     a.w_init = hd[0]; a.delta = hd[1]; a.mu = hd[2]; a.beta = hd[3]; a.V = hd[4];
     a.mode = mode; a.use_sigma = use_sigma; a.order = 2;
-    a.Hemb = Hemb.data(); a.sig1 = sig1.data(); a.sig = use_sigma ? sig.data() : nullptr; a.sig_ld = sig_ld;
-    a.gamL_s = gl.data(); a.gamR_s = gr.data(); a.gamL_cl = glcl.data();
-    a.act = act.data(); a.cl = cl.data(); a.pos = pos.data(); a.suppos = suppos.data(); a.tri_map = tri.data();
+    a.Hemb = P.Hemb.data(); a.sig1 = P.sig1.data(); a.sig = use_sigma ? P.sig.data() : nullptr; a.sig_ld = sig_ld;
+    a.gamL_s = P.gl.data(); a.gamR_s = P.gr.data(); a.gamL_cl = P.glcl.data();
+    a.act = P.act.data(); a.cl = P.cl.data(); a.pos = P.pos.data(); a.suppos = P.suppos.data();
+    a.tri_map = P.tri.data();
     a.atom_first = a.atom_norb = a.atom_off = nullptr;
-    a.x = x.empty() ? nullptr : x.data();
-    a.xout = mode == QS_M_INIT2 ? xout.data() : nullptr;
-    a.F = mode == QS_M_SCF2 ? xout.data() : nullptr;
-    a.Fold = Fold.empty() ? nullptr : Fold.data();
-    a.partials = partials.data();
-    a.diagR = diagR.data(); a.diagL = diagL.data(); a.onsL = onsL.data(); a.dosd = dosd.data();
-    a.atomL = nullptr; a.trUD = trUD.data(); a.status = status.data();
+    a.x = P.x.empty() ? nullptr : P.x.data();
+    a.xout = mode == QS_M_INIT2 ? P.xout.data() : nullptr;
+    a.F = mode == QS_M_SCF2 ? P.xout.data() : nullptr;
+    a.Fold = P.Fold.empty() ? nullptr : P.Fold.data();
+    a.partials = P.partials.data();
+    a.diagR = P.diagR.data(); a.diagL = P.diagL.data(); a.onsL = P.onsL.data(); a.dosd = P.dosd.data();
+    a.atomL = nullptr; a.trUD = P.trUD.data(); a.status = P.status.data();
+#ifdef QS_EMU_WAVE
+    // the small-LDS kernel's slab holds M and the pivot maps per energy
+    P.mscr.assign(qs_dyson_wave_scratch_bytes(n) / sizeof(double2) * count, make_double2(qnan, qnan));
+    a.mscr = P.mscr.data();
+    a.mscr_energies = count;
+#endif
+    return true;
+}
 
+static bool save(const char *path, const Problem &P) {
+    FILE *o = fopen(path, "wb");
+    if (!o) return false;
+    wr(o, P.xout); wr(o, P.partials); wr(o, P.diagR); wr(o, P.diagL); wr(o, P.onsL); wr(o, P.dosd); wr(o, P.trUD);
+    wr(o, P.status);
+    fclose(o);
+    return true;
+}
+
+int main(int argc, char **argv) {
+    if (argc != 6) return 2;
+#ifdef QS_EMU_WAVE
+    // group / gsplit: the problems of argv[1] and argv[4] as the two members of
+    // one fused launch (launch shape 2 / 4); outputs to argv[2] and argv[5]
+    const std::string first = argv[3];
+    if (first == "group" || first == "gsplit") {
+        Problem P[2];
+        if (!load(argv[1], P[0]) || !load(argv[4], P[1])) return 2;
+        DysonArgs tab[2] = {P[0].a, P[1].a};
+        const hipError_t rc = qs_launch_dyson_group(tab, tab, 2, first == "group" ? QS_DYSON_WAVE : QS_DYSON_SPLIT,
+                                                    nullptr);
+        if (rc != hipSuccess) {
+            fprintf(stderr, "grouped launch failed: %d\n", rc);
+            return 3;
+        }
+        return save(argv[2], P[0]) && save(argv[5], P[1]) ? 0 : 2;
+    }
+#endif
+    Problem P;
+    if (!load(argv[1], P)) return 2;
+    const int n = P.n, count = P.count;
+    DysonArgs &a = P.a;
     const std::string shape = std::string(argv[3]) + "," + argv[4] + "," + argv[5];
     hipError_t rc = hipErrorInvalidValue;
 #ifdef QS_EMU_WAVE
-    // the small-LDS kernel picks its tile from n; its slab holds M per energy
-    std::vector<double2> mscr(qs_dyson_wave_scratch_bytes(n) / sizeof(double2) * count);
-    a.mscr = mscr.data();
-    a.mscr_energies = count;
+    (void)n;
     if (shape == "wave,0,0") rc = qs_launch_dyson_wave(a, count, false, nullptr);
     else if (shape == "split,0,0") rc = qs_launch_dyson_wave(a, count, true, nullptr);
 #else
@@ -117,9 +168,5 @@ int main(int argc, char **argv) {
         fprintf(stderr, "launch of shape %s failed: %d\n", shape.c_str(), rc);
         return 3;
     }
-    FILE *o = fopen(argv[2], "wb");
-    if (!o) return 2;
-    wr(o, xout); wr(o, partials); wr(o, diagR); wr(o, diagL); wr(o, onsL); wr(o, dosd); wr(o, trUD); wr(o, status);
-    fclose(o);
-    return 0;
+    return save(argv[2], P) ? 0 : 2;
 }

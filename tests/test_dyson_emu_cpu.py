@@ -206,3 +206,25 @@ def test_no_unordered_lds_access_thread_sanitizer(builds, tmp_path, kernel, n, s
     o = read_out(tmp_path / "o.bin", n, 1)
     for k in written(o, mode):
         assert not np.isnan(o[k]).any(), k
+
+
+@pytest.mark.parametrize("sizes,kind", [((80, 72), "group"), ((80, 66), "gsplit"), ((16, 10), "group"),
+                                        ((44, 40), "gsplit")])
+@pytest.mark.parametrize("mode", [M_INIT2, M_SCF2, M_FINAL])
+def test_grouped_launch_gives_each_member_its_own_bits(builds, tmp_path, sizes, kind, mode):
+    """Two solves of different orbital counts and grid lengths, one tile class, in one
+    grouped launch (fused or split): each member's outputs equal its own launch bit
+    for bit, and the emulated LDS and buffers stay in bounds (AddressSanitizer)."""
+    exes, _ = builds
+    exe = exes[("k_dyson_w", "address")]
+    solo = "wave" if kind == "group" else "split"
+    counts = (2, 3)
+    for k, (n, c) in enumerate(zip(sizes, counts)):
+        make_problem(tmp_path / ("p%d.bin" % k), n, c, mode, use_sigma=int(mode != M_INIT2), seed=n + k,
+                     V=0.3 + 0.2 * k)
+        run(exe, tmp_path / ("p%d.bin" % k), tmp_path / ("s%d.bin" % k), (solo, "0", "0"))
+    run(exe, tmp_path / "p0.bin", tmp_path / "g0.bin", (kind, str(tmp_path / "p1.bin"), str(tmp_path / "g1.bin")))
+    for k, (n, c) in enumerate(zip(sizes, counts)):
+        s = read_out(tmp_path / ("s%d.bin" % k), n, c)
+        g = read_out(tmp_path / ("g%d.bin" % k), n, c)
+        compare(s, g, written(s, mode), 0.0)
