@@ -341,6 +341,177 @@ def serve_bench(args):
     return 0
 
 
+# batched bias points of one system: (helix ions, turns, U, energy window, V window, default log2 grid, points)
+BATCH_WORKLOADS = {"helix16": ((4, 2), 1.0, 10.0, (0.0, 1.0), 14, 64),
+                   "helix80": ((8, 5), 2.0, 12.0, (-1.0, 1.0), 12, 32)}
+
+
+def batch_bench(args):
+    """Many independent bias points of one system (seeded V, fresh starts, tolf 1e-8,
+    nhist 16), solved three ways on the same points: one at a time on one handle (the
+    plain path), by B host threads with a handle and stream each, and by a group of B
+    handles stepped together (qs_group_*: one Dyson launch per step for all members,
+    a finished point's handle takes the next point at once)."""
+    import importlib
+    import threading
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    (ni, nt), U, wmax, (v0, v1), lg_def, np_def = BATCH_WORKLOADS[args.batch]
+    lg = args.batch_log2nw or lg_def
+    npts = args.batch_points or np_def
+    B = args.batch_size
+    H = helix_H(ni, nt)
+    n = H.shape[0]
+    GL = np.zeros((n, n), complex)
+    GR = np.zeros((n, n), complex)
+    for i in (0, 1):
+        GL[i, i] = 0.1
+        GR[n - 2 + i, n - 2 + i] = 0.1
+    Us = np.zeros(n)
+    Us[0::2] = U
+    nw = 1 << lg
+    prob = pkg.ScfProblem(H=H, gammaL=GL, gammaR=GR, U_site=Us, order=2, n_omega=nw, w_init=-wmax,
+                          delta=2 * wmax / nw, beta=1.0 / (KB * 300.0), mu=0.0, nhist=16, tolf=1e-8)
+    Vs = [float(v) for v in np.random.default_rng(3).uniform(v0, v1, npts)]
+    desc = ("%d bias points of the helix %dx%d (%d spin-orbitals), U=%g, order 2, 2^%d energies on (-%g,%g], "
+            "V ~ U[%g,%g] (seed 3), fresh starts, tolf 1e-8, Broyden nhist 16 FP32 history"
+            % (npts, ni, nt, n, U, lg, wmax, wmax, v0, v1))
+    print("bench: batch: %s; %d in flight" % (desc, B), file=sys.stderr, flush=True)
+    engines = [pkg.ScfEngine(prob) for _ in range(B)]
+    print("bench: batch: %d handles, %.1f GB each, Dyson launch shape %s" %
+          (B, engines[0].device_bytes() / 1e9, engines[0].dyson_shape_name()), file=sys.stderr, flush=True)
+    # warm-up: code loading and graph-free first launches on every handle
+    for e in engines:
+        e.solve_scf(Vs[0], True)
+
+    def plain():
+        e = engines[0]
+        res = []
+        t0 = time.perf_counter()
+        for V in Vs:
+            r = e.solve_scf(V, True)
+            res.append((r, e.current(), time.perf_counter() - t0))
+        return time.perf_counter() - t0, res
+
+    def threads():
+        res = [None] * npts
+        nxt = [0]
+        lock = threading.Lock()
+        t0 = time.perf_counter()
+
+        def worker(e):
+            while True:
+                with lock:
+                    k = nxt[0]
+                    nxt[0] += 1
+                if k >= npts:
+                    return
+                r = e.solve_scf(Vs[k], True)
+                res[k] = (r, e.current(), time.perf_counter() - t0)
+
+        ts = [threading.Thread(target=worker, args=(e,)) for e in engines]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return time.perf_counter() - t0, res
+
+    def group():
+        g = pkg.ScfGroup(0)
+        res = [None] * npts
+        point = {}
+        k = 0
+        t0 = time.perf_counter()
+        for e in engines:
+            if k < npts:
+                g.add(e, Vs[k], True)
+                point[id(e)] = k
+                k += 1
+        steps = 0
+        while len(g):
+            for e, r, st in g.step():
+                if st != 0:
+                    raise RuntimeError("group member failed: %d" % st)
+                res[point[id(e)]] = (r, e.current(), time.perf_counter() - t0)
+                if k < npts:
+                    g.add(e, Vs[k], True)
+                    point[id(e)] = k
+                    k += 1
+            steps += 1
+        total = time.perf_counter() - t0
+        launches, fused, dyson_ms, gsteps = g.info()
+        g.close()
+        return total, res, {"group_steps": steps, "dyson_ms": dyson_ms}
+
+    def two_groups():
+        # two groups of B/2, each step begun while the other group's step runs
+        gs = [pkg.ScfGroup(0), pkg.ScfGroup(0)]
+        res = [None] * npts
+        point = {}
+        k = 0
+        t0 = time.perf_counter()
+        for j, e in enumerate(engines):
+            if k < npts:
+                gs[j % 2].add(e, Vs[k], True)
+                point[id(e)] = (k, j % 2)
+                k += 1
+        for g in gs:
+            g.step_begin()
+        live = [True, True]
+        while any(live):
+            for j, g in enumerate(gs):
+                if not live[j]:
+                    continue
+                for e, r, st in g.step_end():
+                    if st != 0:
+                        raise RuntimeError("group member failed: %d" % st)
+                    res[point[id(e)][0]] = (r, e.current(), time.perf_counter() - t0)
+                    if k < npts:
+                        g.add(e, Vs[k], True)
+                        point[id(e)] = (k, j)
+                        k += 1
+                if len(g):
+                    g.step_begin()
+                else:
+                    live[j] = False
+        total = time.perf_counter() - t0
+        for g in gs:
+            g.close()
+        return total, res
+
+    out = {}
+    for name, fn in (("plain", plain), ("threads", threads), ("group", group), ("two_groups", two_groups)):
+        r = fn()
+        total, res = r[0], r[1]
+        ok = [x for x in res if x[0].converged]
+        lat = np.array([x[2] for x in res])
+        out[name] = {"value": len(ok) / total, "wall_s": total, "converged": len(ok), "points": npts,
+                     "scf_steps_per_point": float(np.mean([x[0].n_iter for x in res])),
+                     "completion_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
+                                       "p99": 1e3 * float(np.percentile(lat, 99))},
+                     "J": [list(x[1]) for x in res]}
+        if len(r) > 2:
+            out[name].update(r[2])
+        print("bench: batch %s: %d/%d converged in %.2f s, %.2f points/s, %.1f steps per point" %
+              (name, len(ok), npts, total, len(ok) / total, out[name]["scf_steps_per_point"]),
+              file=sys.stderr, flush=True)
+    Jp = np.array(out["plain"]["J"])
+    same = {nm: bool(np.array_equal(np.array(out[nm]["J"]), Jp)) for nm in ("threads", "group", "two_groups")}
+    for nm in out:
+        del out[nm]["J"]
+    line = {"metric": "batched bias points: converged bias points/s", "value": out["group"]["value"],
+            "unit": "bias points/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": desc, "in_flight": B,
+                       "form": "group of %d handles stepped together, one Dyson launch per step" % B},
+            "plain": out["plain"], "threads": out["threads"], "group": out["group"], "two_groups": out["two_groups"],
+            "speedup_vs_plain": out["group"]["value"] / out["plain"]["value"],
+            "threads_speedup_vs_plain": out["threads"]["value"] / out["plain"]["value"],
+            "currents_equal_to_plain": same}
+    for e in engines:
+        e.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def chain_gf(N, dw, up, delta, E=2.0, t=-0.4, gamma=1.0, mu=2.0, T=300.0):
     """Non-interacting G^r, G^< of SANCHO's chain (E_CC on every site, t_hop
     between neighbours, GammaL on site 1, GammaR on site N) on its grid
@@ -468,6 +639,11 @@ def make_parser():
     ap.add_argument("--serve-sizes", default="16,48,80,160,240", help="spin-orbital counts of the serving stream")
     ap.add_argument("--maxscf", type=int, default=0,
                     help="cap on SCF steps per bias point (counter-collection passes; 0 = the workload's 300)")
+    ap.add_argument("--batch", default="", choices=[""] + list(BATCH_WORKLOADS),
+                    help="time independent bias points of this system: one at a time, threads, and a group")
+    ap.add_argument("--batch-points", type=int, default=0, help="bias points of --batch (default per system)")
+    ap.add_argument("--batch-log2nw", type=int, default=0, help="log2 of the --batch grid (default per system)")
+    ap.add_argument("--batch-size", type=int, default=16, help="solves in flight for --batch")
     ap.add_argument("--sancho", default="", choices=[""] + list(SANCHO_WORKLOADS),
                     help="time SANCHO's order-2/3 self-energy build on this workload instead")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -490,6 +666,8 @@ def main():
         return serve_bench(args)
     if args.sancho:
         return sancho_bench(args)
+    if args.batch:
+        return batch_bench(args)
     if args.calibrate_copy:
         return copy_calibration()
     import importlib

@@ -140,12 +140,13 @@ int qs_step_info(const qs_handle *h, int32_t *device_steps_active, int32_t *grap
  * No counterpart in the reference, whose voltage loop solves one bias point at a
  * time (src/quichotte.f90:125-143).  For callers with many independent bias
  * points (a sweep split over several handles, a serving loop): a group runs one
- * SCF step of every member per qs_group_step, on one stream.  Each member's
- * step is its own solve_scf step (src/scf.f90:78-151; residual src/scf.f90:158-188,
+ * SCF step of every member per qs_group_step.  Each member's step is its own
+ * solve_scf step (src/scf.f90:78-151; residual src/scf.f90:158-188,
  * broyden_solve common/broyden_mixing.f90:51-184) with device-controlled scalars
- * (step mode 1), except that the Dyson/Keldysh sweeps of all members that share a
- * kernel instance (same register tile and launch shape: orbital counts may differ
- * within a tile class, grid lengths may differ) run as one launch.  A member's
+ * (step mode 1) on its handle's stream, except that the Dyson/Keldysh sweeps of
+ * all members that share a kernel instance (same register tile and launch shape:
+ * orbital counts may differ within a tile class, grid lengths may differ) run as
+ * one launch on the group's stream, between the members' stages.  A member's
  * results equal those of qs_solve_scf on its handle bit for bit.  Solves join
  * (qs_group_add) and leave (when they end) between steps, so a loop can keep the
  * group full as requests arrive.  One thread drives a group; while a handle is a
@@ -171,6 +172,13 @@ int qs_group_members(const qs_group *g);
  * handle are read as after qs_solve_scf. */
 int qs_group_step(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
                   int32_t *n_done);
+/* qs_group_step in two halves: _begin enqueues the step and returns at once, _end
+ * waits for it and reports as qs_group_step.  A caller with two groups can begin
+ * one while the other's step runs, so that one group's Dyson sweep overlaps the
+ * other's memory-bound stages.  No qs_group_add between the halves. */
+int qs_group_step_begin(qs_group *g);
+int qs_group_step_end(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
+                      int32_t *n_done);
 /* measurement: Dyson launches of the last step and how many members ran in
  * grouped launches; HIP-event time of all steps' Dyson sweeps and the step count */
 int qs_group_info(const qs_group *g, int32_t *launches, int32_t *fused_members, double *dyson_ms, int64_t *steps);

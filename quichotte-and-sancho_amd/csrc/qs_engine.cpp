@@ -145,11 +145,12 @@ struct qs_handle {
     qs_group *group = nullptr;      // the group stepping this handle's solve (qs_group_add)
 };
 
-// Several order-2 solves stepped together on one stream (qs_group_*): each step
-// runs every member's Sigma1/Sigma2 stage, then one Dyson sweep launch per kernel
-// instance for all members that share it, then every member's sanity check and
-// Broyden passes.  A member keeps its own handle state; its handle's stream is
-// set to the group's while it is a member.
+// Several order-2 solves stepped together (qs_group_*): each step runs every
+// member's Sigma1/Sigma2 stage on the member's own stream, then, on the group's
+// stream once those have finished, one Dyson sweep launch per kernel instance for
+// all members that share it, then every member's sanity check and Broyden passes
+// on its own stream again.  Members' stages overlap on the device; the shared
+// sweep is one launch.
 struct qs_group {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -159,15 +160,17 @@ struct qs_group {
         double V = 0.0;
         int its = 0;          // residuals of this solve so far
         double fmax = 0.0;    // max|F| of the last residual that was not aborted
-        hipStream_t own = nullptr;
+        hipEvent_t ev = nullptr;  // the end of the member's stage before the shared sweep
     };
     std::vector<Member> mem;
     DysonArgs *h_tab = nullptr;  // pinned: the members' Dyson arguments of a step
     DysonArgs *d_tab = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};  // around the step's Dyson sweeps
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around the step's shared Dyson sweeps
     int last_launches = 0, last_fused = 0;  // Dyson launches of the last step; members in grouped launches
-    double dyson_ms = 0.0;                  // the Dyson sweeps of every step so far
+    double dyson_ms = 0.0;                  // the shared Dyson sweeps of every step so far
     int64_t steps = 0;
+    bool in_step = false;                   // a step was begun and not yet ended
+    std::vector<int> rc_step;               // each member's launch status in that step
 };
 
 namespace {
@@ -1185,12 +1188,13 @@ int gfail(qs_group *g, int code, const char *fmt, ...) {
     return code;
 }
 
-// the member leaves: its handle gets its own stream back
+// the member leaves the group (its stream and the group's are idle)
 void group_release(qs_group *g, size_t i) {
     qs_group::Member &m = g->mem[i];
-    m.h->stream = m.own;
+    if (m.ev) (void)hipEventDestroy(m.ev);
     m.h->group = nullptr;
     g->mem.erase(g->mem.begin() + (long)i);
+    if (i < g->rc_step.size()) g->rc_step.erase(g->rc_step.begin() + (long)i);
 }
 
 }  // namespace
@@ -1415,6 +1419,7 @@ void qs_destroy(qs_handle *h) {
     if (h->group) {
         qs_group *g = h->group;
         (void)hipStreamSynchronize(g->stream);
+        (void)hipStreamSynchronize(h->stream);
         for (size_t i = 0; i < g->mem.size(); i++)
             if (g->mem[i].h == h) {
                 group_release(g, i);
@@ -1885,6 +1890,7 @@ void qs_group_destroy(qs_group *g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (auto &m : g->mem) (void)hipStreamSynchronize(m.h->stream);
     while (!g->mem.empty()) group_release(g, g->mem.size() - 1);
     for (hipEvent_t e : g->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1901,6 +1907,7 @@ int qs_group_members(const qs_group *g) { return g ? (int)g->mem.size() : QS_ERR
 int qs_group_add(qs_group *g, qs_handle *h, double V, int32_t fresh) {
     if (!g || !h) return QS_ERR_ARG;
     if (h->group) return gfail(g, QS_ERR_STATE, "the handle is already a member of a group");
+    if (g->in_step) return gfail(g, QS_ERR_STATE, "solves join between steps (a step is in flight)");
     if ((int)g->mem.size() >= QS_GROUP_MAX) return gfail(g, QS_ERR_UNSUPPORTED, "a group holds at most %d solves", QS_GROUP_MAX);
     if (h->p.device != g->device) return gfail(g, QS_ERR_ARG, "the handle is on device %d, the group on %d", h->p.device, g->device);
     if (!(h->step_mode > 0 && ctl_steps_ok(h)))
@@ -1910,8 +1917,7 @@ int qs_group_add(qs_group *g, qs_handle *h, double V, int32_t fresh) {
     qs_group::Member m;
     m.h = h;
     m.V = V;
-    m.own = h->stream;
-    h->stream = g->stream;
+    HIPCHK(h, hipEventCreateWithFlags(&m.ev, hipEventDisableTiming));
     h->group = g;
     h->V_scf = V;
     h->final_valid = false;
@@ -1931,7 +1937,7 @@ int qs_group_add(qs_group *g, qs_handle *h, double V, int32_t fresh) {
     if (rc == QS_OK && e != hipSuccess) rc = fail(h, QS_ERR_HIP, "group start: %s", hipGetErrorString(e));
     if (rc != QS_OK) {
         (void)hipStreamSynchronize(h->stream);
-        h->stream = m.own;
+        (void)hipEventDestroy(m.ev);
         h->group = nullptr;
         return gfail(g, rc, "%s", h->err.c_str());
     }
@@ -1939,69 +1945,92 @@ int qs_group_add(qs_group *g, qs_handle *h, double V, int32_t fresh) {
     return QS_OK;
 }
 
-int qs_group_step(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
-                  int32_t *n_done) {
-    if (!g || !n_done) return QS_ERR_ARG;
-    *n_done = 0;
+int qs_group_step_begin(qs_group *g) {
+    if (!g) return QS_ERR_ARG;
+    if (g->in_step) return gfail(g, QS_ERR_STATE, "a step of this group is already in flight");
     const int nm = (int)g->mem.size();
+    g->rc_step.assign(nm, QS_OK);
+    g->in_step = true;
     if (nm == 0) return QS_OK;
-    if (max_done < nm || !done || !res || !status)
-        return gfail(g, QS_ERR_ARG, "room for %d finished solves is needed", nm);
     if (hipSetDevice(g->device) != hipSuccess) return gfail(g, QS_ERR_HIP, "hipSetDevice");
     // a member whose launches fail leaves with the error; the others go on
-    std::vector<int> rcm(nm, QS_OK);
-    for (int i = 0; i < nm; i++) rcm[i] = enqueue_ctl_pre(g->mem[i].h);
+    std::vector<int> &rcm = g->rc_step;
     // members that share a Dyson kernel instance sweep in one launch; the others
-    // (the large-Norb path, the shapes without a grouped form, a slab shorter than
-    // the grid) launch on their own
-    std::vector<std::pair<int, int>> order;  // (key, member)
+    // (the shapes without a grouped form, a slab shorter than the grid) launch on
+    // their own streams
+    std::vector<int> key(nm, -1);
     for (int i = 0; i < nm; i++) {
-        if (rcm[i]) continue;
         const qs_handle *h = g->mem[i].h;
-        int key = qs_dyson_group_key(h->norb, h->dyson_variant);
-        if (h->mscr_energies < h->nw) key = -1;
-        order.push_back({key, i});
+        key[i] = h->mscr_energies >= h->nw ? qs_dyson_group_key(h->norb, h->dyson_variant) : -1;
     }
+    for (int i = 0; i < nm; i++) {
+        qs_group::Member &m = g->mem[i];
+        qs_handle *h = m.h;
+        if ((rcm[i] = enqueue_ctl_pre(h))) continue;
+        if (key[i] >= 0) {
+            hipError_t e = hipEventRecord(m.ev, h->stream);
+            if (e != hipSuccess) rcm[i] = fail(h, QS_ERR_HIP, "group step: %s", hipGetErrorString(e));
+        } else {
+            DysonArgs a = dyson_args(h, QS_M_SCF2, true, m.V);
+            hipEvent_t t0 = prof_begin(h);
+            hipError_t e = qs_launch_dyson_variant(a, h->nw, h->dyson_variant, h->stream);
+            prof_end(h, ST_DYSON, t0);
+            if (e != hipSuccess) rcm[i] = fail(h, QS_ERR_HIP, "Dyson sweep: %s", hipGetErrorString(e));
+        }
+    }
+    std::vector<std::pair<int, int>> order;  // (key, member) of the shared sweeps
+    for (int i = 0; i < nm; i++)
+        if (!rcm[i] && key[i] >= 0) order.push_back({key[i], i});
     std::stable_sort(order.begin(), order.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
         return a.first < b.first;
     });
-    int ntab = 0;
-    for (auto &o : order)
-        if (o.first >= 0) g->h_tab[ntab++] = dyson_args(g->mem[o.second].h, QS_M_SCF2, true, g->mem[o.second].V);
+    const int ntab = (int)order.size();
+    for (int t = 0; t < ntab; t++)
+        g->h_tab[t] = dyson_args(g->mem[order[t].second].h, QS_M_SCF2, true, g->mem[order[t].second].V);
     hipError_t e = hipSuccess;
     if (ntab) e = hipMemcpyAsync(g->d_tab, g->h_tab, sizeof(DysonArgs) * ntab, hipMemcpyHostToDevice, g->stream);
+    for (int t = 0; t < ntab && e == hipSuccess; t++) e = hipStreamWaitEvent(g->stream, g->mem[order[t].second].ev, 0);
     if (e == hipSuccess) e = hipEventRecord(g->ev[0], g->stream);
-    g->last_launches = 0;
+    g->last_launches = nm - ntab;
     g->last_fused = ntab;
-    int t = 0;
-    for (size_t k = 0; k < order.size() && e == hipSuccess;) {
-        const int key = order[k].first;
-        size_t k2 = k;
-        while (k2 < order.size() && order[k2].first == key) k2++;
-        if (key >= 0) {
-            const int cnt = (int)(k2 - k);
-            e = qs_launch_dyson_group(g->h_tab + t, g->d_tab + t, cnt, g->mem[order[k].second].h->dyson_variant,
-                                      g->stream);
-            t += cnt;
-            g->last_launches++;
-        } else {
-            for (size_t j = k; j < k2 && e == hipSuccess; j++) {
-                qs_handle *h = g->mem[order[j].second].h;
-                e = qs_launch_dyson_variant(dyson_args(h, QS_M_SCF2, true, g->mem[order[j].second].V), h->nw,
-                                            h->dyson_variant, g->stream);
-                g->last_launches++;
-            }
-        }
-        k = k2;
+    for (int t = 0; t < ntab && e == hipSuccess;) {
+        int t2 = t;
+        while (t2 < ntab && order[t2].first == order[t].first) t2++;
+        e = qs_launch_dyson_group(g->h_tab + t, g->d_tab + t, t2 - t, g->mem[order[t].second].h->dyson_variant,
+                                  g->stream);
+        g->last_launches++;
+        t = t2;
     }
     if (e == hipSuccess) e = hipEventRecord(g->ev[1], g->stream);
+    for (int t = 0; t < ntab && e == hipSuccess; t++) e = hipStreamWaitEvent(g->mem[order[t].second].h->stream, g->ev[1], 0);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(g->stream);
+        for (auto &m : g->mem) (void)hipStreamSynchronize(m.h->stream);
+        g->in_step = false;
         return gfail(g, QS_ERR_HIP, "group Dyson sweep: %s", hipGetErrorString(e));
     }
     for (int i = 0; i < nm; i++)
         if (!rcm[i]) rcm[i] = enqueue_ctl_post(g->mem[i].h);
-    e = hipStreamSynchronize(g->stream);
+    return QS_OK;
+}
+
+int qs_group_step_end(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
+                      int32_t *n_done) {
+    if (!g || !n_done) return QS_ERR_ARG;
+    *n_done = 0;
+    if (!g->in_step) return gfail(g, QS_ERR_STATE, "no step of this group is in flight");
+    const int nm = (int)g->mem.size();
+    if (nm && (max_done < nm || !done || !res || !status))
+        return gfail(g, QS_ERR_ARG, "room for %d finished solves is needed", nm);
+    g->in_step = false;
+    if (nm == 0) return QS_OK;
+    if (hipSetDevice(g->device) != hipSuccess) return gfail(g, QS_ERR_HIP, "hipSetDevice");
+    std::vector<int> &rcm = g->rc_step;
+    hipError_t e = hipStreamSynchronize(g->stream);
+    for (int i = 0; i < nm; i++) {
+        const hipError_t e2 = hipStreamSynchronize(g->mem[i].h->stream);
+        if (e == hipSuccess) e = e2;
+    }
     if (e != hipSuccess) return gfail(g, QS_ERR_HIP, "group step: %s", hipGetErrorString(e));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, g->ev[0], g->ev[1]) == hipSuccess) g->dyson_ms += ms;
@@ -2043,6 +2072,18 @@ int qs_group_step(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result
     for (int i = nm - 1; i >= 0; i--)
         if (leave[i]) group_release(g, (size_t)i);
     return QS_OK;
+}
+
+int qs_group_step(qs_group *g, int32_t max_done, qs_handle **done, qs_scf_result *res, int32_t *status,
+                  int32_t *n_done) {
+    if (!g || !n_done) return QS_ERR_ARG;
+    *n_done = 0;
+    const int nm = (int)g->mem.size();
+    if (nm && (max_done < nm || !done || !res || !status))
+        return gfail(g, QS_ERR_ARG, "room for %d finished solves is needed", nm);
+    int rc = qs_group_step_begin(g);
+    if (rc) return rc;
+    return qs_group_step_end(g, max_done, done, res, status, n_done);
 }
 
 int qs_group_info(const qs_group *g, int32_t *launches, int32_t *fused_members, double *dyson_ms, int64_t *steps) {

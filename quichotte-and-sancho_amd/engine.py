@@ -113,6 +113,8 @@ def load_library(path=SO):
     L.qs_group_add.argtypes = [P, P, D, I32]
     L.qs_group_members.argtypes = [P]
     L.qs_group_step.argtypes = [P, I32, P, P, P, P]
+    L.qs_group_step_begin.argtypes = [P]
+    L.qs_group_step_end.argtypes = [P, I32, P, P, P, P]
     L.qs_group_info.argtypes = [P, P, P, P, P]
     _lib = L
     return L
@@ -385,7 +387,7 @@ _hip = None
 
 
 class ScfGroup:
-    """qs_group_*: several order-2 solves stepped together on one stream, with the
+    """qs_group_*: several order-2 solves stepped together, with the
     Dyson sweeps of members that share a kernel instance in one launch.  Members
     are ScfEngine objects; add() starts a solve, step() runs one SCF step of every
     member and returns the solves that ended as (engine, ScfResult, status) with
@@ -414,12 +416,22 @@ class ScfGroup:
         return self.L.qs_group_members(self.g)
 
     def step(self):
+        return self._finish(self.L.qs_group_step)
+
+    def step_begin(self):
+        """enqueue one step and return at once (step_end waits for it)"""
+        self._check(self.L.qs_group_step_begin(self.g))
+
+    def step_end(self):
+        return self._finish(self.L.qs_group_step_end)
+
+    def _finish(self, fn):
         nm = max(len(self), 1)
         done = (ctypes.c_void_p * nm)()
         res = (_Result * nm)()
         st = (ctypes.c_int32 * nm)()
         nd = ctypes.c_int32()
-        self._check(self.L.qs_group_step(self.g, nm, done, res, st, ctypes.byref(nd)))
+        self._check(fn(self.g, nm, done, res, st, ctypes.byref(nd)))
         out = []
         for k in range(nd.value):
             eng = self._members.pop(done[k])

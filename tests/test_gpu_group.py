@@ -1,9 +1,8 @@
-"""Group steps (qs_group_*): several order-2 solves stepped together on one
-stream, with the Dyson/Keldysh sweeps of the members that share a kernel
-instance in one launch.  Each member's results equal qs_solve_scf on its own
-handle bit for bit (the same per-energy arithmetic and the same Broyden
-steps), whatever joins or leaves the group around it; one member is also
-compared with the oracle."""
+"""Group steps (qs_group_*): several order-2 solves stepped together, with
+the Dyson/Keldysh sweeps of the members that share a kernel instance in one
+launch.  Each member's results equal qs_solve_scf on its own handle bit for
+bit (the same per-energy arithmetic and the same Broyden steps), whatever joins
+or leaves the group around it; one member is also compared with the oracle."""
 import numpy as np
 import pytest
 
@@ -169,3 +168,42 @@ def test_group_rejects_what_it_cannot_step():
     assert g.step() == []
     g.close()
     e1.close()
+
+
+def test_two_groups_stepped_half_a_step_apart():
+    """Two groups whose steps are begun while the other's runs (step_begin /
+    step_end): every member still equals its solve alone; joining during a step is
+    refused."""
+    pkg = __import__("qs_testlib").product()
+    specs = [(4, 2, 256, 0.96, 0.24), (4, 2, 512, 0.72, -0.94), (6, 4, 256, 0.35, -0.74), (4, 2, 384, 0.6, 0.15)]
+    probs = [chain(ni, nt, nw, U, V) for ni, nt, nw, U, V in specs]
+    ref = [alone(p, s[4]) for p, s in zip(probs, specs)]
+    engines = [gpu_engine(p) for p in probs]
+    gs = [pkg.ScfGroup(0), pkg.ScfGroup(0)]
+    for k, (e, s) in enumerate(zip(engines, specs)):
+        gs[k % 2].add(e, s[4], True)
+    out = {}
+    for g in gs:
+        g.step_begin()
+    extra = gpu_engine(probs[0])
+    with pytest.raises(pkg.QsError) as ei:
+        gs[0].add(extra, 0.1)
+    assert ei.value.code == QS_ERR_STATE
+    live = [True, True]
+    while any(live):
+        for j, g in enumerate(gs):
+            if not live[j]:
+                continue
+            for eng, r, st in g.step_end():
+                assert st == 0
+                out[engines.index(eng)] = (r, eng.current(), eng.g_less_obs())
+            if len(g):
+                g.step_begin()
+            else:
+                live[j] = False
+    for g in gs:
+        g.close()
+    for k in range(len(specs)):
+        same(out[k], ref[k])
+    for e in engines + [extra]:
+        e.close()
