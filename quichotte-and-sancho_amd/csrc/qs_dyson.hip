@@ -531,7 +531,7 @@ bool qs_dyson_shape_ok(int norb, int shape) {
     if (shape == QS_DYSON_WG256) return norb <= qs_dyson_max_norb() && (norb + 15) / 16 != 5;
     if (shape == QS_DYSON_WG512) return norb > 16 && norb <= qs_dyson_max_norb();
     if (shape == QS_DYSON_WAVE) return norb <= qs_dyson_wave_max_norb();
-    if (shape == QS_DYSON_SPLIT) return norb > 16 && norb <= qs_dyson_wave_max_norb();
+    if (shape == QS_DYSON_SPLIT || shape == QS_DYSON_SPLIT_MFMA) return norb > 16 && norb <= qs_dyson_wave_max_norb();
     if (shape == QS_DYSON_BIG) return norb <= qs_dyson_big_max_norb();
     return false;
 }
@@ -545,8 +545,10 @@ const char *qs_dyson_shape_desc(int shape, int *uses_mfma) {
         "16x16 workgroup, matrix in registers, scratch slab (FP64 vector FMA)",
         "large-Norb chunks through HBM: blocked Gauss-Jordan and products on FP64 MFMA",
         "16x16 workgroups, matrix in registers, scratch slab: Gauss-Jordan kernel (three energies per CU) "
-        "then Keldysh kernel (FP64 vector FMA)"};
-    static const int mfma[QS_DYSON_NSHAPES] = {0, 0, 0, 1, 0};
+        "then Keldysh kernel (FP64 vector FMA)",
+        "16x16 workgroups, matrix in registers, scratch slab: Gauss-Jordan kernel (three energies per CU) "
+        "then Keldysh kernel on FP64 MFMA (one wavefront per 16-column strip)"};
+    static const int mfma[QS_DYSON_NSHAPES] = {0, 0, 0, 1, 0, 1};
     if (shape < 0 || shape >= QS_DYSON_NSHAPES) {
         if (uses_mfma) *uses_mfma = 0;
         return "";
@@ -558,8 +560,9 @@ const char *qs_dyson_shape_desc(int shape, int *uses_mfma) {
 hipError_t qs_launch_dyson_variant(const DysonArgs &a, int count, int variant, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     if (!qs_dyson_shape_ok(a.norb, variant)) return hipErrorInvalidValue;
-    if (variant == QS_DYSON_WAVE) return qs_launch_dyson_wave(a, count, false, s);
-    if (variant == QS_DYSON_SPLIT) return qs_launch_dyson_wave(a, count, true, s);
+    if (variant == QS_DYSON_WAVE) return qs_launch_dyson_wave(a, count, 0, s);
+    if (variant == QS_DYSON_SPLIT) return qs_launch_dyson_wave(a, count, 1, s);
+    if (variant == QS_DYSON_SPLIT_MFMA) return qs_launch_dyson_wave(a, count, 2, s);
     if (variant == QS_DYSON_BIG) return qs_launch_dyson_big(a, count, s);
     if (qs_dyson_lds_bytes(a.norb, a.nCL) > 160 * 1024) return hipErrorInvalidValue;
     const int RB = (a.norb + 15) / 16;

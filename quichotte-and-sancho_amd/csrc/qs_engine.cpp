@@ -1961,7 +1961,8 @@ int qs_group_step_begin(qs_group *g) {
     std::vector<int> key(nm, -1);
     for (int i = 0; i < nm; i++) {
         const qs_handle *h = g->mem[i].h;
-        key[i] = h->mscr_energies >= h->nw ? qs_dyson_group_key(h->norb, h->dyson_variant) : -1;
+        const bool fits = h->mscr_energies >= h->nw && !(h->dyson_variant == QS_DYSON_SPLIT_MFMA && (h->nw & 1));
+        key[i] = fits ? qs_dyson_group_key(h->norb, h->dyson_variant) : -1;
     }
     for (int i = 0; i < nm; i++) {
         qs_group::Member &m = g->mem[i];

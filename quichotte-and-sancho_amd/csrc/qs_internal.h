@@ -68,8 +68,9 @@ int qs_dyson_max_norb();
 // launch shapes: 0 = 256-thread workgroup, 1 = 512-thread workgroup (norb > 16),
 // 2 = one wavefront per energy (norb <= qs_dyson_wave_max_norb(), needs mscr)
 // 4 = shape 2 split into a Gauss-Jordan kernel (three energies per CU) and a Keldysh kernel
+// 5 = shape 4 with the Keldysh part on FP64 MFMA
 enum { QS_DYSON_WG256 = 0, QS_DYSON_WG512 = 1, QS_DYSON_WAVE = 2, QS_DYSON_BIG = 3, QS_DYSON_SPLIT = 4,
-       QS_DYSON_NSHAPES = 5 };
+       QS_DYSON_SPLIT_MFMA = 5, QS_DYSON_NSHAPES = 6 };
 // large-Norb path (qs_dyson_big.hip): energies in chunks through HBM, batched
 // library factorisation and products, hand-written build and packing kernels
 struct QsBigWork;
@@ -90,9 +91,10 @@ hipError_t qs_launch_gj_inv(int np, int count, void *M, void *G, int *status, bo
 // C[b] = A[b] op(B[b]), op = identity or conjugate transpose
 hipError_t qs_launch_cgemm(int np, int count, const void *A, const void *B, void *C, bool conjB, bool fp32,
                            hipStream_t s);
-// small-LDS kernels (qs_dyson_wave.hip): one fused kernel, or (split) the Gauss-Jordan
-// part and the Keldysh part as two kernels through the slab (17..96 orbitals)
-hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, bool split, hipStream_t s);
+// small-LDS kernels (qs_dyson_wave.hip): form 0, one fused kernel; 1, the Gauss-Jordan
+// part and the Keldysh part as two kernels through the slab (17..96 orbitals); 2, as 1
+// with the Keldysh part on FP64 MFMA
+hipError_t qs_launch_dyson_wave(const DysonArgs &a, int count, int form, hipStream_t s);
 size_t qs_dyson_wave_lds_bytes(int norb, int nCL);
 // One launch over the whole sweeps of several solves (a group step).  Member m
 // owns workgroups [first[m], first[m + 1]) of each part's launch; its arguments
@@ -105,7 +107,7 @@ struct DysonGroupIdx { int nmem; int first[QS_GROUP_MAX + 1]; };
 int qs_dyson_group_key(int norb, int variant);
 // ht: the members' arguments on the host (sizes, checks), dt: the same in device
 // memory.  Each member sweeps its nw energies (iw0 = 0) through its own slab,
-// which must hold them all (mscr_energies >= nw).
+// which must hold them all (mscr_energies >= nw); shape 5 also needs nw even.
 hipError_t qs_launch_dyson_group(const DysonArgs *ht, const DysonArgs *dt, int nmem, int variant, hipStream_t s);
 size_t qs_dyson_wave_scratch_bytes(int norb);
 int qs_dyson_wave_max_norb();

@@ -9,8 +9,10 @@
 //        qs_dyson_wave.hip as the source: the small-LDS kernel's own launch)
 //        dyson_emu <problem.bin> <out.bin> split 0 0     (the same kernel as its
 //        Gauss-Jordan and Keldysh parts, two launches through the slab)
-//        dyson_emu <p1.bin> <o1.bin> group|gsplit <p2.bin> <o2.bin>  (two problems
-//        as the members of one grouped launch, fused or split)
+//        dyson_emu <problem.bin> <out.bin> mfsplit 0 0   (the Gauss-Jordan kernel, then
+//        the Keldysh kernel on MFMA, emulated lane by lane)
+//        dyson_emu <p1.bin> <o1.bin> group|gsplit|gmfsplit <p2.bin> <o2.bin>  (two
+//        problems as the members of one grouped launch: fused, split, split on MFMA)
 // problem.bin: int32 n, count, mode, use_sigma, nCL; float64 w_init, delta,
 // mu, beta, V; complex128 Hemb (n*n), GamL (n*n), GamR (n*n), GamL_cl
 // (nCL*nCL); int32 cl (nCL); complex128 sig1 (n/2*4); complex128 sig
@@ -22,7 +24,7 @@
 #include QS_DYSON_SOURCE
 
 #ifndef QS_EMU_WAVE
-hipError_t qs_launch_dyson_wave(const DysonArgs &, int, bool, hipStream_t) { return hipErrorInvalidValue; }
+hipError_t qs_launch_dyson_wave(const DysonArgs &, int, int, hipStream_t) { return hipErrorInvalidValue; }
 int qs_dyson_wave_max_norb() { return 0; }
 #endif
 hipError_t qs_launch_dyson_big(const DysonArgs &, int, hipStream_t) { return hipErrorInvalidValue; }
@@ -134,12 +136,12 @@ int main(int argc, char **argv) {
     // group / gsplit: the problems of argv[1] and argv[4] as the two members of
     // one fused launch (launch shape 2 / 4); outputs to argv[2] and argv[5]
     const std::string first = argv[3];
-    if (first == "group" || first == "gsplit") {
+    if (first == "group" || first == "gsplit" || first == "gmfsplit") {
         Problem P[2];
         if (!load(argv[1], P[0]) || !load(argv[4], P[1])) return 2;
         DysonArgs tab[2] = {P[0].a, P[1].a};
-        const hipError_t rc = qs_launch_dyson_group(tab, tab, 2, first == "group" ? QS_DYSON_WAVE : QS_DYSON_SPLIT,
-                                                    nullptr);
+        const int variant = first == "group" ? QS_DYSON_WAVE : first == "gsplit" ? QS_DYSON_SPLIT : QS_DYSON_SPLIT_MFMA;
+        const hipError_t rc = qs_launch_dyson_group(tab, tab, 2, variant, nullptr);
         if (rc != hipSuccess) {
             fprintf(stderr, "grouped launch failed: %d\n", rc);
             return 3;
@@ -155,8 +157,9 @@ int main(int argc, char **argv) {
     hipError_t rc = hipErrorInvalidValue;
 #ifdef QS_EMU_WAVE
     (void)n;
-    if (shape == "wave,0,0") rc = qs_launch_dyson_wave(a, count, false, nullptr);
-    else if (shape == "split,0,0") rc = qs_launch_dyson_wave(a, count, true, nullptr);
+    if (shape == "wave,0,0") rc = qs_launch_dyson_wave(a, count, 0, nullptr);
+    else if (shape == "split,0,0") rc = qs_launch_dyson_wave(a, count, 1, nullptr);
+    else if (shape == "mfsplit,0,0") rc = qs_launch_dyson_wave(a, count, 2, nullptr);
 #else
     if (shape == "16,5,5") rc = launch_v<16, 5, 5>(a, count, nullptr);
     else if (shape == "32,3,5") rc = launch_v<32, 3, 5>(a, count, nullptr);

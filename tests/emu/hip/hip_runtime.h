@@ -165,6 +165,24 @@ inline unsigned long long ballot(bool pred) {
     return m;
 }
 
+// v_mfma_f64_16x16x4f64: D (16 x 16) = C + A (16 x 4) B (4 x 16), lane l holding
+// A[l % 16][l / 16] and B[l / 16][l % 16]; register q of lane l is D[l / 16 + 4 q][l % 16]
+typedef double mf4_emu __attribute__((vector_size(4 * sizeof(double))));
+inline mf4_emu mfma_f64_16x16x4(double a, double b, mf4_emu c) {
+    double A[64], B[64];
+    gather64(a, A);
+    gather64(b, B);
+    const int l = (int)emu_tid.x & 63;
+    mf4_emu d = c;
+    for (int q = 0; q < 4; q++) {
+        const int row = (l >> 4) + 4 * q, col = l & 15;
+        double s = 0.0;
+        for (int k = 0; k < 4; k++) s += A[16 * k + row] * B[16 * k + col];
+        d[q] = c[q] + s;
+    }
+    return d;
+}
+
 template <class T>
 T exchange_from(T v, int src) {
     const int t = (int)emu_tid.x;
@@ -216,6 +234,7 @@ inline double __hiloint2double(int hi, int lo) {
 #define __builtin_amdgcn_s_setprio(p) ((void)0)
 #define __builtin_amdgcn_update_dpp(old, src, ctrl, rmask, bmask, bc) emu::update_dpp((old), (src), (ctrl), (rmask), (bmask), (bc))
 #define __ballot(p) emu::ballot(p)
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, cbsz, abid, blgp) emu::mfma_f64_16x16x4((a), (b), (c))
 // a wavefront-scope fence: the lanes of a wavefront meet, which orders their
 // LDS accesses as the hardware's in-order LDS queue does
 #define __builtin_amdgcn_fence(order, scope) emu::wave_fence(scope)

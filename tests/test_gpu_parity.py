@@ -180,13 +180,14 @@ def test_runaway_and_errors_are_reported(shape, monkeypatch):
     assert ab and not F.any()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("norb", [18, 48, 80, 88])
 def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     """Every launch shape of the Dyson kernel: 256- and 512-thread workgroups with
     the matrix staged in LDS, the 16 x 16 workgroup with a scratch slab (fused, and
-    split into its Gauss-Jordan and Keldysh kernels), and the large-Norb path
-    (hand-written MFMA inverse and products) forced at these sizes."""
+    split into its Gauss-Jordan and Keldysh kernels, the Keldysh products on the
+    vector unit or on MFMA), and the large-Norb path (hand-written MFMA inverse and
+    products) forced at these sizes."""
     if variant == 0 and (norb + 15) // 16 == 5:
         pytest.skip("the 256-thread shape at 65..80 orbitals is switched off in the library: it returned wrong "
                     "init2 values at 80 orbitals and a GPU memory fault followed; its source is clean under the "
@@ -206,7 +207,7 @@ def test_each_dyson_launch_shape_matches_oracle(norb, variant, monkeypatch):
     assert rel_max(Fg, Fo) < 1e-11, worst_packed(Fg, Fo, o.nA)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("order", [1, 2])
 def test_each_dyson_launch_shape_full_solve(order, variant, monkeypatch):
     """Full solves through one launch shape: the SCF1, SCF2, INIT2 and FINAL
