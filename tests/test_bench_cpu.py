@@ -8,6 +8,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, REPO)
 
 import bench  # noqa: E402
+import numpy as np  # noqa: E402
 
 
 def test_driver_command_line_parses():
@@ -42,3 +43,15 @@ def test_more_than_one_gpu_is_refused(capsys, monkeypatch):
     monkeypatch.setattr(sys, "argv", ["bench.py", "--gpus", "2"])
     assert bench.main() == 2
     assert "one GPU" in capsys.readouterr().err
+
+
+def test_batch_mode_arguments_and_workloads():
+    a = bench.make_parser().parse_args(["--batch", "helix80", "--batch-size", "12", "--batch-log2nw", "12"])
+    assert (a.batch, a.batch_size, a.batch_log2nw, a.batch_points) == ("helix80", 12, 12, 0)
+    for name, ((ni, nt), U, wmax, (v0, v1), lg, npts) in bench.BATCH_WORKLOADS.items():
+        H = bench.helix_H(ni, nt)
+        assert H.shape == (2 * ni * nt, 2 * ni * nt) and np.allclose(H, H.conj().T)
+        assert v0 < v1 and 10 <= lg <= 16 and npts > 0 and U > 0 and wmax > 0
+    # the batch systems are the benchmark helices: 16 and 80 spin-orbitals
+    assert bench.helix_H(*bench.BATCH_WORKLOADS["helix16"][0]).shape[0] == 16
+    assert bench.helix_H(*bench.BATCH_WORKLOADS["helix80"][0]).shape[0] == 80
