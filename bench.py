@@ -313,7 +313,13 @@ def serve_bench(args):
             prob = pkg.ScfProblem(H=H, gammaL=GL, gammaR=GR, U_site=Us, order=2, n_omega=nw, w_init=-12.0,
                                   delta=24.0 / nw, beta=1.0 / (KB * 300.0), mu=0.0, nhist=16, tolf=1e-8)
             reqs.append(serving.Request(rid, prob, V, arrival=t))
-        srv = serving.Server(max_running=args.serve_streams)
+        def progress(r, tag):
+            print("bench: serve rate %g/s %s: request %d done (%s, %d steps, %.2f s after arrival; device steps %s, "
+                  "Dyson shape %d)" % (rate, tag, r.rid, "converged" if r.converged else ("error" if r.error else
+                                       "not converged"), r.n_iter, r.latency, r.device_steps, r.dyson_shape),
+                  file=sys.stderr, flush=True)
+
+        srv = serving.Server(max_running=args.serve_streams, on_done=lambda r: progress(r, "mix"))
         t0 = time.perf_counter()
         res = srv.run(reqs)
         wall = time.perf_counter() - t0
@@ -337,6 +343,26 @@ def serve_bench(args):
         print("bench: serve rate %g/s: %d/%d converged, %.2f points/s, p50 %.0f ms, p99 %.0f ms, %d handles reused"
               % (rate, len(ok), len(res), len(ok) / wall, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99),
                  srv.reused), file=sys.stderr, flush=True)
+        if args.serve_plain:
+            # the plain FP64 path on the same stream: one request on the GPU at a time
+            srv1 = serving.Server(max_running=1, on_done=lambda r: progress(r, "plain"))
+            t0 = time.perf_counter()
+            res1 = srv1.run(reqs)
+            wall1 = time.perf_counter() - t0
+            srv1.shutdown()
+            lat1 = np.array([r.latency for r in res1])
+            ok1 = [r for r in res1 if r.converged and not r.error]
+            same = all(a.J_up == b.J_up and a.J_down == b.J_down and a.n_iter == b.n_iter
+                       for a, b in zip(sorted(res, key=lambda r: r.rid), sorted(res1, key=lambda r: r.rid)))
+            d = out["rates"][str(rate)]
+            d["plain_one_at_a_time"] = {"bias_points_per_s": len(ok1) / wall1, "wall_s": wall1, "converged": len(ok1),
+                                        "p50_ms": 1e3 * float(np.percentile(lat1, 50)),
+                                        "p99_ms": 1e3 * float(np.percentile(lat1, 99))}
+            d["speedup_vs_plain"] = (len(ok) / wall) / (len(ok1) / wall1) if ok1 else None
+            d["results_equal_to_plain"] = same
+            print("bench: serve rate %g/s, plain (one at a time): %.2f points/s, p50 %.0f ms, p99 %.0f ms; same results %s"
+                  % (rate, len(ok1) / wall1, 1e3 * np.percentile(lat1, 50), 1e3 * np.percentile(lat1, 99), same),
+                  file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -637,6 +663,8 @@ def make_parser():
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-sizes", default="16,48,80,160,240", help="spin-orbital counts of the serving stream")
+    ap.add_argument("--serve-plain", action="store_true",
+                    help="also serve each rate's stream one request at a time (the plain path) for comparison")
     ap.add_argument("--maxscf", type=int, default=0,
                     help="cap on SCF steps per bias point (counter-collection passes; 0 = the workload's 300)")
     ap.add_argument("--batch", default="", choices=[""] + list(BATCH_WORKLOADS),

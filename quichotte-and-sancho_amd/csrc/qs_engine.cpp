@@ -522,8 +522,10 @@ int setup_device(qs_handle *h) {
         if ((rc = dalloc(h, &h->d_Z, (size_t)h->Nf * h->ncolS))) return rc;
         // x - xold for the history pass (saves it reading x and xold), where the fused
         // update runs; counted in qs_footprint_bytes.  Without it (allocation
-        // failure) the history pass reads x and xold.
-        if (uses_dx(h->p, h->nA, h->ncolS, h->n_vec)) {
+        // failure, or QS_DX_BUFFER=0 in the environment) the history pass reads x
+        // and xold, with the same results.
+        const char *dxe = getenv("QS_DX_BUFFER");
+        if (uses_dx(h->p, h->nA, h->ncolS, h->n_vec) && !(dxe && atoi(dxe) == 0)) {
             void *p = nullptr;
             if (hipMalloc(&p, nv * sizeof(double)) == hipSuccess) {
                 h->allocs.push_back(p);
@@ -1005,10 +1007,12 @@ int broyden_device(qs_handle *h, double V, int *converged, int *n_iter, double *
 }
 
 // whether the device-controlled steps apply: order 2, FP64 sweeps, FP32 history of at
-// most 16 columns with the fused update (x - xold kept)
+// most 16 columns with the fused update.  The optional x - xold buffer only saves the
+// history pass reading x and xold (the same difference, the same bits), so whether its
+// allocation succeeded changes neither the path nor the results.
 bool ctl_steps_ok(const qs_handle *h) {
     return h->order == 2 && h->precision == 0 && h->p.single_history && h->p.nhist > 0 && h->p.nhist <= 16 &&
-           h->d_dx && h->d_ctl && qs_update_gt_fits(h->nA, h->ncolS, h->n_vec, 16);
+           h->d_ctl && qs_update_gt_fits(h->nA, h->ncolS, h->n_vec, 16);
 }
 
 // One order-2 SCF step with its scalar logic on the device: the residual
@@ -1051,8 +1055,9 @@ int enqueue_ctl_post(qs_handle *h) {
     prof_end(h, ST_REDUCE, t0);
     const int nblocks = qs_vec_blocks(n);
     t0 = prof_begin(h);
-    HIPCHK(h, qs_launch_broyden_hist_f32_ctl(n, h->d_F, h->d_Fold, h->d_dx, h->p.alpha_mix, (float *)h->d_hist_dF,
-                                             (float *)h->d_hist_U, h->d_ctl, h->d_scratch, nblocks, h->stream));
+    HIPCHK(h, qs_launch_broyden_hist_f32_ctl(n, h->d_F, h->d_Fold, h->d_x, h->d_xold, h->d_dx, h->p.alpha_mix,
+                                             (float *)h->d_hist_dF, (float *)h->d_hist_U, h->d_ctl, h->d_scratch,
+                                             nblocks, h->stream));
     prof_end(h, ST_HIST, t0);
     HIPCHK(h, qs_launch_reduce_cols(h->d_scratch, 2 * MAXH_DEV, nblocks, h->d_csum + 2 * nA,
                                     h->d_scratch + (size_t)nblocks * 2 * MAXH_DEV, h->stream));
@@ -1949,10 +1954,10 @@ int qs_group_step_begin(qs_group *g) {
     if (!g) return QS_ERR_ARG;
     if (g->in_step) return gfail(g, QS_ERR_STATE, "a step of this group is already in flight");
     const int nm = (int)g->mem.size();
+    if (nm && hipSetDevice(g->device) != hipSuccess) return gfail(g, QS_ERR_HIP, "hipSetDevice");
     g->rc_step.assign(nm, QS_OK);
     g->in_step = true;
     if (nm == 0) return QS_OK;
-    if (hipSetDevice(g->device) != hipSuccess) return gfail(g, QS_ERR_HIP, "hipSetDevice");
     // a member whose launches fail leaves with the error; the others go on
     std::vector<int> &rcm = g->rc_step;
     // members that share a Dyson kernel instance sweep in one launch; the others

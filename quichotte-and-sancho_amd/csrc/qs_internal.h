@@ -181,10 +181,11 @@ hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *
                                       const double *xold, const double *dx, double rf, double alpha, float *dFs,
                                       float *Us, int slot, int nh, double *partials, int nblocks, hipStream_t s);
 // the FP32-history pass with slot, 1/|F - Fold| and the history count from the control
-// block (dx as above; nothing is stored when the step stopped or slot < 0)
-hipError_t qs_launch_broyden_hist_f32_ctl(int64_t n, const double *F, const double *Fold, const double *dx,
-                                          double alpha, float *dFs, float *Us, const QsCtrl *c, double *partials,
-                                          int nblocks, hipStream_t s);
+// block (dx as above, or x - xold formed from x and xold when dx is null: the same FP64
+// difference, the same bits; nothing is stored when the step stopped or slot < 0)
+hipError_t qs_launch_broyden_hist_f32_ctl(int64_t n, const double *F, const double *Fold, const double *x,
+                                          const double *xold, const double *dx, double alpha, float *dFs, float *Us,
+                                          const QsCtrl *c, double *partials, int nblocks, hipStream_t s);
 hipError_t qs_launch_broyden_hist_f64(int64_t n, const double *F, const double *Fold, const double *x,
                                       const double *xold, double fnorm, double alpha, double *dF, double *U,
                                       int slot, int nh, double *partials, int nblocks, hipStream_t s);

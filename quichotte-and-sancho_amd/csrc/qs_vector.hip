@@ -433,12 +433,16 @@ hipError_t qs_launch_broyden_hist_f32(int64_t n, const double *F, const double *
     return hipGetLastError();
 }
 
-hipError_t qs_launch_broyden_hist_f32_ctl(int64_t n, const double *F, const double *Fold, const double *dx,
-                                          double alpha, float *dFs, float *Us, const QsCtrl *c, double *partials,
-                                          int nblocks, hipStream_t s) {
-    if (n % 4 != 0 || !dx || !c) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_hist_f32x4<16, true>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, nullptr, nullptr, dx,
-                       0.0, alpha, dFs, Us, -1, 0, partials, c);
+hipError_t qs_launch_broyden_hist_f32_ctl(int64_t n, const double *F, const double *Fold, const double *x,
+                                          const double *xold, const double *dx, double alpha, float *dFs, float *Us,
+                                          const QsCtrl *c, double *partials, int nblocks, hipStream_t s) {
+    if (n % 4 != 0 || !c || (!dx && (!x || !xold))) return hipErrorInvalidValue;
+    if (dx)
+        hipLaunchKernelGGL((k_hist_f32x4<16, true>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, nullptr, nullptr,
+                           dx, 0.0, alpha, dFs, Us, -1, 0, partials, c);
+    else
+        hipLaunchKernelGGL((k_hist_f32x4<16, false>), dim3(nblocks), dim3(NT), 0, s, n, n / 4, F, Fold, x, xold,
+                           nullptr, 0.0, alpha, dFs, Us, -1, 0, partials, c);
     return hipGetLastError();
 }
 

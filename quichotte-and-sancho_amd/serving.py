@@ -17,7 +17,9 @@ import threading
 import time
 from dataclasses import dataclass
 
-from .engine import ScfEngine, ScfProblem, device_memory, footprint_bytes
+from .engine import QsError, ScfEngine, ScfProblem, device_memory, footprint_bytes
+
+QS_ERR_NOMEM = 4
 
 
 @dataclass
@@ -40,6 +42,9 @@ class Response:
     done: float
     footprint: int
     error: str = ""
+    device_steps: bool = False    # the solve's scalar logic ran on the device (step modes 1/2)
+    dyson_shape: int = -1         # the handle's Dyson launch shape
+    reused: bool = False          # the request ran on a pooled handle
 
     @property
     def latency(self):
@@ -60,10 +65,12 @@ class Server:
     allocating and tuning a new one.  Idle handles count against the budget and
     are released, oldest first, when a request of another size needs the memory."""
 
-    def __init__(self, max_running=4, hbm_fraction=0.85, device=0, pool=True):
+    def __init__(self, max_running=4, hbm_fraction=0.85, device=0, pool=True, on_done=None):
         free, total = device_memory(device)
         self.budget = int(free * hbm_fraction)
         self.max_running = max_running
+        self.on_done = on_done     # called with each Response as it finishes (from a worker thread)
+        self.mem_waiting = 0       # requests waiting for device memory to create their handle
         self.cv = threading.Condition()
         self.used = 0              # footprints of the running requests
         self.idle = []             # pooled handles: (key, engine, footprint), oldest first
@@ -102,6 +109,35 @@ class Server:
                 return req, fp, None
         return None
 
+    def _create(self, problem):
+        """A handle for the problem.  If the device cannot hold it although the budget
+        admitted it (the runtime's own allocations, fragmentation), the idle handles
+        are released, then the request waits for a running one to finish, and tries
+        again; it fails only when every other running request is waiting as well."""
+        while True:
+            try:
+                return ScfEngine(problem)
+            except QsError as e:
+                if e.code != QS_ERR_NOMEM:
+                    raise
+            with self.cv:
+                idle, self.idle, self.idle_bytes = self.idle, [], 0
+            for _, eng, _ in idle:
+                eng.close()
+            if idle:
+                continue
+            with self.cv:
+                # requests that are solving (not this one, not others waiting here)
+                if self.running - 1 - self.mem_waiting <= 0:
+                    raise QsError(QS_ERR_NOMEM, "qs_create failed: the device cannot hold this request")
+                target = self.running
+                self.mem_waiting += 1
+                try:
+                    while self.running >= target and self.running - self.mem_waiting > 0:
+                        self.cv.wait(0.05)
+                finally:
+                    self.mem_waiting -= 1
+
     def _worker(self, t0, out):
         while True:
             evict = []
@@ -117,20 +153,23 @@ class Server:
             req, fp, eng = item
             start = time.perf_counter() - t0
             keep = False
+            reused = False
             try:
                 if eng is not None:
                     try:
                         eng.reset(req.problem)
                         self.reused += 1
+                        reused = True
                     except Exception:   # another active structure: a handle of its own
                         eng.close()
                         eng = None
                 if eng is None:
-                    eng = ScfEngine(req.problem)
+                    eng = self._create(req.problem)
                 r = eng.solve_scf(req.V, True)
                 ju, jd = eng.current()
                 resp = Response(req.rid, ju, jd, r.converged, r.n_iter, req.arrival, start,
-                                time.perf_counter() - t0, fp)
+                                time.perf_counter() - t0, fp, "", bool(eng.step_info()[0]), eng.dyson_choice()[0],
+                                reused)
                 keep = self.use_pool
             except Exception as e:  # the request fails, the server goes on
                 resp = Response(req.rid, float("nan"), float("nan"), False, 0, req.arrival, start,
@@ -146,6 +185,8 @@ class Server:
                 self.running -= 1
                 out.append(resp)
                 self.cv.notify_all()
+            if self.on_done is not None:
+                self.on_done(resp)
 
     def shutdown(self):
         """Release the pooled handles."""

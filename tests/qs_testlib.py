@@ -519,3 +519,36 @@ def sancho_oracle_bubbles(GR, GL, delta, order):
         raise RuntimeError("orc_sancho_bubbles failed: %d" % rc)
     # the C arrays are (q, b, a, m) in C order; return (q, a, b, m)
     return chi.transpose(0, 2, 1, 3), phi.transpose(0, 2, 1, 3)
+
+
+def landauer_integrand(prob, V):
+    """Spin-resolved current integrand of a non-interacting system from the
+    Landauer form, computed here from G^r alone: per energy
+    -(f_L - f_R) Re sum_{a even / odd} [Gamma_L G^r Gamma_R G^a]_aa / 2 pi.
+    For G^< = G^r i(f_L Gamma_L + f_R Gamma_R) G^a and G^> = G^< + G^r - G^a the
+    reference's integrand i Tr Gamma_L [(f_L - 1) G^< - f_L G^>] (src/scf.f90:390-409)
+    equals it at every energy: the Gamma_L terms cancel, leaving
+    i Gamma_L G^r Gamma_R G^a (f_L - f_R) inside the trace.  Returns (trU, trD)."""
+    n = prob.norb
+    Hemb = -prob.H + 0.5j * (prob.GL + prob.GR)
+    tu = np.zeros(prob.n_omega)
+    td = np.zeros(prob.n_omega)
+    for k, w in enumerate(prob.omega):
+        G = np.linalg.inv(w * np.eye(n) + Hemb)
+        fL = 1.0 / (np.exp(min((w - prob.mu + V) * prob.beta, 700.0)) + 1.0)
+        fR = 1.0 / (np.exp(min((w - prob.mu) * prob.beta, 700.0)) + 1.0)
+        d = np.diag(prob.GL @ G @ prob.GR @ G.conj().T).real
+        tu[k] = -(fL - fR) * d[0::2].sum() / (2 * np.pi)
+        td[k] = -(fL - fR) * d[1::2].sum() / (2 * np.pi)
+    return tu, td
+
+
+def polarized_helix(V, order=0, n_omega=300, U=0.0, ions=4, turns=2, **kw):
+    """A helix (4 x 2 by default) with strong spin-orbit coupling and a spin-polarized
+    left lead (GammaL_up 0.9, GammaL_down 0.3, src/input_output.f90:310-346), so that
+    the spin channels carry different currents."""
+    H = helix_hamiltonian(ions, turns, 1.4, 3.4, 1.0, 2.0, -0.4, 0.08)
+    GL, GR = leads(H.shape[0], 0.9, 0.3, 0.6)
+    Us = np.zeros(H.shape[0])
+    Us[0::2] = U
+    return Problem(H, GL, GR, Us, order, 300.0, 2.0, 0, 0, 4.0 / n_omega, [V], n_omega=n_omega, w_init=0.0, **kw)

@@ -85,3 +85,26 @@ def test_graph_runaway_abort_matches_host_loop():
     h, g = sweep(prob, 0)[0], sweep(prob, 2)[0]
     assert h[2] and g[2] and not h[1] and not g[1]
     assert h[0] == g[0]
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_device_steps_without_the_dx_buffer_give_the_same_bits(mode, monkeypatch):
+    """The optional x - xold buffer (left out with QS_DX_BUFFER=0, as a failed
+    allocation leaves it out) only saves the history pass reading x and xold: a
+    handle without it still takes device-controlled steps, and its sweep is the
+    same bit for bit."""
+    prob = helix16([0.0, 0.1, 0.2])
+    g = gpu_engine(prob)
+    full = g.device_bytes()
+    g.close()
+    with_dx = sweep(prob, mode)
+    monkeypatch.setenv("QS_DX_BUFFER", "0")
+    g = gpu_engine(prob)
+    assert g.device_bytes() < full
+    g.close()
+    without = sweep(prob, mode)
+    for a, b in zip(with_dx, without):
+        assert a[5] and b[5]                      # device-controlled steps in both
+        assert a[:3] == b[:3]                     # steps, converged, runaway
+        assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+        assert np.array_equal(a[7], b[7])         # every step's max|F|

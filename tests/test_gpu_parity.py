@@ -286,3 +286,28 @@ def test_large_norb_inverse_and_products_match_oracle(norb):
     Fg, _ = g.residual2(-0.2, x1)
     assert rel_max(Fg, Fo) < 1e-11, worst_packed(Fg, Fo, o.nA)
     g.close()
+
+
+@pytest.mark.parametrize("ions,variant", [(4, None), (4, 3), (6, 4), (6, 5), (6, 2)])
+def test_order0_current_is_landauer(ions, variant, monkeypatch):
+    """The GPU's non-interacting current integrand (the FINAL epilogue of each launch
+    shape) equals the Landauer form computed with numpy, per energy and per spin
+    channel, with a spin-polarized lead; J is its sum (src/scf.f90:390-409,
+    src/observables.f90:22-39)."""
+    from qs_testlib import landauer_integrand, polarized_helix
+    if variant is not None:
+        monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
+    V = 0.35
+    prob = polarized_helix(V, ions=ions)
+    g = gpu_engine(prob)
+    if variant is not None:
+        assert g.dyson_choice()[0] == variant
+    g.solve_scf(V, True)
+    tu, td = g.trUD()
+    lu, ld = landauer_integrand(prob, V)
+    scale = max(np.abs(lu).max(), np.abs(ld).max())
+    assert np.abs(tu - lu).max() <= 1e-12 * scale and np.abs(td - ld).max() <= 1e-12 * scale
+    assert np.abs(lu - ld).max() > 1e-3 * scale
+    J = np.array(g.current())
+    assert np.allclose(J, [prob.delta * lu.sum(), prob.delta * ld.sum()], rtol=1e-11, atol=0)
+    g.close()

@@ -117,3 +117,34 @@ def test_pool_reuses_handles_of_equal_sizes():
         if r.rid >= 100:
             a = first[r.rid - 100]
             assert r.error == "" and r.n_iter == a.n_iter and r.J_up == a.J_up and r.J_down == a.J_down
+
+
+@pytest.mark.parametrize("ions,turns,nw,variant", [(8, 5, 2048, "4"), (4, 2, 512, "2"), (6, 4, 1024, "1"),
+                                                   (10, 5, 512, "3")])
+def test_reused_handle_equals_a_fresh_one(ions, turns, nw, variant, monkeypatch):
+    """qs_reset_problem: a handle that solved one system (other U, V and lead
+    couplings) gives a second system the same bits as a handle made for it, in the
+    register, split and large-Norb shapes (a pool's handle carries nothing over)."""
+    pkg = importlib.import_module("quichotte-and-sancho_amd")
+    monkeypatch.setenv("QS_DYSON_VARIANT", variant)
+
+    def prob(U, gam):
+        p = helix_problem(ions, turns, U, order=2, n_omega=nw, wlo=-12.0, whi=12.0, E_CC=0.0, t_hop=-1.0,
+                          lamb=0.01, gamma=gam, mu=0.0, tolf=1e-8, nhist=16, voltages=[-1.0])
+        return pkg.ScfProblem(H=p.H, gammaL=p.GL, gammaR=p.GR, U_site=p.U_site, order=2, n_omega=p.n_omega,
+                              w_init=p.w_init, delta=p.delta, beta=p.beta, mu=p.mu, nhist=16, tolf=1e-8, maxscf=60)
+
+    A, B = prob(2.5, 0.15), prob(1.2, 0.1)
+    e = pkg.ScfEngine(A)
+    e.solve_scf(-0.8, True)
+    e.reset(B)
+    rb = e.solve_scf(-1.0, True)
+    got = (rb.n_iter, rb.converged, rb.fmax, e.current(), e.g_less_obs().copy(), rb.log_fmax.copy())
+    e.close()
+    f = pkg.ScfEngine(B)
+    rf = f.solve_scf(-1.0, True)
+    want = (rf.n_iter, rf.converged, rf.fmax, f.current(), f.g_less_obs().copy(), rf.log_fmax.copy())
+    f.close()
+    print("reused: %d steps, fresh: %d steps" % (got[0], want[0]))
+    assert got[:4] == want[:4]
+    assert np.array_equal(got[4], want[4]) and np.array_equal(got[5], want[5])

@@ -281,3 +281,26 @@ def test_custom_chain_reproduces_sanchos_order2_case():
             for n in (n_up, n_dn):
                 worst = max(worst, abs(n - p[2]) / (1e-6 + 1e-5 * abs(p[2])))
     assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("V", [0.0, 0.3, -0.45])
+def test_order0_current_is_landauer(V):
+    """The non-interacting current (order 0) equals the Landauer form per energy and
+    per spin, with spin-orbit coupling mixing the spins (src/observables.f90:22-39,
+    src/scf.f90:390-409); J(V = 0) = 0."""
+    from qs_testlib import landauer_integrand, polarized_helix
+    prob = polarized_helix(V)
+    o = Oracle(prob)
+    o.solve(V, True)
+    tu, td = o.trUD()
+    lu, ld = landauer_integrand(prob, V)
+    scale = max(np.abs(lu).max(), np.abs(ld).max(), 1e-300)
+    if V == 0.0:
+        assert np.abs(tu).max() < 1e-15 and np.abs(td).max() < 1e-15
+        assert np.abs(np.array(o.current())).max() < 1e-15
+    else:
+        assert np.abs(tu - lu).max() <= 1e-12 * scale and np.abs(td - ld).max() <= 1e-12 * scale
+        # the spin-polarized lead makes the spin channels differ
+        assert np.abs(lu - ld).max() > 1e-3 * scale
+        J = np.array(o.current())
+        assert np.allclose(J, [prob.delta * lu.sum(), prob.delta * ld.sum()], rtol=1e-11, atol=0)
