@@ -18,6 +18,9 @@
 // packing epilogues widen the results to FP64.
 // The padding block is the identity in A and zero in S'', so it stays out of
 // G^r's top-left n x n block and of G^<.
+#include <cstdlib>
+#include <cstring>
+
 #include "qs_device.h"
 #include "qs_internal.h"
 
@@ -243,6 +246,12 @@ QsBigWork *qs_big_create(int n, int nw, size_t budget, size_t *bytes) {
         (void)hipGetLastError();
         qs_big_destroy(bw);
         return nullptr;
+    }
+    // QS_FILL=nan|zero: the new buffers filled before use (as the engine's own)
+    if (const char *f = getenv("QS_FILL")) {
+        const int v = !strcmp(f, "nan") ? 0xff : !strcmp(f, "zero") ? 0 : -1;
+        if (v >= 0)
+            for (void *p : {(void *)bw->A, (void *)bw->S, (void *)bw->T}) (void)hipMemset(p, v, mat);
     }
     if (bytes) *bytes = 3 * mat;
     return bw;

@@ -268,6 +268,22 @@ bool uses_dx(const qs_problem &p, int nA, int ncolS, int64_t n_vec) {
     return p.order == 2 && p.single_history && p.nhist > 0 && qs_update_gt_fits(nA, ncolS, n_vec, std::min(p.nhist, 16));
 }
 
+// QS_FILL=nan|zero in the environment fills every device buffer of a new handle
+// (NaN bytes or zeros) before use: no result may depend on what the memory held
+int fill_mode() {
+    const char *e = getenv("QS_FILL");
+    if (!e) return 0;
+    if (!strcmp(e, "nan")) return 1;
+    if (!strcmp(e, "zero")) return 2;
+    return 0;
+}
+
+int fill_new(qs_handle *h, void *p, size_t bytes) {
+    const int m = fill_mode();
+    if (m) HIPCHK(h, hipMemset(p, m == 1 ? 0xff : 0, bytes));
+    return QS_OK;
+}
+
 template <class T>
 int dalloc(qs_handle *h, T **ptr, size_t count) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
@@ -276,6 +292,10 @@ int dalloc(qs_handle *h, T **ptr, size_t count) {
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(h, QS_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    if (int rc = fill_new(h, p, bytes)) {
+        (void)hipFree(p);
+        return rc;
     }
     h->allocs.push_back(p);
     h->dev_bytes += (int64_t)bytes;
@@ -529,6 +549,7 @@ int setup_device(qs_handle *h) {
             void *p = nullptr;
             if (hipMalloc(&p, nv * sizeof(double)) == hipSuccess) {
                 h->allocs.push_back(p);
+                if (int rc = fill_new(h, p, nv * sizeof(double))) return rc;
                 h->dev_bytes += (int64_t)(nv * sizeof(double));
                 h->d_dx = (double *)p;
             } else {

@@ -28,7 +28,10 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
-#define __shared__
+// a kernel's static LDS array is one array for all threads of the running workgroup
+// (workgroups run one after another); the kernels' dynamic-LDS declaration is replaced
+// by the test that builds them
+#define __shared__ static
 #define __align__(n) alignas(n)
 
 struct dim3 {
@@ -38,6 +41,23 @@ struct dim3 {
 struct double2 {
     double x, y;
 };
+struct double4 {
+    double x, y, z, w;
+};
+struct float4 {
+    float x, y, z, w;
+};
+struct int2 {
+    int x, y;
+};
+inline double4 make_double4(double a, double b, double c, double d) { return double4{a, b, c, d}; }
+inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
+inline int2 make_int2(int a, int b) { return int2{a, b}; }
+inline unsigned __brev(unsigned v) {
+    unsigned r = 0;
+    for (int i = 0; i < 32; i++) r |= ((v >> i) & 1u) << (31 - i);
+    return r;
+}
 struct float2 {
     float x, y;
 };
@@ -49,6 +69,11 @@ enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorLaunchFailure = 719 };
 typedef void *hipStream_t;
 enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
 inline hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
+// a device of two CUs: the persistent kernels run two workgroups that each walk
+// several work items
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
 
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
@@ -272,6 +297,8 @@ template <class... P, class... A>
 hipError_t launch(void (*k)(P...), dim3 grid, dim3 block, size_t lds_bytes, hipStream_t, A... args) {
     const int nt = (int)(block.x * block.y * block.z);
     if (nt % 64 || lds_bytes > 160 * 1024) return last_error = hipErrorInvalidValue;
+    for (unsigned bz = 0; bz < grid.z; bz++)
+    for (unsigned by = 0; by < grid.y; by++)
     for (unsigned bx = 0; bx < grid.x; bx++) {
         Block blk;
         blk.bar.n = nt;
@@ -286,7 +313,7 @@ hipError_t launch(void (*k)(P...), dim3 grid, dim3 block, size_t lds_bytes, hipS
             th.emplace_back([&, t] {
                 cur = &blk;
                 emu_tid = dim3(t);
-                emu_bid = dim3(bx);
+                emu_bid = dim3(bx, by, bz);
                 emu_bdim = block;
                 emu_gdim = grid;
                 k(args...);

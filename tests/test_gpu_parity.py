@@ -311,3 +311,29 @@ def test_order0_current_is_landauer(ions, variant, monkeypatch):
     J = np.array(g.current())
     assert np.allclose(J, [prob.delta * lu.sum(), prob.delta * ld.sum()], rtol=1e-11, atol=0)
     g.close()
+
+
+@pytest.mark.parametrize("ions,turns,nw,variant", [(4, 2, 512, 2), (4, 2, 512, 0), (6, 4, 1024, 0), (6, 4, 1024, 2),
+                                                   (8, 5, 1024, 4), (8, 5, 1024, 5), (10, 5, 1024, 3)])
+def test_results_do_not_depend_on_what_new_memory_held(ions, turns, nw, variant, monkeypatch):
+    """Every device buffer of a new handle filled with NaN bytes (QS_FILL=nan) or with
+    zeros: the solve (and its final sweep) gives the same bits, so nothing reads
+    memory it has not written, in every launch shape."""
+    monkeypatch.setenv("QS_DYSON_VARIANT", str(variant))
+    prob = helix_problem(ions, turns, 1.5, order=2, n_omega=nw, wlo=-12.0, whi=12.0, E_CC=0.0, t_hop=-1.0,
+                         lamb=0.01, gamma=0.1, mu=0.0, tolf=1e-8, nhist=16, voltages=[-1.0], maxscf=40)
+    out = []
+    for fill in ("nan", "zero"):
+        monkeypatch.setenv("QS_FILL", fill)
+        g = gpu_engine(prob)
+        assert g.dyson_choice()[0] == variant
+        r = g.solve_scf(-1.0, True)
+        out.append((r.n_iter, r.converged, r.fmax, g.current(), g.g_less_obs().copy(), r.log_fmax.copy(),
+                    g.dosd().copy()))
+        g.close()
+    a, b = out
+    print("nan fill: %d steps, fmax %.3e; zero fill: %d steps, fmax %.3e" % (a[0], a[2], b[0], b[2]))
+    assert a[:4] == b[:4]
+    assert a[0] > 5 and a[2] < 1.0
+    for k in (4, 5, 6):
+        assert np.array_equal(a[k], b[k])
