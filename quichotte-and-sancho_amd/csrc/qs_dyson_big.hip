@@ -250,8 +250,10 @@ QsBigWork *qs_big_create(int n, int nw, size_t budget, size_t *bytes) {
     // QS_FILL=nan|zero: the new buffers filled before use (as the engine's own)
     if (const char *f = getenv("QS_FILL")) {
         const int v = !strcmp(f, "nan") ? 0xff : !strcmp(f, "zero") ? 0 : -1;
-        if (v >= 0)
+        if (v >= 0) {
             for (void *p : {(void *)bw->A, (void *)bw->S, (void *)bw->T}) (void)hipMemset(p, v, mat);
+            (void)hipDeviceSynchronize();
+        }
     }
     if (bytes) *bytes = 3 * mat;
     return bw;

@@ -622,6 +622,11 @@ __device__ __forceinline__ void dyson_w_body(const DysonArgs &A, const int blk) 
     // P = sum_c T(:, c) conj(M(:, c))^T, TJ steps at a time.  Columns k0 .. k0 + TJ - 1
     // of T are this thread's register column bb (TI = TJ): they go from registers to
     // stage[kk][*], and the columns of M from Ms to chunk[kk][*].
+    // P = M S'' M^dagger is anti-Hermitian (S'' is), so only the register tiles on
+    // and above the diagonal are formed (b >= a; a tile below it, (a, b) with a > b,
+    // is -conj of tile (b, a) transposed, which the thread (lj, li) holds): 15 of 25
+    // tiles at 80 orbitals.  The epilogue stages the missing ones from their mirrors.
+    static_assert(RA == RB && TI == TJ, "square register tiles");
     double2 P[RA][RB];
 #pragma unroll
     for (int a = 0; a < RA; a++)
@@ -658,7 +663,7 @@ __device__ __forceinline__ void dyson_w_body(const DysonArgs &A, const int blk) 
             for (int b = 0; b < RB; b++) {
                 const double2 mb = L.chunk[kk * n + min(lj + TJ * b, n - 1)];
 #pragma unroll
-                for (int a = 0; a < RA; a++) cfmac(P[a][b], ta[a], mb);
+                for (int a = 0; a <= b; a++) cfmac(P[a][b], ta[a], mb);
             }
         }
         __syncthreads();
@@ -666,14 +671,22 @@ __device__ __forceinline__ void dyson_w_body(const DysonArgs &A, const int blk) 
 
     // ---- 6. epilogues ---------------------------------------------------------------
     Epi E = epi_begin(A, iw);
-    // G^< in TI-row stages: pivoted row r = TI a + s is natural row qinv[r]
+    // G^< in TI-row stages: pivoted row r = TI a + s is natural row qinv[r].  Stage a
+    // takes this thread's tiles (a, b >= a) as they are, and for b < a its tile
+    // (b, a) transposed and negated-conjugated: element (li + TI b, lj + TI a) of P
+    // gives (lj + TI a, li + TI b) as -conj.
 #pragma unroll
     for (int a = 0; a < RA; a++) {
         if (TI * a >= n) break;
 #pragma unroll
         for (int b = 0; b < RB; b++) {
-            const int r = li + TI * a, c = lj + TJ * b;
-            if (r < n && c < n) L.stage[li * n + L.qinv[c]] = P[a][b];
+            if (b >= a) {
+                const int r = li + TI * a, c = lj + TJ * b;
+                if (r < n && c < n) L.stage[li * n + L.qinv[c]] = P[a][b];
+            } else {
+                const int r = lj + TI * a, c = li + TI * b;
+                if (r < n && c < n) L.stage[lj * n + L.qinv[c]] = make_double2(-P[b][a].x, P[b][a].y);
+            }
         }
         __syncthreads();
         epi_rows<NT, EBL>(A, L, E, TI * a, min(TI, n - TI * a), iw, tid);

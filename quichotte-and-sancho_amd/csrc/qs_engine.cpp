@@ -280,7 +280,7 @@ int fill_mode() {
 
 int fill_new(qs_handle *h, void *p, size_t bytes) {
     const int m = fill_mode();
-    if (m) HIPCHK(h, hipMemset(p, m == 1 ? 0xff : 0, bytes));
+    if (m) HIPCHK(h, hipMemsetAsync(p, m == 1 ? 0xff : 0, bytes, h->stream));
     return QS_OK;
 }
 
@@ -303,19 +303,27 @@ int dalloc(qs_handle *h, T **ptr, size_t count) {
     return QS_OK;
 }
 
+// Every write into a handle's buffers goes through the handle's stream (created
+// non-blocking), so each is ordered before the kernels that read it; a copy from a
+// host vector is waited for before the vector can go away.
+int copy_in(qs_handle *h, void *dst, const void *src, size_t bytes) {
+    if (!bytes) return QS_OK;
+    HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return QS_OK;
+}
+
 template <class T>
 int upload(qs_handle *h, T **ptr, const std::vector<T> &v) {
     int rc = dalloc(h, ptr, v.size());
     if (rc) return rc;
-    if (!v.empty()) HIPCHK(h, hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return QS_OK;
+    return copy_in(h, *ptr, v.data(), v.size() * sizeof(T));
 }
 
 int upload_z(qs_handle *h, double2 **ptr, const std::vector<zd> &v) {
     int rc = dalloc(h, ptr, v.size());
     if (rc) return rc;
-    if (!v.empty()) HIPCHK(h, hipMemcpy(*ptr, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice));
-    return QS_OK;
+    return copy_in(h, *ptr, v.data(), v.size() * sizeof(double2));
 }
 
 inline int su_idx(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
@@ -410,9 +418,9 @@ int setup_device(qs_handle *h) {
     if ((rc = upload(h, &h->d_atom_norb, h->atom_norb))) return rc;
     if ((rc = upload(h, &h->d_atom_off, h->atom_off))) return rc;
     if ((rc = dalloc(h, &h->d_status, 1))) return rc;
-    HIPCHK(h, hipMemset(h->d_status, 0, sizeof(int)));
+    HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(int), h->stream));
     if ((rc = dalloc(h, &h->d_sig1, (size_t)4 * h->nsites))) return rc;
-    HIPCHK(h, hipMemset(h->d_sig1, 0, sizeof(double2) * 4 * h->nsites));
+    HIPCHK(h, hipMemsetAsync(h->d_sig1, 0, sizeof(double2) * 4 * h->nsites, h->stream));
     if ((rc = dalloc(h, &h->d_onsL, (size_t)nw * 4 * h->nsites))) return rc;
     if ((rc = dalloc(h, &h->d_dosd, (size_t)nw * n))) return rc;
     if ((rc = dalloc(h, &h->d_trUD, (size_t)nw * 2))) return rc;
@@ -520,8 +528,8 @@ int setup_device(qs_handle *h) {
         if ((rc = dalloc(h, &h->d_F, nv))) return rc;
         if ((rc = dalloc(h, &h->d_Fold, nv))) return rc;
         if ((rc = dalloc(h, &h->d_xold, nv))) return rc;
-        HIPCHK(h, hipMemset(h->d_Fold, 0, nv * sizeof(double)));
-        HIPCHK(h, hipMemset(h->d_xold, 0, nv * sizeof(double)));
+        HIPCHK(h, hipMemsetAsync(h->d_Fold, 0, nv * sizeof(double), h->stream));
+        HIPCHK(h, hipMemsetAsync(h->d_xold, 0, nv * sizeof(double), h->stream));
         const int nhist = h->p.nhist;
         if (nhist > 0) {
             if (h->p.single_history) {
@@ -1802,10 +1810,11 @@ int qs_reset_problem(qs_handle *h, const qs_problem *p) {
     h->p = *p;
     h->H.swap(t.H); h->GamL.swap(t.GamL); h->GamR.swap(t.GamR); h->U.swap(t.U);
     h->Hemb.swap(t.Hemb); h->GamL_s.swap(t.GamL_s); h->GamR_s.swap(t.GamR_s); h->GamL_cl.swap(t.GamL_cl);
-    HIPCHK(h, hipMemcpy(h->d_Hemb, h->Hemb.data(), h->Hemb.size() * sizeof(double2), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_gamL_s, h->GamL_s.data(), h->GamL_s.size() * sizeof(double2), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_gamR_s, h->GamR_s.data(), h->GamR_s.size() * sizeof(double2), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_gamL_cl, h->GamL_cl.data(), h->GamL_cl.size() * sizeof(double2), hipMemcpyHostToDevice));
+    int rc;
+    if ((rc = copy_in(h, h->d_Hemb, h->Hemb.data(), h->Hemb.size() * sizeof(double2)))) return rc;
+    if ((rc = copy_in(h, h->d_gamL_s, h->GamL_s.data(), h->GamL_s.size() * sizeof(double2)))) return rc;
+    if ((rc = copy_in(h, h->d_gamR_s, h->GamR_s.data(), h->GamR_s.size() * sizeof(double2)))) return rc;
+    if ((rc = copy_in(h, h->d_gamL_cl, h->GamL_cl.data(), h->GamL_cl.size() * sizeof(double2)))) return rc;
     if (h->order == 2) {
         std::vector<double> uact(h->nAs);
         for (int ia = 0; ia < h->nAs; ia++) uact[ia] = h->U[2 * h->asite[ia]];
@@ -1813,8 +1822,8 @@ int qs_reset_problem(qs_handle *h, const qs_problem *p) {
         std::vector<double> upair((size_t)h->nAs * h->nAs);
         for (int ia = 0; ia < h->nAs; ia++)
             for (int ja = 0; ja < h->nAs; ja++) upair[(size_t)ia * h->nAs + ja] = uact[ia] * uact[ja] * pp * pp / h->Nf;
-        HIPCHK(h, hipMemcpy(h->d_U_act, uact.data(), uact.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(h, hipMemcpy(h->d_Upair, upair.data(), upair.size() * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = copy_in(h, h->d_U_act, uact.data(), uact.size() * sizeof(double)))) return rc;
+        if ((rc = copy_in(h, h->d_Upair, upair.data(), upair.size() * sizeof(double)))) return rc;
     }
     h->x_valid = false;
     h->sigma_valid = false;

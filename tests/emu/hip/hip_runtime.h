@@ -65,7 +65,7 @@ inline double2 make_double2(double a, double b) { return double2{a, b}; }
 inline float2 make_float2(float a, float b) { return float2{a, b}; }
 
 typedef int hipError_t;
-enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorLaunchFailure = 719 };
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorNotSupported = 801, hipErrorLaunchFailure = 719 };
 typedef void *hipStream_t;
 enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
 inline hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
@@ -297,30 +297,36 @@ template <class... P, class... A>
 hipError_t launch(void (*k)(P...), dim3 grid, dim3 block, size_t lds_bytes, hipStream_t, A... args) {
     const int nt = (int)(block.x * block.y * block.z);
     if (nt % 64 || lds_bytes > 160 * 1024) return last_error = hipErrorInvalidValue;
-    for (unsigned bz = 0; bz < grid.z; bz++)
-    for (unsigned by = 0; by < grid.y; by++)
-    for (unsigned bx = 0; bx < grid.x; bx++) {
-        Block blk;
-        blk.bar.n = nt;
-        blk.waves = std::vector<Wave>(nt / 64);
-        for (auto &w : blk.waves) w.bar.n = 64;
-        const size_t sz = (lds_bytes + 15) & ~(size_t)15;
-        blk.lds = (char *)aligned_alloc(16, sz ? sz : 16);
-        memset(blk.lds, 0xff, sz);
-        std::vector<std::thread> th;
-        th.reserve(nt);
-        for (int t = 0; t < nt; t++)
-            th.emplace_back([&, t] {
-                cur = &blk;
-                emu_tid = dim3(t);
-                emu_bid = dim3(bx, by, bz);
-                emu_bdim = block;
-                emu_gdim = grid;
+    // one host thread per GPU thread for the whole launch; the workgroups run one
+    // after another (x fastest), each on freshly NaN-filled LDS, between two
+    // barriers over all threads
+    Block blk;
+    blk.bar.n = nt;
+    blk.waves = std::vector<Wave>(nt / 64);
+    for (auto &w : blk.waves) w.bar.n = 64;
+    const size_t sz = (lds_bytes + 15) & ~(size_t)15;
+    blk.lds = (char *)aligned_alloc(16, sz ? sz : 16);
+    Barrier between;
+    between.n = nt;
+    const unsigned nblocks = grid.x * grid.y * grid.z;
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            cur = &blk;
+            emu_tid = dim3(t);
+            emu_bdim = block;
+            emu_gdim = grid;
+            for (unsigned b = 0; b < nblocks; b++) {
+                if (t == 0) memset(blk.lds, 0xff, sz);
+                between.wait();
+                emu_bid = dim3(b % grid.x, (b / grid.x) % grid.y, b / (grid.x * grid.y));
                 k(args...);
-            });
-        for (auto &t : th) t.join();
-        free(blk.lds);
-    }
+                between.wait();
+            }
+        });
+    for (auto &t : th) t.join();
+    free(blk.lds);
     return hipSuccess;
 }
 }  // namespace emu

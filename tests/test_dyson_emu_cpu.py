@@ -86,7 +86,15 @@ def make_problem(path, n, count, mode, use_sigma, seed=1, V=0.5):
         np.array([0, 1], np.int32).tofile(f)
         (0.1 * (rng.uniform(-1, 1, 4 * ns) + 1j * rng.uniform(-1, 1, 4 * ns))).tofile(f)
         if use_sigma:
-            (0.05 * (rng.standard_normal(count * sig_ld) + 1j * rng.standard_normal(count * sig_ld))).tofile(f)
+            sig = 0.05 * (rng.standard_normal((count, sig_ld)) + 1j * rng.standard_normal((count, sig_ld)))
+            # Sigma^< is anti-Hermitian; its site-diagonal 2x2 blocks (the only lesser
+            # blocks stored whole) are made so, as the physics has them
+            for I in range(ns):
+                b = 12 * (I * ns - I * (I - 1) // 2) + 4
+                blk = sig[:, b:b + 4].reshape(count, 2, 2)
+                blk = (blk - np.conj(np.transpose(blk, (0, 2, 1)))) / 2
+                sig[:, b:b + 4] = blk.reshape(count, 4)
+            sig.tofile(f)
         if mode == M_SCF2:
             rng.standard_normal(count * 3 * n * n).tofile(f)
             rng.standard_normal(count * 3 * n * n).tofile(f)
@@ -170,7 +178,7 @@ def test_k_dyson_shapes_address_sanitizer(builds, tmp_path, n, shapes, mode):
             compare(outs[0], o, ["partials"], 1e-13)
 
 
-@pytest.mark.parametrize("n", [16, 44, 80, 94])
+@pytest.mark.parametrize("n", [16, 44, 48, 80, 94])
 @pytest.mark.parametrize("mode", [M_INIT2, M_SCF2, M_FINAL])
 def test_small_lds_kernel_address_sanitizer(builds, tmp_path, n, mode):
     """k_dyson_w (8x8 tile at 16 orbitals, 16x16 above) against numpy and k_dyson, and
@@ -201,7 +209,9 @@ def test_small_lds_kernel_address_sanitizer(builds, tmp_path, n, mode):
                                             ("k_dyson_w", 80, ("wave", "0", "0")),
                                             ("k_dyson_w", 80, ("split", "0", "0")),
                                             ("k_dyson_w", 80, ("mfsplit", "0", "0")),
-                                            ("k_dyson_w", 16, ("wave", "0", "0"))])
+                                            ("k_dyson_w", 16, ("wave", "0", "0")),
+                                            ("k_dyson_w", 48, ("wave", "0", "0")),
+                                            ("k_dyson_w", 48, ("split", "0", "0"))])
 @pytest.mark.parametrize("mode", [M_SCF2, M_FINAL])
 def test_no_unordered_lds_access_thread_sanitizer(builds, tmp_path, kernel, n, shape, mode):
     exes, _ = builds

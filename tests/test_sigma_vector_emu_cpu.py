@@ -1,0 +1,97 @@
+"""CPU builds of the self-energy, gather/update, streaming and control kernels
+under the host sanitizers (no GPU).
+
+As for the Dyson kernels (test_dyson_emu_cpu.py), `tests/emu/hip/hip_runtime.h`
+runs every GPU thread of a workgroup as a host thread with the kernels' barriers
+as barriers, AddressSanitizer checks every access against the buffers and the
+launch's exact LDS size, and ThreadSanitizer checks that every pair of accesses
+of different threads to one LDS array is ordered by a barrier.  The drivers
+(tests/emu/sigma2_emu.cpp, tests/emu/vector_emu.cpp) run the launches of one SCF
+step on random inputs and check what has a plain host form against it: the two
+forms of the gather agree bit for bit, the fused Broyden update gives the plain
+update's x_new, its dx and the gather of x_new to the bit, the history pass
+gives the same bits from dx as from x and xold, and the column sums, Gram and
+projection sums, equal-time G^< and Sigma1 match host loops.  The self-energy
+passes run at grid sizes whose FFT splits use each LDS layout of the kernels
+(N2 = 8, 16, 128 and 256; N1 = 16, 128 and 256).
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "quichotte-and-sancho_amd", "csrc")
+EMU = os.path.join(ROOT, "tests", "emu")
+LDS_DECL = "extern __shared__ __align__(16) char lds_raw[];"
+
+pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ is needed for the CPU builds")
+
+
+@pytest.fixture(scope="module")
+def builds(tmp_path_factory):
+    d = tmp_path_factory.mktemp("emu_sv")
+    text = open(os.path.join(CSRC, "qs_sigma2.hip")).read()
+    assert text.count(LDS_DECL) >= 4
+    s2 = d / "qs_sigma2_cpu.hip"
+    s2.write_text(text.replace(LDS_DECL, "char *lds_raw = emu::lds();"))
+    procs, exes = [], {}
+    for san in ("address", "thread"):
+        base = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=" + san, "-fno-omit-frame-pointer", "-I", EMU, "-I", CSRC,
+                "-pthread"]
+        exes[("sigma2", san)] = str(d / ("sigma2_%s" % san))
+        exes[("vector", san)] = str(d / ("vector_%s" % san))
+        cmds = [base + ['-DQS_SIGMA2_SOURCE="%s"' % s2, os.path.join(EMU, "sigma2_emu.cpp"), "-o",
+                        exes[("sigma2", san)]],
+                base + ['-DQS_VECTOR_SOURCE="%s"' % os.path.join(CSRC, "qs_vector.hip"),
+                        '-DQS_CONTROL_SOURCE="%s"' % os.path.join(CSRC, "qs_control.hip"),
+                        os.path.join(EMU, "vector_emu.cpp"), "-o", exes[("vector", san)]]]
+        for cmd in cmds:
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    for cmd, p in procs:
+        out, _ = p.communicate(timeout=600)
+        assert p.returncode == 0, "CPU build failed: %s\n%s" % (" ".join(cmd), out[-4000:])
+    return exes
+
+
+def run(exe, *args, timeout=900):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="halt_on_error=1 report_signal_unsafe=0")
+    p = subprocess.run([exe] + [str(a) for a in args], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=timeout, env=env)
+    assert p.returncode == 0, "%s %s: exit %d\n%s" % (os.path.basename(exe), args, p.returncode, p.stdout[-6000:])
+    assert "WARNING: ThreadSanitizer" not in p.stdout, p.stdout[-6000:]
+
+
+def sig_out(path, nA, nw):
+    nAs = nA // 2
+    ncol = (12 * (nAs * (nAs + 1) // 2) + 7) // 8 * 8
+    raw = np.fromfile(path, dtype=np.uint8)
+    nsig = nw * ncol * 16
+    return raw[:nsig], raw[nsig:]
+
+
+# (nA, nw): Nf = 128 (N1 16, N2 8), 256 (16 x 16), 2^14 (128 x 128), 2^16 (256 x 256);
+# the long grids run the self-energy passes only ("fft"), on the host's gather
+@pytest.mark.parametrize("nA,nw,fft", [(6, 100, False), (12, 100, False), (4, 32768, True)])
+def test_sigma2_step_address_sanitizer(builds, tmp_path, nA, nw, fft):
+    run(builds[("sigma2", "address")], nA, nw, nA + nw, tmp_path / "a.bin", *(["fft"] if fft else []))
+
+
+@pytest.mark.parametrize("nA,nw,fft", [(8, 64, False), (6, 100, False), (4, 8192, True)])
+def test_sigma2_step_thread_sanitizer_and_same_bits(builds, tmp_path, nA, nw, fft):
+    """No unordered LDS access; the TSan and ASan builds give the same bits."""
+    extra = ["fft"] if fft else []
+    run(builds[("sigma2", "thread")], nA, nw, 7, tmp_path / "t.bin", *extra)
+    run(builds[("sigma2", "address")], nA, nw, 7, tmp_path / "a.bin", *extra)
+    ts, tx = sig_out(tmp_path / "t.bin", nA, nw)
+    as_, ax = sig_out(tmp_path / "a.bin", nA, nw)
+    assert np.array_equal(ts, as_) and np.array_equal(tx, ax)
+
+
+@pytest.mark.parametrize("san", ["address", "thread"])
+def test_vector_and_control_kernels(builds, san):
+    """Several device-controlled steps (the history ring of 3 wraps) at 4 active
+    spin-orbitals on 24 energies."""
+    run(builds[("vector", san)], 4, 24, 3, 11)
