@@ -112,7 +112,15 @@ __global__ __launch_bounds__(NT) void k_big_pack(DysonArgs A, int iwc, int np, c
     const int n = A.norb, nA = A.nA, nA2 = nA * nA, nCL = A.nCL, iw = iwc + blockIdx.x, tid = threadIdx.x;
     const CT *gr = Gr + (size_t)blockIdx.x * np * np, *gl = Gl + (size_t)blockIdx.x * np * np;
     auto GR = [&](int i, int j) { return get(gr + (size_t)i * np + j); };
-    auto GL = [&](int i, int j) { return get(gl + (size_t)i * np + j); };
+    // G^< is anti-Hermitian and its product forms only the tiles on and above the
+    // diagonal: an element of a tile below it is -conj of its mirror
+    auto GL = [&](int i, int j) {
+        if (i / QS_CGEMM_TILE > j / QS_CGEMM_TILE) {
+            const double2 v = get(gl + (size_t)j * np + i);
+            return make_double2(-v.x, v.y);
+        }
+        return get(gl + (size_t)i * np + j);
+    };
     const double w = A.w_init + (double)(iw + 1) * A.delta;
     const double fL = qs_fermi(w, A.V, A.mu, A.beta);
     if (A.mode == QS_M_INIT2 || A.mode == QS_M_SCF2) {
@@ -284,9 +292,10 @@ hipError_t qs_launch_dyson_big(const DysonArgs &a, int count, hipStream_t s) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
         // G^r into T (A is left as scratch)
         if ((e = qs_launch_gj_inv(np, cnt, bw->A, bw->T, a.status, f32, s)) != hipSuccess) return e;
-        // T1 = G^r S'' into A;  G^< = T1 G^r^dagger into S
+        // T1 = G^r S'' into A;  G^< = T1 G^r^dagger into S, the tiles on and above the
+        // diagonal only (G^< = G^r S'' G^r^dagger is anti-Hermitian, as S'' is)
         if ((e = qs_launch_cgemm(np, cnt, bw->T, bw->S, bw->A, false, f32, s)) != hipSuccess) return e;
-        if ((e = qs_launch_cgemm(np, cnt, bw->A, bw->T, bw->S, true, f32, s)) != hipSuccess) return e;
+        if ((e = qs_launch_cgemm(np, cnt, bw->A, bw->T, bw->S, true, f32, s, true)) != hipSuccess) return e;
         if (f32)
             hipLaunchKernelGGL(k_big_pack<float2>, dim3(cnt), dim3(NT), 0, s, a, iwc, np, (const float2 *)bw->T,
                                (const float2 *)bw->S);

@@ -299,19 +299,26 @@ __global__ __launch_bounds__(GJ_NT) void k_gj_inv(int np, typename Cx<R>::t *__r
 // ---------------------------------------------------------------------------
 // C = A op(B), row-major np x np, one 64 x 64 tile of C per workgroup
 // ---------------------------------------------------------------------------
-constexpr int GM_T = 64, GM_K = 16, GM_LD = GM_K + 1;
+constexpr int GM_T = QS_CGEMM_TILE, GM_K = 16, GM_LD = GM_K + 1;
 
+// upper: blockIdx.x numbers the nt (nt + 1) / 2 tiles (ti, tj >= ti) row by row
 template <typename R, bool CONJB>
 __global__ __launch_bounds__(256) void k_cgemm(int np, const typename Cx<R>::t *__restrict__ Ab,
                                                const typename Cx<R>::t *__restrict__ Bb,
-                                               typename Cx<R>::t *__restrict__ Cb) {
+                                               typename Cx<R>::t *__restrict__ Cb, int upper) {
     using C = typename Cx<R>::t;
     using MF = Mfma<R>;
     using v4 = typename MF::v4;
     __shared__ C As[GM_T * GM_LD];      // As[i][k]
     __shared__ C Bs[GM_K * GM_T];       // Bs[k][j] = op(B)[k][j]
     const int nt = (np + GM_T - 1) / GM_T;
-    const int ti = blockIdx.x / nt, tj = blockIdx.x - ti * nt;
+    int ti = blockIdx.x / nt, tj = blockIdx.x - ti * nt;
+    if (upper) {
+        int r = 0, rest = blockIdx.x;
+        while (rest >= nt - r) { rest -= nt - r; r++; }
+        ti = r;
+        tj = r + rest;
+    }
     const int i0 = ti * GM_T, j0 = tj * GM_T;
     const size_t mo = (size_t)blockIdx.y * np * np;
     const C *A = Ab + mo, *B = Bb + mo;
@@ -405,25 +412,25 @@ hipError_t qs_launch_gj_inv(int np, int count, void *M, void *G, int *status, bo
 }
 
 hipError_t qs_launch_cgemm(int np, int count, const void *A, const void *B, void *Cm, bool conjB, bool fp32,
-                           hipStream_t s) {
+                           hipStream_t s, bool upper) {
     if (count <= 0) return hipSuccess;
     if (np <= 0 || (np & 15)) return hipErrorInvalidValue;
-    const int nt = (np + GM_T - 1) / GM_T;
-    const dim3 grid(nt * nt, count);
+    const int nt = (np + GM_T - 1) / GM_T, up = upper ? 1 : 0;
+    const dim3 grid(upper ? nt * (nt + 1) / 2 : nt * nt, count);
     if (fp32) {
         if (conjB)
             hipLaunchKernelGGL((k_cgemm<float, true>), grid, dim3(256), 0, s, np, (const float2 *)A, (const float2 *)B,
-                               (float2 *)Cm);
+                               (float2 *)Cm, up);
         else
             hipLaunchKernelGGL((k_cgemm<float, false>), grid, dim3(256), 0, s, np, (const float2 *)A, (const float2 *)B,
-                               (float2 *)Cm);
+                               (float2 *)Cm, up);
     } else {
         if (conjB)
             hipLaunchKernelGGL((k_cgemm<double, true>), grid, dim3(256), 0, s, np, (const double2 *)A,
-                               (const double2 *)B, (double2 *)Cm);
+                               (const double2 *)B, (double2 *)Cm, up);
         else
             hipLaunchKernelGGL((k_cgemm<double, false>), grid, dim3(256), 0, s, np, (const double2 *)A,
-                               (const double2 *)B, (double2 *)Cm);
+                               (const double2 *)B, (double2 *)Cm, up);
     }
     return hipGetLastError();
 }

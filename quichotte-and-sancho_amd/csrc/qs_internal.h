@@ -88,9 +88,12 @@ int qs_gj_max_np();
 size_t qs_gj_lds_bytes(int np, bool fp32);
 // G[b] = M[b]^-1 for count matrices (M is overwritten); bit 0 of *status on an exact zero pivot
 hipError_t qs_launch_gj_inv(int np, int count, void *M, void *G, int *status, bool fp32, hipStream_t s);
-// C[b] = A[b] op(B[b]), op = identity or conjugate transpose
+// C[b] = A[b] op(B[b]), op = identity or conjugate transpose.  upper: only the
+// QS_CGEMM_TILE x QS_CGEMM_TILE tiles of C on and above the diagonal are formed (for
+// a product known to be Hermitian or anti-Hermitian; the tiles below keep what they held)
+constexpr int QS_CGEMM_TILE = 64;
 hipError_t qs_launch_cgemm(int np, int count, const void *A, const void *B, void *C, bool conjB, bool fp32,
-                           hipStream_t s);
+                           hipStream_t s, bool upper = false);
 // small-LDS kernels (qs_dyson_wave.hip): form 0, one fused kernel; 1, the Gauss-Jordan
 // part and the Keldysh part as two kernels through the slab (17..96 orbitals); 2, as 1
 // with the Keldysh part on FP64 MFMA
