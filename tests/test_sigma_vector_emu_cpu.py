@@ -56,12 +56,40 @@ def builds(tmp_path_factory):
     return exes
 
 
-def run(exe, *args, timeout=900):
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="halt_on_error=1 report_signal_unsafe=0")
-    p = subprocess.run([exe] + [str(a) for a in args], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
-                       timeout=timeout, env=env)
-    assert p.returncode == 0, "%s %s: exit %d\n%s" % (os.path.basename(exe), args, p.returncode, p.stdout[-6000:])
-    assert "WARNING: ThreadSanitizer" not in p.stdout, p.stdout[-6000:]
+ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="halt_on_error=1 report_signal_unsafe=0")
+
+# every run of this file: (name, sanitizer, args); they go out at once (each is a
+# single process of host threads), and the tests read their results
+SIGMA2_ASAN = [(8, 64, False), (6, 100, False), (12, 100, False), (4, 8192, True), (4, 32768, True)]
+SIGMA2_TSAN = [(8, 64, False), (6, 100, False), (4, 8192, True)]
+
+
+@pytest.fixture(scope="module")
+def runs(builds, tmp_path_factory):
+    d = tmp_path_factory.mktemp("emu_sv_runs")
+    jobs = {}
+    for nA, nw, fft in SIGMA2_ASAN:
+        jobs[("sigma2", "address", nA, nw)] = [builds[("sigma2", "address")], nA, nw, 7, d / ("a_%d_%d.bin" % (nA, nw))] + (
+            ["fft"] if fft else [])
+    for nA, nw, fft in SIGMA2_TSAN:
+        jobs[("sigma2", "thread", nA, nw)] = [builds[("sigma2", "thread")], nA, nw, 7, d / ("t_%d_%d.bin" % (nA, nw))] + (
+            ["fft"] if fft else [])
+    for san in ("address", "thread"):
+        jobs[("vector", san)] = [builds[("vector", san)], 4, 24, 3, 11]
+    procs = {k: subprocess.Popen([str(a) for a in v], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                                 env=ENV) for k, v in jobs.items()}
+    out = {}
+    for k, p in procs.items():
+        text, _ = p.communicate(timeout=1200)
+        out[k] = (p.returncode, text, jobs[k])
+    return out
+
+
+def check(result):
+    rc, text, args = result
+    name = " ".join(str(a) for a in args[1:])
+    assert rc == 0, "%s %s: exit %d\n%s" % (os.path.basename(str(args[0])), name, rc, text[-6000:])
+    assert "WARNING: ThreadSanitizer" not in text, text[-6000:]
 
 
 def sig_out(path, nA, nw):
@@ -74,24 +102,23 @@ def sig_out(path, nA, nw):
 
 # (nA, nw): Nf = 128 (N1 16, N2 8), 256 (16 x 16), 2^14 (128 x 128), 2^16 (256 x 256);
 # the long grids run the self-energy passes only ("fft"), on the host's gather
-@pytest.mark.parametrize("nA,nw,fft", [(6, 100, False), (12, 100, False), (4, 32768, True)])
-def test_sigma2_step_address_sanitizer(builds, tmp_path, nA, nw, fft):
-    run(builds[("sigma2", "address")], nA, nw, nA + nw, tmp_path / "a.bin", *(["fft"] if fft else []))
+@pytest.mark.parametrize("nA,nw,fft", SIGMA2_ASAN)
+def test_sigma2_step_address_sanitizer(runs, nA, nw, fft):
+    check(runs[("sigma2", "address", nA, nw)])
 
 
-@pytest.mark.parametrize("nA,nw,fft", [(8, 64, False), (6, 100, False), (4, 8192, True)])
-def test_sigma2_step_thread_sanitizer_and_same_bits(builds, tmp_path, nA, nw, fft):
+@pytest.mark.parametrize("nA,nw,fft", SIGMA2_TSAN)
+def test_sigma2_step_thread_sanitizer_and_same_bits(runs, nA, nw, fft):
     """No unordered LDS access; the TSan and ASan builds give the same bits."""
-    extra = ["fft"] if fft else []
-    run(builds[("sigma2", "thread")], nA, nw, 7, tmp_path / "t.bin", *extra)
-    run(builds[("sigma2", "address")], nA, nw, 7, tmp_path / "a.bin", *extra)
-    ts, tx = sig_out(tmp_path / "t.bin", nA, nw)
-    as_, ax = sig_out(tmp_path / "a.bin", nA, nw)
+    check(runs[("sigma2", "thread", nA, nw)])
+    check(runs[("sigma2", "address", nA, nw)])
+    ts, tx = sig_out(runs[("sigma2", "thread", nA, nw)][2][4], nA, nw)
+    as_, ax = sig_out(runs[("sigma2", "address", nA, nw)][2][4], nA, nw)
     assert np.array_equal(ts, as_) and np.array_equal(tx, ax)
 
 
 @pytest.mark.parametrize("san", ["address", "thread"])
-def test_vector_and_control_kernels(builds, san):
+def test_vector_and_control_kernels(runs, san):
     """Several device-controlled steps (the history ring of 3 wraps) at 4 active
     spin-orbitals on 24 energies."""
-    run(builds[("vector", san)], 4, 24, 3, 11)
+    check(runs[("vector", san)])
