@@ -13,6 +13,9 @@
 //        the Keldysh kernel on MFMA, emulated lane by lane)
 //        dyson_emu <p1.bin> <o1.bin> group|gsplit|gmfsplit <p2.bin> <o2.bin>  (two
 //        problems as the members of one grouped launch: fused, split, split on MFMA)
+//        dyson_emu <problem.bin> <out.bin> big|big32 0 0   (built with QS_EMU_BIG,
+//        qs_dyson_big.hip as the source and qs_gj_mfma.hip linked: the large-Norb
+//        chunked path in FP64, or in complex FP32, its MFMA emulated lane by lane)
 // problem.bin: int32 n, count, mode, use_sigma, nCL; float64 w_init, delta,
 // mu, beta, V; complex128 Hemb (n*n), GamL (n*n), GamR (n*n), GamL_cl
 // (nCL*nCL); int32 cl (nCL); complex128 sig1 (n/2*4); complex128 sig
@@ -27,8 +30,10 @@
 hipError_t qs_launch_dyson_wave(const DysonArgs &, int, int, hipStream_t) { return hipErrorInvalidValue; }
 int qs_dyson_wave_max_norb() { return 0; }
 #endif
+#ifndef QS_EMU_BIG
 hipError_t qs_launch_dyson_big(const DysonArgs &, int, hipStream_t) { return hipErrorInvalidValue; }
 int qs_dyson_big_max_norb() { return 0; }
+#endif
 
 #include <string>
 
@@ -155,7 +160,18 @@ int main(int argc, char **argv) {
     DysonArgs &a = P.a;
     const std::string shape = std::string(argv[3]) + "," + argv[4] + "," + argv[5];
     hipError_t rc = hipErrorInvalidValue;
-#ifdef QS_EMU_WAVE
+#if defined(QS_EMU_BIG)
+    (void)n;
+    if (shape == "big,0,0" || shape == "big32,0,0") {
+        size_t bytes = 0;
+        QsBigWork *bw = qs_big_create(n, count, (size_t)1 << 30, &bytes);
+        if (!bw) return 3;
+        a.big = bw;
+        a.fp32 = shape == "big32,0,0";
+        rc = qs_launch_dyson_big(a, count, nullptr);
+        qs_big_destroy(bw);
+    }
+#elif defined(QS_EMU_WAVE)
     (void)n;
     if (shape == "wave,0,0") rc = qs_launch_dyson_wave(a, count, 0, nullptr);
     else if (shape == "split,0,0") rc = qs_launch_dyson_wave(a, count, 1, nullptr);

@@ -73,6 +73,18 @@ inline hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { ret
 // several work items
 enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+// device memory is host memory (new buffers hold NaN bytes, as the library's own
+// checks of unwritten memory assume nothing)
+inline hipError_t hipMalloc(void **p, size_t n) {
+    *p = aligned_alloc(256, (n + 255) & ~(size_t)255);
+    if (!*p) return hipErrorInvalidValue;
+    memset(*p, 0xff, n);
+    return hipSuccess;
+}
+inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+inline hipError_t hipMemset(void *p, int v, size_t n) { memset(p, v, n); return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline const char *hipGetErrorString(hipError_t) { return "emulated HIP error"; }
 inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
 
 inline int min(int a, int b) { return a < b ? a : b; }
@@ -192,16 +204,20 @@ inline unsigned long long ballot(bool pred) {
 
 // v_mfma_f64_16x16x4f64: D (16 x 16) = C + A (16 x 4) B (4 x 16), lane l holding
 // A[l % 16][l / 16] and B[l / 16][l % 16]; register q of lane l is D[l / 16 + 4 q][l % 16]
-typedef double mf4_emu __attribute__((vector_size(4 * sizeof(double))));
-inline mf4_emu mfma_f64_16x16x4(double a, double b, mf4_emu c) {
-    double A[64], B[64];
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, one 16 x 16 x 4 block: lane l holds
+// A(l % 16, l / 16) and B(l / 16, l % 16); register q of the accumulator holds
+// D(l / 16 + 4 q, l % 16) in the FP64 form and D(4 (l / 16) + q, l % 16) in the FP32
+// form.  V is the kernel's own 4-vector type.
+template <class V, class R>
+inline V mfma_16x16x4(R a, R b, V c, bool f64) {
+    R A[64], B[64];
     gather64(a, A);
     gather64(b, B);
     const int l = (int)emu_tid.x & 63;
-    mf4_emu d = c;
+    V d = c;
     for (int q = 0; q < 4; q++) {
-        const int row = (l >> 4) + 4 * q, col = l & 15;
-        double s = 0.0;
+        const int row = f64 ? (l >> 4) + 4 * q : 4 * (l >> 4) + q, col = l & 15;
+        R s = 0;
         for (int k = 0; k < 4; k++) s += A[16 * k + row] * B[16 * k + col];
         d[q] = c[q] + s;
     }
@@ -259,7 +275,8 @@ inline double __hiloint2double(int hi, int lo) {
 #define __builtin_amdgcn_s_setprio(p) ((void)0)
 #define __builtin_amdgcn_update_dpp(old, src, ctrl, rmask, bmask, bc) emu::update_dpp((old), (src), (ctrl), (rmask), (bmask), (bc))
 #define __ballot(p) emu::ballot(p)
-#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, cbsz, abid, blgp) emu::mfma_f64_16x16x4((a), (b), (c))
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, cbsz, abid, blgp) emu::mfma_16x16x4((double)(a), (double)(b), (c), true)
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, cbsz, abid, blgp) emu::mfma_16x16x4((float)(a), (float)(b), (c), false)
 // a wavefront-scope fence: the lanes of a wavefront meet, which orders their
 // LDS accesses as the hardware's in-order LDS queue does
 #define __builtin_amdgcn_fence(order, scope) emu::wave_fence(scope)

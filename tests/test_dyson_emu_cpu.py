@@ -12,6 +12,9 @@ anything unwritten shows in the outputs.  The outputs are compared with numpy
 (init2: G^r and G^< of one energy, src/scf.f90:315-355) and between launch
 shapes, which compute the same products in the same order.
 
+The large-Norb path (chunked build, blocked Gauss-Jordan and products on MFMA,
+packing) is built with clang++ and checked against numpy and the register kernel.
+
 k_dyson<16, 5, 5> (256 threads at 65..80 orbitals) is the shape switched off
 in the library after a wrong result and a memory fault on the GPU; these
 builds are how its source was checked for a race or an out-of-range access.
@@ -255,3 +258,77 @@ def test_mfma_keldysh_odd_energy_count(builds, tmp_path, n, count):
     run(exe, tmp_path / "p.bin", tmp_path / "m.bin", ("mfsplit", "0", "0"))
     w, m = read_out(tmp_path / "w.bin", n, count), read_out(tmp_path / "m.bin", n, count)
     compare(w, m, written(w, M_SCF2), 1e-12)
+
+
+# ---- the large-Norb path (qs_dyson_big.hip + qs_gj_mfma.hip) ------------------------
+# Its kernels use clang's ext_vector_type, so these builds use clang++ (ROCm's, on
+# the host); the MFMA instructions (FP64 and FP32 16 x 16 x 4) are emulated lane by
+# lane.  The G^< product forms only the 64 x 64 tiles on and above the diagonal and
+# the packing reads a tile below it through its mirror: at 100 and 130 orbitals
+# (padded to 112 and 144: 2 and 3 tile rows) some outputs come from mirrored tiles.
+CLANG = "/opt/rocm/llvm/bin/clang++" if os.path.exists("/opt/rocm/llvm/bin/clang++") else shutil.which("clang++")
+GJ_DECL = "extern __shared__ __align__(16) char smem[];"
+
+
+@pytest.fixture(scope="module")
+def big_builds(tmp_path_factory):
+    if CLANG is None:
+        pytest.skip("clang++ is needed for the large-Norb CPU builds")
+    d = tmp_path_factory.mktemp("emu_big")
+    text = open(os.path.join(CSRC, "qs_gj_mfma.hip")).read()
+    assert text.count(GJ_DECL) == 1
+    gj = d / "qs_gj_mfma_cpu.hip"
+    gj.write_text(text.replace(GJ_DECL, "char *smem = emu::lds();"))
+    exes, procs = {}, []
+    for san in ("address", "thread"):
+        base = [CLANG, "-x", "c++", "-std=c++17", "-O1", "-g", "-fsanitize=" + san, "-fno-omit-frame-pointer",
+                "-I", EMU, "-I", CSRC, "-c"]
+        o1, o2 = str(d / ("gj_%s.o" % san)), str(d / ("drv_%s.o" % san))
+        procs.append((san, [base + [str(gj), "-o", o1],
+                            base + ["-DQS_EMU_BIG", '-DQS_DYSON_SOURCE="%s"' % os.path.join(CSRC, "qs_dyson_big.hip"),
+                                    os.path.join(EMU, "dyson_emu.cpp"), "-o", o2]],
+                      [CLANG, "-fsanitize=" + san, o1, o2, "-o", str(d / ("big_%s" % san)), "-pthread"]))
+        exes[san] = str(d / ("big_%s" % san))
+    running = [(cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+               for _, comp, _ in procs for cmd in comp]
+    for cmd, p in running:
+        out, _ = p.communicate(timeout=600)
+        assert p.returncode == 0, "CPU build failed: %s\n%s" % (" ".join(cmd), out[-4000:])
+    for _, _, link in procs:
+        r = subprocess.run(link, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, "link failed: %s\n%s" % (" ".join(link), r.stderr[-4000:])
+    return exes
+
+
+@pytest.mark.parametrize("n", [44, 100, 130])
+def test_large_norb_path_init2_against_numpy(big_builds, tmp_path, n):
+    """G^r and G^< of two energies, FP64, to 1e-12 of numpy; the complex-FP32 form to 1e-4."""
+    P = make_problem(tmp_path / "p.bin", n, 2, M_INIT2, use_sigma=0, seed=n)
+    run(big_builds["address"], tmp_path / "p.bin", tmp_path / "b.bin", ("big", "0", "0"))
+    run(big_builds["address"], tmp_path / "p.bin", tmp_path / "f.bin", ("big32", "0", "0"))
+    ref = init2_reference(P, 2)
+    b, f = read_out(tmp_path / "b.bin", n, 2), read_out(tmp_path / "f.bin", n, 2)
+    assert np.max(np.abs(b["x"] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(f["x"] - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("mode", [M_SCF2, M_FINAL])
+def test_large_norb_path_against_the_register_kernel(builds, big_builds, tmp_path, mode):
+    """At 44 orbitals (with Sigma2) the chunked path gives the register kernel's
+    outputs to rounding in every written output."""
+    n = 44
+    make_problem(tmp_path / "p.bin", n, 2, mode, use_sigma=1, seed=3)
+    exes, _ = builds
+    run(exes[("k_dyson_w", "address")], tmp_path / "p.bin", tmp_path / "w.bin", ("wave", "0", "0"))
+    run(big_builds["address"], tmp_path / "p.bin", tmp_path / "b.bin", ("big", "0", "0"))
+    w, b = read_out(tmp_path / "w.bin", n, 2), read_out(tmp_path / "b.bin", n, 2)
+    compare(w, b, written(w, mode), 1e-12)
+
+
+def test_large_norb_path_thread_sanitizer(big_builds, tmp_path):
+    n = 100
+    make_problem(tmp_path / "p.bin", n, 1, M_SCF2, use_sigma=1, seed=5)
+    run(big_builds["thread"], tmp_path / "p.bin", tmp_path / "t.bin", ("big", "0", "0"))
+    o = read_out(tmp_path / "t.bin", n, 1)
+    for k in written(o, M_SCF2):
+        assert not np.isnan(o[k]).any(), k
