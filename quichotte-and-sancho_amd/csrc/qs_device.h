@@ -99,6 +99,32 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t k) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The same over lanes 0..31 only (lanes 32..63 must hold 0), on every lane: four
+// DPP shifts within rows of 16, then row broadcast 15, then lane 31.  The shifts
+// fill from outside their row with 0 (bound_ctrl), the identity of the max, so
+// they need no copy of the old value.
+__device__ __forceinline__ uint64_t wave_max_u64_lo32(uint64_t k) {
+    auto step = [&](auto ctrl_c, auto rmask_c, auto bc_c) {
+        constexpr int ctrl = decltype(ctrl_c)::value, rmask = decltype(rmask_c)::value;
+        constexpr bool bc = decltype(bc_c)::value;
+        const int lo = (int)(uint32_t)k, hi = (int)(uint32_t)(k >> 32);
+        const uint32_t olo = (uint32_t)__builtin_amdgcn_update_dpp(bc ? 0 : lo, lo, ctrl, rmask, 0xf, bc);
+        const uint32_t ohi = (uint32_t)__builtin_amdgcn_update_dpp(bc ? 0 : hi, hi, ctrl, rmask, 0xf, bc);
+        const uint64_t o = ((uint64_t)ohi << 32) | olo;
+        k = o > k ? o : k;
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    step(std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xf>{}, T{});   // row_shr:1
+    step(std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xf>{}, T{});   // row_shr:2
+    step(std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xf>{}, T{});   // row_shr:4
+    step(std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xf>{}, T{});   // row_shr:8
+    step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xa>{}, F{});   // row_bcast:15
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, 31);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), 31);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Fermi-Dirac as the reference evaluates it (src/green_functions.f90:74-84):
 // f = 1/(exp(min(hbar*(w - mu + V/hbar)*beta, 700)) + 1), hbar = 1.
 __device__ __host__ __forceinline__ double qs_fermi(double w, double V, double mu, double beta) {

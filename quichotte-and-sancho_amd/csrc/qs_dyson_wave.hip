@@ -505,7 +505,8 @@ __device__ __forceinline__ void dyson_w_body(const DysonArgs &A, const int blk) 
                     key = v != v ? 1 : (uint64_t)__double_as_longlong(v) + 2;
                 }
             }
-            const uint64_t wmax = wave_max_u64(key);
+            static_assert(LW * RA <= 32, "the candidate keys sit in lanes 0..31");
+            const uint64_t wmax = wave_max_u64_lo32(key);
             int rw = -1;
             double best = -1.0;
             if (wmax != 0) {
