@@ -81,10 +81,49 @@ inline hipError_t hipMalloc(void **p, size_t n) {
     memset(*p, 0xff, n);
     return hipSuccess;
 }
+template <class T>
+inline hipError_t hipMalloc(T **p, size_t n) {
+    return hipMalloc((void **)p, n);
+}
 inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
 inline hipError_t hipMemset(void *p, int v, size_t n) { memset(p, v, n); return hipSuccess; }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "emulated HIP error"; }
+// streams and events: launches run to completion when made, so every stream is
+// always idle; copies are plain memcpy
+enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2,
+                     hipMemcpyDeviceToDevice = 3, hipMemcpyDefault = 4 };
+typedef void *hipEvent_t;
+constexpr unsigned hipStreamNonBlocking = 1;
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) {
+    static char tag;
+    *s = &tag;
+    return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipMemcpy(void *d, const void *src, size_t n, hipMemcpyKind) { memcpy(d, src, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *src, size_t n, hipMemcpyKind, hipStream_t) {
+    memcpy(d, src, n);
+    return hipSuccess;
+}
+inline hipError_t hipEventCreate(hipEvent_t *e) {
+    static char tag;
+    *e = &tag;
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+// a file-scope __device__ array is a host array: its address is the symbol's own
+#define HIP_SYMBOL(x) (x)
+template <class T>
+inline hipError_t hipGetSymbolAddress(void **p, const T &sym) {
+    *p = (void *)&sym;
+    return hipSuccess;
+}
 inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
 
 inline int min(int a, int b) { return a < b ? a : b; }

@@ -14,6 +14,13 @@ gives the same bits from dx as from x and xold, and the column sums, Gram and
 projection sums, equal-time G^< and Sigma1 match host loops.  The self-energy
 passes run at grid sizes whose FFT splits use each LDS layout of the kernels
 (N2 = 8, 16, 128 and 256; N1 = 16, 128 and 256).
+
+SANCHO's self-energy kernels (qs_sancho.hip, tests/emu/sancho_emu.cpp) run through
+the library's own entry points on physical random inputs (G^a = G^r^dagger, G^<
+anti-Hermitian) and are compared with the oracle (oracle/sancho_oracle.c,
+SANCHO/src/self_energies.f90:38-207) to 1e-12; the entry point that allocates
+nothing must give the handle's outputs to the bit, a smaller second call must
+match the oracle too, and an oversized call must be refused.
 """
 import os
 import shutil
@@ -21,6 +28,8 @@ import subprocess
 
 import numpy as np
 import pytest
+
+from qs_testlib import sancho_inputs, sancho_oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "quichotte-and-sancho_amd", "csrc")
@@ -48,6 +57,10 @@ def builds(tmp_path_factory):
                 base + ['-DQS_VECTOR_SOURCE="%s"' % os.path.join(CSRC, "qs_vector.hip"),
                         '-DQS_CONTROL_SOURCE="%s"' % os.path.join(CSRC, "qs_control.hip"),
                         os.path.join(EMU, "vector_emu.cpp"), "-o", exes[("vector", san)]]]
+        exes[("sancho", san)] = str(d / ("sancho_%s" % san))
+        cmds.append(base + ["-I", os.path.join(ROOT, "include"),
+                            '-DQS_SANCHO_SOURCE="%s"' % os.path.join(CSRC, "qs_sancho.hip"),
+                            os.path.join(EMU, "sancho_emu.cpp"), "-o", exes[("sancho", san)]])
         for cmd in cmds:
             procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
     for cmd, p in procs:
@@ -62,6 +75,8 @@ ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TSAN_OPTIONS="halt_on_erro
 # single process of host threads), and the tests read their results
 SIGMA2_ASAN = [(8, 64, False), (6, 100, False), (12, 100, False), (4, 8192, True), (4, 32768, True)]
 SIGMA2_TSAN = [(8, 64, False), (6, 100, False), (4, 8192, True)]
+# (sites, energies, order)
+SANCHO_CASES = [(3, 64, 2), (4, 96, 3)]
 
 
 @pytest.fixture(scope="module")
@@ -76,6 +91,18 @@ def runs(builds, tmp_path_factory):
             ["fft"] if fft else [])
     for san in ("address", "thread"):
         jobs[("vector", san)] = [builds[("vector", san)], 4, 24, 3, 11]
+    for na, nw, order in SANCHO_CASES:
+        GR, GL, U, delta = sancho_inputs(na, nw, seed=na)
+        path = d / ("sn_in_%d_%d_%d.bin" % (na, nw, order))
+        with open(path, "wb") as f:
+            np.array([na, nw, order], np.int32).tofile(f)
+            np.array([delta]).tofile(f)
+            np.asarray(U, np.float64).tofile(f)
+            GR.ravel(order="F").astype(np.complex128).tofile(f)
+            GL.ravel(order="F").astype(np.complex128).tofile(f)
+        for san in ("address", "thread"):
+            out = d / ("sn_out_%s_%d_%d_%d.bin" % (san, na, nw, order))
+            jobs[("sancho", san, na, nw, order)] = [builds[("sancho", san)], path, out]
     procs = {k: subprocess.Popen([str(a) for a in v], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                                  env=ENV) for k, v in jobs.items()}
     out = {}
@@ -122,3 +149,22 @@ def test_vector_and_control_kernels(runs, san):
     """Several device-controlled steps (the history ring of 3 wraps) at 4 active
     spin-orbitals on 24 energies."""
     check(runs[("vector", san)])
+
+
+@pytest.mark.parametrize("san", ["address", "thread"])
+@pytest.mark.parametrize("na,nw,order", SANCHO_CASES)
+def test_sancho_self_energies_against_the_oracle(runs, san, na, nw, order):
+    check(runs[("sancho", san, na, nw, order)])
+    out = runs[("sancho", san, na, nw, order)][2][2]
+    GR, GL, U, delta = sancho_inputs(na, nw, seed=na)
+    nout = 6 if order == 3 else 3
+    raw = np.fromfile(out, np.complex128)
+    nm, nw2 = na * na * nw, nw // 2
+    nm2 = na * na * nw2
+    ref = sancho_oracle(GR, GL, U, delta, order)
+    ref2 = sancho_oracle(np.asfortranarray(GR[:, :, :nw2]), np.asfortranarray(GL[:, :, :nw2]), U, delta, order)
+    for i in range(nout):
+        got = raw[i * nm:(i + 1) * nm].reshape((na, na, nw), order="F")
+        assert np.max(np.abs(got - ref[i])) <= 1e-12 * np.max(np.abs(ref[i])), i
+        got2 = raw[nout * nm + i * nm2:nout * nm + (i + 1) * nm2].reshape((na, na, nw2), order="F")
+        assert np.max(np.abs(got2 - ref2[i])) <= 1e-12 * np.max(np.abs(ref2[i])), i
