@@ -288,7 +288,8 @@ def serve_bench(args):
            "config": {"workload": ("helix systems of %s spin-orbitals, grids 2^13..2^16 on (-12,12] capped per size "
                                    "at %s, U~U[0.5,3], V~U[-2,2], fresh starts, tolf 1e-8, nhist 16 FP32 history, "
                                    "Poisson arrivals" % (sizes, {n: "2^%d" % SERVE_SIZES[n][1] for n in sizes})),
-                      "requests_per_rate": args.requests, "max_running": args.serve_streams},
+                      "requests_per_rate": args.requests, "max_running": args.serve_streams,
+                      "fused_group": bool(args.serve_fuse)},
            "rates": {}}
     for rate in rates:
         rng = np.random.default_rng(int(rate))
@@ -319,7 +320,10 @@ def serve_bench(args):
                                        "not converged"), r.n_iter, r.latency, r.device_steps, r.dyson_shape),
                   file=sys.stderr, flush=True)
 
-        srv = serving.Server(max_running=args.serve_streams, on_done=lambda r: progress(r, "mix"))
+        # with --serve-fuse the requests of at most 82 active spin-orbitals are stepped in one group
+        # (shared Dyson launches, joining and leaving between steps); the others run on their own threads
+        srv = serving.Server(max_running=args.serve_streams, fuse=args.serve_fuse,
+                             on_done=lambda r: progress(r, "mix"))
         t0 = time.perf_counter()
         res = srv.run(reqs)
         wall = time.perf_counter() - t0
@@ -339,7 +343,7 @@ def serve_bench(args):
         out["rates"][str(rate)] = {"per_size": per_size, "bias_points_per_s": len(ok) / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
                                    "p99_ms": 1e3 * float(np.percentile(lat, 99)), "converged": len(ok),
                                    "requests": len(res), "wall_s": wall, "handles_reused": srv.reused,
-                                   "errors": errs[:3]}
+                                   "grouped_requests": sum(1 for r in res if r.grouped), "errors": errs[:3]}
         print("bench: serve rate %g/s: %d/%d converged, %.2f points/s, p50 %.0f ms, p99 %.0f ms, %d handles reused"
               % (rate, len(ok), len(res), len(ok) / wall, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99),
                  srv.reused), file=sys.stderr, flush=True)
@@ -663,6 +667,8 @@ def make_parser():
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-sizes", default="16,48,80,160,240", help="spin-orbital counts of the serving stream")
+    ap.add_argument("--serve-fuse", action="store_true",
+                    help="--serve: step the requests of at most 82 active spin-orbitals in one group (fused Dyson launches)")
     ap.add_argument("--serve-plain", action="store_true",
                     help="also serve each rate's stream one request at a time (the plain path) for comparison")
     ap.add_argument("--maxscf", type=int, default=0,

@@ -148,3 +148,25 @@ def test_reused_handle_equals_a_fresh_one(ions, turns, nw, variant, monkeypatch)
     print("reused: %d steps, fresh: %d steps" % (got[0], want[0]))
     assert got[:4] == want[:4]
     assert np.array_equal(got[4], want[4]) and np.array_equal(got[5], want[5])
+
+
+def test_fused_group_serving_equals_alone():
+    """fuse=True: every request of the stream (all of at most 82 active
+    spin-orbitals) is stepped in the server's group, joining as it is admitted;
+    each result is its solve alone, bit for bit."""
+    pkg, serving, reqs, probs = make_requests()
+    alone = []
+    for r in reqs:
+        e = pkg.ScfEngine(r.problem)
+        res = e.solve_scf(r.V, True)
+        alone.append((e.current(), res.n_iter))
+        e.close()
+    srv = serving.Server(max_running=2, fuse=True, max_grouped=6)
+    out = srv.run(reqs)
+    srv.shutdown()
+    assert len(out) == len(reqs)
+    for r, (J, nit) in zip(out, alone):
+        assert r.error == ""
+        assert r.grouped
+        assert r.converged and r.n_iter == nit
+        assert r.J_up == J[0] and r.J_down == J[1]
