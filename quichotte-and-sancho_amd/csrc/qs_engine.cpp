@@ -662,7 +662,8 @@ Sanity diag_sanity(const double *sumImGL, const double *sumImGR, int nb, double 
 }
 
 // Pick the Dyson launch shape for this size: time each variant on a slice of
-// the grid (mode FINAL, which every order runs) and keep the fastest.
+// the grid (mode FINAL, which every order runs) and keep the fastest (the
+// lowest-numbered shape within 3 % of it).
 // QS_DYSON_VARIANT=<k> in the environment forces variant k.
 // The choice is kept for the process, per (orbitals, contact orbitals, slice
 // length): a handle created while other work runs (a serving loop) then takes
@@ -714,10 +715,18 @@ int tune_dyson(qs_handle *h) {
         float ms = 0.f;
         HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
         if (v < QS_DYSON_NSHAPES) h->dyson_tune_ms[v] = ms / 2;
-        if (ms < best) { best = ms; h->dyson_variant = v; }
+        if (ms < best) best = ms;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // the lowest-numbered shape within 3 % of the fastest: near-ties, whose order
+    // timing noise decides, resolve the same way in every process, so that another
+    // run picks the same shape (and the same rounding) for a size more often
+    for (int v = 0; v < nv && v < QS_DYSON_NSHAPES; v++)
+        if (h->dyson_tune_ms[v] > 0.0 && 2 * h->dyson_tune_ms[v] <= 1.03 * best) {
+            h->dyson_variant = v;
+            break;
+        }
     TuneRecord r{h->dyson_variant, {}};
     for (int i = 0; i < QS_DYSON_NSHAPES; i++) r.ms[i] = h->dyson_tune_ms[i];
     g_tune.emplace(key, r);
