@@ -253,14 +253,27 @@ class Server:
                 for req, eng, fp, start, reused in joins:
                     try:
                         g.add(eng, req.V, True)
-                        members[id(eng)] = (req, fp, start, reused)
+                        members[eng] = (req, fp, start, reused)
                     except QsError:
                         # not a group member after all: the solve runs here, alone
                         self._solo(req, eng, fp, start, reused, t0, out)
                 if len(g) == 0:
                     continue
-                for eng, r, st in g.step():
-                    req, fp, start, reused = members.pop(id(eng))
+                try:
+                    finished = g.step()
+                except QsError as e:
+                    # the group step itself failed: every member is answered with the
+                    # error and closed, and a new group takes the next requests
+                    for eng, (req, fp, start, reused) in list(members.items()):
+                        resp = Response(req.rid, float("nan"), float("nan"), False, 0, req.arrival, start,
+                                        time.perf_counter() - t0, fp, "group step: %s" % e)
+                        self._release(req, eng, fp, resp, False, True, out)
+                    members.clear()
+                    g.close()
+                    g = ScfGroup(self.device)
+                    continue
+                for eng, r, st in finished:
+                    req, fp, start, reused = members.pop(eng)
                     if st == 0:
                         ju, jd = eng.current()
                         resp = Response(req.rid, ju, jd, r.converged, r.n_iter, req.arrival, start,

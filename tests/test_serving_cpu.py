@@ -33,7 +33,7 @@ class FakeResult:
         self.converged = converged
 
 
-def make_fakes(tr, fail_norb=None):
+def make_fakes(tr, fail_norb=None, group_fail_step=None):
     class FakeEngine:
         def __init__(self, prob):
             if fail_norb is not None and prob.H.shape[0] == fail_norb:
@@ -78,6 +78,8 @@ def make_fakes(tr, fail_norb=None):
                 tr.live_bytes -= self.fp
 
     class FakeGroup:
+        steps = 0
+
         def __init__(self, device=0):
             self.m = []
 
@@ -95,6 +97,12 @@ def make_fakes(tr, fail_norb=None):
 
         def step(self):
             time.sleep(0.001)
+            FakeGroup.steps += 1
+            if group_fail_step is not None and FakeGroup.steps == group_fail_step:
+                with tr.lock:
+                    tr.members -= len(self.m)
+                self.m = []
+                raise engine_mod.QsError(1, "stand-in group failure")
             done = []
             for item in list(self.m):
                 item[1] -= 1
@@ -132,9 +140,9 @@ class P:
 
 @pytest.fixture
 def fakes(monkeypatch):
-    def install(fail_norb=None, budget_bytes=10 ** 9):
+    def install(fail_norb=None, budget_bytes=10 ** 9, group_fail_step=None):
         tr = Tracker()
-        FE, FG = make_fakes(tr, fail_norb)
+        FE, FG = make_fakes(tr, fail_norb, group_fail_step)
         monkeypatch.setattr(serving, "ScfEngine", FE)
         monkeypatch.setattr(serving, "ScfGroup", FG)
         monkeypatch.setattr(serving, "footprint_bytes", fake_footprint)
@@ -215,3 +223,17 @@ def test_groupable_follows_the_library_limits():
     q = P(16)
     q.order = 1
     assert not serving.groupable(q)
+
+
+def test_a_failing_group_step_answers_its_members_and_serving_goes_on(fakes):
+    tr = fakes(group_fail_step=3)
+    reqs = stream(SIZES, seed=4)
+    srv = serving.Server(max_running=2, fuse=True, max_grouped=4)
+    out = srv.run(reqs)
+    srv.shutdown()
+    assert [r.rid for r in out] == list(range(len(reqs)))
+    failed = [r for r in out if r.error]
+    assert failed and all("stand-in group failure" in r.error for r in failed)
+    assert all(r.converged for r in out if not r.error)
+    assert any(r.grouped and not r.error for r in out)     # later requests went through a new group
+    assert tr.live_bytes == 0
